@@ -1,0 +1,194 @@
+/*
+ * mlff_amd.h -- C ABI of the MI355X (gfx950) PaiNN student force field.
+ *
+ * The reference (atfrank/MLFF-Distiller) is pure Python: its hot path is
+ * StudentForceField.forward / predict_energy_and_forces
+ * (src/mlff_distiller/models/student_model.py:634-795) called per MD step by
+ * StudentForceFieldCalculator.calculate / calculate_batch
+ * (src/mlff_distiller/inference/ase_calculator.py:279-412, 590-817) and by
+ * ASE VelocityVerlet in NVEMDHarness
+ * (src/mlff_distiller/testing/nve_harness.py:173-261).  It has no FFI of its
+ * own; the entry points below are what its model/calculator would bind
+ * through ctypes to replace those calls (the binding is shown in
+ * INTEGRATION.md).
+ *
+ * Conventions
+ *  - Every pointer argument named *_dev is device memory owned by the
+ *    caller.  The library never allocates device memory: all scratch lives
+ *    in one caller-provided workspace whose size mlff_plan_workspace_bytes
+ *    reports.
+ *  - Functions return 0 on success, a negative mlff_status on error; they
+ *    never throw.  mlff_last_error() returns a message for the calling
+ *    thread's last failure.
+ *  - Work is enqueued on the hipStream_t passed in (as void*); no call
+ *    synchronises the device except where stated.
+ *  - Atoms of one structure are contiguous; structure b owns atoms
+ *    [mol_ptr[b], mol_ptr[b+1]).
+ *  - Results are bit-reproducible run to run: every reduction runs in a
+ *    fixed order (no floating-point atomics).
+ */
+#ifndef MLFF_AMD_H
+#define MLFF_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mlff_status {
+  MLFF_OK = 0,
+  MLFF_ERR_INVALID = -1,      /* bad argument (maps to ValueError) */
+  MLFF_ERR_UNSUPPORTED = -2,  /* configuration the kernels do not cover */
+  MLFF_ERR_HIP = -3,          /* HIP runtime error */
+  MLFF_ERR_WORKSPACE = -4,    /* workspace too small / not bound */
+  MLFF_ERR_STATE = -5         /* call order (e.g. MD not bound) */
+};
+
+/* counts[] slots written by the device (int32) */
+enum mlff_count_slot {
+  MLFF_COUNT_EDGES = 0,       /* directed edges found (may exceed capacity) */
+  MLFF_COUNT_PAIRS = 1,       /* unordered pairs found */
+  MLFF_COUNT_OVERFLOW = 2,    /* sticky: non-zero when a capacity was exceeded */
+  MLFF_COUNT_CELL_SMALL = 3,  /* sticky: periodic cell narrower than 2*cutoff */
+  MLFF_COUNT_STEP = 4,        /* MD step counter */
+  MLFF_COUNT_SLOTS = 8
+};
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mlff_version(void);
+
+/* Message of the calling thread's last error ("" if none). */
+const char* mlff_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Weights.  The network parameters are packed into one float32 device blob.
+ * Each slot is a row-major [rows][ld] matrix whose padding is zero; the
+ * Python side fills the blob from the StudentForceField state dict
+ * (student_model.py:585-605 names the tensors).  kind: 0 = copy of the
+ * state-dict tensor, 1 = its transpose.
+ * ------------------------------------------------------------------------- */
+typedef struct mlff_slot {
+  int32_t tensor;   /* mlff_tensor_id */
+  int32_t layer;    /* interaction block, -1 for model-level tensors */
+  int32_t kind;     /* 0 = as stored, 1 = transposed */
+  int32_t rows, cols, ld;
+  int64_t offset;   /* in floats from the blob start */
+} mlff_slot;
+
+enum mlff_tensor_id {
+  MLFF_T_EMBEDDING = 0,   /* embedding.weight [max_z+1, H] */
+  MLFF_T_RBF_CENTERS = 1, /* rbf.centers [R] */
+  MLFF_T_RBF_WIDTHS = 2,  /* rbf.widths [R] */
+  MLFF_T_F1_W = 3,        /* interactions.k.message.rbf_to_scalar.0.weight [H, R] */
+  MLFF_T_F1_B = 4,
+  MLFF_T_F2_W = 5,        /* interactions.k.message.rbf_to_scalar.2.weight [3H, H] */
+  MLFF_T_F2_B = 6,
+  MLFF_T_U1_W = 7,        /* interactions.k.update.update_mlp.0.weight [H, 2H] */
+  MLFF_T_U1_B = 8,
+  MLFF_T_U2_W = 9,        /* interactions.k.update.update_mlp.2.weight [3H, H] */
+  MLFF_T_U2_B = 10,
+  MLFF_T_MIX = 11,        /* interactions.k.update.mixing_matrix [3, 3] */
+  MLFF_T_R1_W = 12,       /* energy_head.0.weight [H/2, H] */
+  MLFF_T_R1_B = 13,
+  MLFF_T_R2_W = 14,       /* energy_head.2.weight [H/4, H/2] */
+  MLFF_T_R2_B = 15,
+  MLFF_T_R3_W = 16,       /* energy_head.4.weight [1, H/4] */
+  MLFF_T_R3_B = 17
+};
+
+/* Fill up to max_slots slot descriptors for a (H, L, R, max_z) student;
+ * *n_slots and *total_floats receive the slot count and blob size. */
+int mlff_weights_layout(int hidden_dim, int num_interactions, int num_rbf, int max_z,
+                        mlff_slot* slots, int max_slots, int* n_slots, int64_t* total_floats);
+
+/* ---------------------------------------------------------------------------
+ * Neighbour list.  Replaces radius_graph / radius_graph_native
+ * (student_model.py:63-109, 165-191): all ordered pairs i != j of the same
+ * structure with ||p_j - p_i|| <= cutoff, the distance rounded exactly as
+ * torch.norm does on the CPU (sqrtf(fmaf(z,z,fmaf(y,y,x*x)))), so the edge
+ * set is bit-identical.  Periodic structures (pbc all true) use the minimum
+ * image; cutoff must be below half the cell's smallest perpendicular width.
+ *
+ * Output is a receiver CSR: row i lists its neighbours j in ascending order
+ * (col_dev[row_ptr[i] .. row_ptr[i+1]) ), which for this symmetric relation
+ * is exactly torch.where's (src, dst) order with src = row, dst = col.
+ * shift_dev (may be NULL) receives the integer lattice shift of each edge:
+ * edge vector p[col] - p[row] + shift @ cell.  counts_dev[MLFF_COUNT_EDGES]
+ * receives the edge count; entries beyond edge_capacity are not written and
+ * MLFF_COUNT_OVERFLOW is set.  workspace_dev must hold
+ * mlff_neighbor_workspace_bytes(n_atoms) bytes.
+ * ------------------------------------------------------------------------- */
+int64_t mlff_neighbor_workspace_bytes(int n_atoms);
+int mlff_neighbor_list(void* stream, const float* pos_dev, int n_atoms, int n_structures,
+                       const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
+                       const float* cells_dev, const uint8_t* pbc_dev, float cutoff,
+                       int32_t* row_ptr_dev, int32_t* col_dev, int32_t* shift_dev,
+                       int32_t edge_capacity, int32_t* counts_dev, void* workspace_dev);
+
+/* ---------------------------------------------------------------------------
+ * Plan: host-side descriptor of one batch shape.  Holds no device memory of
+ * its own; mlff_plan_bind points it at caller buffers.
+ * ------------------------------------------------------------------------- */
+typedef struct mlff_plan mlff_plan;
+
+typedef struct mlff_config {
+  int32_t hidden_dim, num_interactions, num_rbf, max_z;
+  float cutoff;
+  int32_t n_atoms, n_structures;
+  int32_t edge_capacity;      /* directed edges */
+  int32_t pair_capacity;      /* unordered pairs */
+  int32_t any_periodic;       /* non-zero when some structure is periodic */
+  int32_t persistent_blocks;  /* 0 = library default (8 per XCD-aware CU group) */
+} mlff_config;
+
+int mlff_plan_create(mlff_plan** plan, const mlff_config* cfg);
+int mlff_plan_destroy(mlff_plan* plan);
+int64_t mlff_plan_workspace_bytes(const mlff_plan* plan);
+
+/* Bind the workspace and the per-batch constants.  numbers_dev [N] int64,
+ * mol_ptr_dev [B+1], atom_mol_dev [N] (structure index of each atom),
+ * cells_dev [B,3,3] (rows = lattice vectors) and pbc_dev [B,3] may be NULL
+ * for non-periodic batches.  weights_dev is the packed blob. */
+int mlff_plan_bind(mlff_plan* plan, void* workspace_dev, int64_t workspace_bytes,
+                   const float* weights_dev, const int64_t* numbers_dev,
+                   const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
+                   const float* cells_dev, const uint8_t* pbc_dev);
+
+/* Device pointer to the plan's int32 counts[MLFF_COUNT_SLOTS]. */
+int32_t* mlff_plan_counts(mlff_plan* plan);
+
+/* Energy (float64 per structure), forces (-dE/dx, float32 [N,3]) and the
+ * virial dE/d(strain) (float64 [B,9], row-major 3x3) of positions pos_dev
+ * [N,3].  forces_dev / virial_dev may be NULL.  atom_energy_dev (may be
+ * NULL) receives the per-atom readout [N] float32.  Replaces
+ * StudentForceField.forward + torch.autograd.grad (student_model.py:634-795,
+ * ase_calculator.py:708-770); the forces come from the hand-written adjoint. */
+int mlff_energy_forces(mlff_plan* plan, void* stream, const float* pos_dev,
+                       double* energy_dev, float* forces_dev, double* virial_dev,
+                       float* atom_energy_dev);
+
+/* ---------------------------------------------------------------------------
+ * Molecular dynamics.  Velocity Verlet as ASE's VelocityVerlet.step
+ * (called from nve_harness.py:214-235): p += dt/2 F; x += dt p/m; F = F(x);
+ * p += dt/2 F.  State is float64 (as ASE keeps it); the model sees a float32
+ * copy of x.  Every step appends (E_pot, E_kin) per structure to a float64
+ * ledger [ledger_len, B, 2] at row step % ledger_len.
+ * ------------------------------------------------------------------------- */
+int mlff_md_bind(mlff_plan* plan, double* x_dev, double* p_dev, const double* inv_mass_dev,
+                 float* pos32_dev, float* forces_dev, double* energy_dev,
+                 double* ledger_dev, int32_t ledger_len, double dt);
+/* Forces at the current positions (call once before the first step). */
+int mlff_md_init_forces(mlff_plan* plan, void* stream);
+/* Enqueue n_steps steps (eager launches). */
+int mlff_md_run(mlff_plan* plan, void* stream, int n_steps);
+/* Capture steps_per_graph steps into a HIP graph (once), then replay it
+ * n_launches times. */
+int mlff_md_graph_run(mlff_plan* plan, void* stream, int steps_per_graph, int n_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLFF_AMD_H */

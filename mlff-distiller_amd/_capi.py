@@ -1,0 +1,151 @@
+"""ctypes binding of libmlff_amd.so (declarations in include/mlff_amd.h).
+
+torch is imported first on purpose: the library links libamdhip64.so.7,
+and loading it after torch makes the dynamic linker reuse the HIP runtime
+torch already loaded (same SONAME), so there is one runtime per process.
+
+There is no fallback: if the library is missing, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must precede the library load, see above)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libmlff_amd.so")
+
+MLFF_COUNT_EDGES = 0
+MLFF_COUNT_PAIRS = 1
+MLFF_COUNT_OVERFLOW = 2
+MLFF_COUNT_CELL_SMALL = 3
+MLFF_COUNT_STEP = 4
+MLFF_COUNT_SLOTS = 8
+
+T_EMBEDDING, T_RBF_CENTERS, T_RBF_WIDTHS = 0, 1, 2
+T_F1_W, T_F1_B, T_F2_W, T_F2_B = 3, 4, 5, 6
+T_U1_W, T_U1_B, T_U2_W, T_U2_B, T_MIX = 7, 8, 9, 10, 11
+T_R1_W, T_R1_B, T_R2_W, T_R2_B, T_R3_W, T_R3_B = 12, 13, 14, 15, 16, 17
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "HIP error",
+          -4: "workspace", -5: "call order"}
+
+# every symbol include/mlff_amd.h declares
+EXPORTS = (
+    "mlff_version", "mlff_last_error", "mlff_weights_layout", "mlff_neighbor_workspace_bytes",
+    "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
+    "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
+    "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
+)
+
+
+class Slot(ctypes.Structure):
+    _fields_ = [("tensor", ctypes.c_int32), ("layer", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("ld", ctypes.c_int32),
+                ("offset", ctypes.c_int64)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("hidden_dim", ctypes.c_int32), ("num_interactions", ctypes.c_int32),
+                ("num_rbf", ctypes.c_int32), ("max_z", ctypes.c_int32),
+                ("cutoff", ctypes.c_float),
+                ("n_atoms", ctypes.c_int32), ("n_structures", ctypes.c_int32),
+                ("edge_capacity", ctypes.c_int32), ("pair_capacity", ctypes.c_int32),
+                ("any_periodic", ctypes.c_int32), ("persistent_blocks", ctypes.c_int32)]
+
+
+class MlffError(RuntimeError):
+    """A C entry point returned a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({STATUS.get(code, code)}): {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+vp = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+
+
+def _declare(lib):
+    lib.mlff_version.restype = ctypes.c_int
+    lib.mlff_version.argtypes = []
+    lib.mlff_last_error.restype = ctypes.c_char_p
+    lib.mlff_last_error.argtypes = []
+    lib.mlff_weights_layout.restype = ctypes.c_int
+    lib.mlff_weights_layout.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(Slot), ctypes.c_int,
+                                                             ctypes.POINTER(ctypes.c_int),
+                                                             ctypes.POINTER(i64)]
+    lib.mlff_neighbor_workspace_bytes.restype = i64
+    lib.mlff_neighbor_workspace_bytes.argtypes = [ctypes.c_int]
+    lib.mlff_neighbor_list.restype = ctypes.c_int
+    lib.mlff_neighbor_list.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
+                                       ctypes.c_float, vp, vp, vp, i32, vp, vp]
+    lib.mlff_plan_create.restype = ctypes.c_int
+    lib.mlff_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
+    lib.mlff_plan_destroy.restype = ctypes.c_int
+    lib.mlff_plan_destroy.argtypes = [vp]
+    lib.mlff_plan_workspace_bytes.restype = i64
+    lib.mlff_plan_workspace_bytes.argtypes = [vp]
+    lib.mlff_plan_bind.restype = ctypes.c_int
+    lib.mlff_plan_bind.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, vp]
+    lib.mlff_plan_counts.restype = vp
+    lib.mlff_plan_counts.argtypes = [vp]
+    lib.mlff_energy_forces.restype = ctypes.c_int
+    lib.mlff_energy_forces.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.mlff_md_bind.restype = ctypes.c_int
+    lib.mlff_md_bind.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_double]
+    lib.mlff_md_init_forces.restype = ctypes.c_int
+    lib.mlff_md_init_forces.argtypes = [vp, vp]
+    lib.mlff_md_run.restype = ctypes.c_int
+    lib.mlff_md_run.argtypes = [vp, vp, ctypes.c_int]
+    lib.mlff_md_graph_run.restype = ctypes.c_int
+    lib.mlff_md_graph_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                        "(the HIP library is required; there is no CPU fallback)")
+                handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+                _declare(handle)
+                _lib = handle
+    return _lib
+
+
+def check(fn: str, rc: int):
+    if rc != 0:
+        msg = lib().mlff_last_error().decode(errors="replace")
+        raise MlffError(fn, rc, msg)
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a tensor, or 0 for None."""
+    return 0 if t is None else t.data_ptr()
+
+
+def stream_handle(stream=None) -> int:
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream
+
+
+def weights_layout(H: int, L: int, R: int, max_z: int):
+    n = ctypes.c_int(0)
+    total = i64(0)
+    check("mlff_weights_layout", lib().mlff_weights_layout(H, L, R, max_z, None, 0,
+                                                           ctypes.byref(n), ctypes.byref(total)))
+    slots = (Slot * n.value)()
+    check("mlff_weights_layout", lib().mlff_weights_layout(H, L, R, max_z, slots, n.value,
+                                                           ctypes.byref(n), ctypes.byref(total)))
+    return list(slots), int(total.value)

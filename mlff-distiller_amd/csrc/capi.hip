@@ -1,0 +1,437 @@
+// extern "C" entry points (include/mlff_amd.h).  Host-side bookkeeping
+// only: the plan records the caller's device buffers, the workspace carve-up
+// and the weight layout; no device memory is allocated here.
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace mlff {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+static int hip_fail(hipError_t e, const char* what) {
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return MLFF_ERR_HIP;
+}
+
+static inline int64_t align_floats(int64_t x) { return (x + 63) & ~(int64_t)63; }
+
+int build_layout(int H, int L, int R, int max_z, ModelW* mw, mlff_slot* slots, int max_slots,
+                 int* n_slots) {
+  if (H != 32 && H != 64 && H != 128) {
+    set_error("hidden_dim %d not supported (kernels cover 32, 64, 128)", H);
+    return MLFF_ERR_UNSUPPORTED;
+  }
+  if (L < 1 || L > 8) { set_error("num_interactions %d outside 1..8", L); return MLFF_ERR_UNSUPPORTED; }
+  if (R < 1 || R > 64) { set_error("num_rbf %d outside 1..64", R); return MLFF_ERR_UNSUPPORTED; }
+  if (max_z < 1) { set_error("max_z must be >= 1"); return MLFF_ERR_INVALID; }
+  ModelW m;
+  memset(&m, 0, sizeof(m));
+  m.H = H; m.L = L; m.R = R; m.RP = pad16(R); m.HP2 = pad16(H / 2); m.HP4 = pad16(H / 4);
+  m.max_z = max_z;
+  int64_t off = 0;
+  int ns = 0;
+  auto add = [&](int tensor, int layer, int kind, int rows, int cols, int ld) -> int64_t {
+    const int64_t o = off;
+    if (slots != nullptr && ns < max_slots) {
+      slots[ns].tensor = tensor; slots[ns].layer = layer; slots[ns].kind = kind;
+      slots[ns].rows = rows; slots[ns].cols = cols; slots[ns].ld = ld; slots[ns].offset = o;
+    }
+    ++ns;
+    off = align_floats(off + (int64_t)rows * ld);
+    return o;
+  };
+  const int RP = m.RP, HP2 = m.HP2, HP4 = m.HP4;
+  m.emb = add(MLFF_T_EMBEDDING, -1, 0, max_z + 1, H, H);
+  m.centers = add(MLFF_T_RBF_CENTERS, -1, 0, 1, R, RP);
+  m.widths = add(MLFF_T_RBF_WIDTHS, -1, 0, 1, R, RP);
+  for (int l = 0; l < L; ++l) {
+    LayerW& w = m.layer[l];
+    w.f1w = add(MLFF_T_F1_W, l, 0, H, R, RP);
+    w.f1b = add(MLFF_T_F1_B, l, 0, 1, H, H);
+    w.f1t = add(MLFF_T_F1_W, l, 1, RP, H, H);
+    w.f2w = add(MLFF_T_F2_W, l, 0, 3 * H, H, H);
+    w.f2b = add(MLFF_T_F2_B, l, 0, 1, 3 * H, 3 * H);
+    w.f2t = add(MLFF_T_F2_W, l, 1, H, 3 * H, 3 * H);
+    w.u1w = add(MLFF_T_U1_W, l, 0, H, 2 * H, 2 * H);
+    w.u1b = add(MLFF_T_U1_B, l, 0, 1, H, H);
+    w.u1t = add(MLFF_T_U1_W, l, 1, 2 * H, H, H);
+    w.u2w = add(MLFF_T_U2_W, l, 0, 3 * H, H, H);
+    w.u2b = add(MLFF_T_U2_B, l, 0, 1, 3 * H, 3 * H);
+    w.u2t = add(MLFF_T_U2_W, l, 1, H, 3 * H, 3 * H);
+    w.mix = add(MLFF_T_MIX, l, 0, 3, 3, 3);
+  }
+  m.r1w = add(MLFF_T_R1_W, -1, 0, HP2, H, H);
+  m.r1b = add(MLFF_T_R1_B, -1, 0, 1, H / 2, HP2);
+  m.r1t = add(MLFF_T_R1_W, -1, 1, H, H / 2, HP2);
+  m.r2w = add(MLFF_T_R2_W, -1, 0, HP4, H / 2, HP2);
+  m.r2b = add(MLFF_T_R2_B, -1, 0, 1, H / 4, HP4);
+  m.r2t = add(MLFF_T_R2_W, -1, 1, HP2, H / 4, HP4);
+  m.r3w = add(MLFF_T_R3_W, -1, 0, 1, H / 4, HP4);
+  m.r3b = add(MLFF_T_R3_B, -1, 0, 1, 1, 16);
+  m.total = off + 64;   // zero tail
+  if (mw) *mw = m;
+  if (n_slots) *n_slots = ns;
+  return MLFF_OK;
+}
+
+}  // namespace mlff
+
+using namespace mlff;
+
+struct mlff_plan {
+  mlff_config cfg;
+  ModelW mw;
+  Buffers buf;
+  NLArgs nl;
+  int64_t ws_bytes;
+  bool bound;
+  // MD
+  bool md_bound;
+  double* x; double* p; const double* inv_mass; float* pos32; float* forces; double* energy;
+  double* ledger; int ledger_len; double dt;
+  hipGraph_t graph; hipGraphExec_t graph_exec; int graph_steps; hipStream_t graph_stream;
+  // workspace carve-up (byte offsets)
+  struct { int64_t off, bytes; } seg[32];
+  int nseg;
+};
+
+namespace {
+
+enum Seg {
+  S_COUNTS, S_DEG, S_UDEG, S_ROWPTR, S_PAIRPTR, S_COL, S_EPAIR, S_PLO, S_PHI, S_PSHIFT, S_GEOM,
+  S_PACC, S_GVEC, S_WF, S_S, S_V, S_SMID, S_VMID, S_AUPD, S_G12, S_GS0, S_GV0, S_GS1, S_GV1,
+  S_EPS, S_VIR, S_COUNT_
+};
+
+void carve(mlff_plan* pl) {
+  const mlff_config& c = pl->cfg;
+  const int64_t N = c.n_atoms, H = c.hidden_dim, L = c.num_interactions;
+  const int64_t E = c.edge_capacity, P = c.pair_capacity;
+  int64_t sz[S_COUNT_] = {};
+  sz[S_COUNTS] = 4 * MLFF_COUNT_SLOTS;
+  sz[S_DEG] = 4 * N;
+  sz[S_UDEG] = 4 * N;
+  sz[S_ROWPTR] = 4 * (N + 1);
+  sz[S_PAIRPTR] = 4 * (N + 1);
+  sz[S_COL] = 4 * E;
+  sz[S_EPAIR] = 4 * E;
+  sz[S_PLO] = 4 * P;
+  sz[S_PHI] = 4 * P;
+  sz[S_PSHIFT] = c.any_periodic ? 12 * P : 0;
+  sz[S_GEOM] = 16 * P;
+  sz[S_PACC] = 16 * P;
+  sz[S_GVEC] = 16 * P;
+  sz[S_WF] = 4 * L * P * 3 * H;
+  sz[S_S] = 4 * (L + 1) * N * H;
+  sz[S_V] = 4 * (L + 1) * N * 3 * H;
+  sz[S_SMID] = 4 * L * N * H;
+  sz[S_VMID] = 4 * L * N * 3 * H;
+  sz[S_AUPD] = 4 * L * N * H;
+  sz[S_G12] = 4 * L * N * 2 * H;
+  sz[S_GS0] = 4 * N * H;
+  sz[S_GV0] = 4 * N * 3 * H;
+  sz[S_GS1] = 4 * N * H;
+  sz[S_GV1] = 4 * N * 3 * H;
+  sz[S_EPS] = 4 * N;
+  sz[S_VIR] = 4 * 9 * N;
+  int64_t off = 0;
+  for (int k = 0; k < S_COUNT_; ++k) {
+    pl->seg[k].off = off;
+    pl->seg[k].bytes = sz[k];
+    off += (sz[k] + 255) & ~(int64_t)255;
+  }
+  pl->nseg = S_COUNT_;
+  pl->ws_bytes = off;
+}
+
+template <typename T>
+T* segp(mlff_plan* pl, void* ws, int k) {
+  return pl->seg[k].bytes ? reinterpret_cast<T*>(static_cast<char*>(ws) + pl->seg[k].off) : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlff_version(void) { return 100; }
+
+const char* mlff_last_error(void) { return g_err; }
+
+int mlff_weights_layout(int hidden_dim, int num_interactions, int num_rbf, int max_z,
+                        mlff_slot* slots, int max_slots, int* n_slots, int64_t* total_floats) {
+  ModelW mw;
+  int rc = build_layout(hidden_dim, num_interactions, num_rbf, max_z, &mw, slots, max_slots,
+                        n_slots);
+  if (rc != MLFF_OK) return rc;
+  if (total_floats) *total_floats = mw.total;
+  return MLFF_OK;
+}
+
+int64_t mlff_neighbor_workspace_bytes(int n_atoms) {
+  if (n_atoms < 0) return -1;
+  // deg, udeg, pair_ptr
+  return (int64_t)4 * (3 * (int64_t)n_atoms + 1) + 256;
+}
+
+int mlff_neighbor_list(void* stream, const float* pos_dev, int n_atoms, int n_structures,
+                       const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
+                       const float* cells_dev, const uint8_t* pbc_dev, float cutoff,
+                       int32_t* row_ptr_dev, int32_t* col_dev, int32_t* shift_dev,
+                       int32_t edge_capacity, int32_t* counts_dev, void* workspace_dev) {
+  if (n_atoms < 0 || n_structures < 0 || edge_capacity < 0 || !(cutoff > 0.f)) {
+    set_error("mlff_neighbor_list: invalid sizes or cutoff");
+    return MLFF_ERR_INVALID;
+  }
+  if (n_atoms > 0 && (!pos_dev || !mol_ptr_dev || !atom_mol_dev || !row_ptr_dev ||
+                      !counts_dev || !workspace_dev || (edge_capacity > 0 && !col_dev))) {
+    set_error("mlff_neighbor_list: null pointer");
+    return MLFF_ERR_INVALID;
+  }
+  NLArgs a;
+  memset(&a, 0, sizeof(a));
+  int32_t* ws = static_cast<int32_t*>(workspace_dev);
+  a.pos = pos_dev; a.N = n_atoms; a.B = n_structures;
+  a.mol_ptr = mol_ptr_dev; a.atom_mol = atom_mol_dev;
+  a.cells = cells_dev; a.pbc = pbc_dev; a.rc = cutoff;
+  a.deg = ws; a.udeg = ws + n_atoms; a.pair_ptr = ws + 2 * n_atoms;
+  a.row_ptr = row_ptr_dev; a.col = col_dev; a.edge_shift = shift_dev; a.ecap = edge_capacity;
+  a.counts = counts_dev;
+  if (n_atoms == 0) return MLFF_OK;
+  hipError_t e = launch_neighbor_list(a, static_cast<hipStream_t>(stream), false);
+  if (e != hipSuccess) return hip_fail(e, "mlff_neighbor_list");
+  return MLFF_OK;
+}
+
+int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
+  if (!out || !cfg) { set_error("mlff_plan_create: null argument"); return MLFF_ERR_INVALID; }
+  *out = nullptr;
+  if (cfg->n_atoms < 1 || cfg->n_structures < 1 || cfg->edge_capacity < 0 ||
+      cfg->pair_capacity < 0 || !(cfg->cutoff > 0.f)) {
+    set_error("mlff_plan_create: invalid sizes");
+    return MLFF_ERR_INVALID;
+  }
+  ModelW mw;
+  int rc = build_layout(cfg->hidden_dim, cfg->num_interactions, cfg->num_rbf, cfg->max_z, &mw,
+                        nullptr, 0, nullptr);
+  if (rc != MLFF_OK) return rc;
+  mlff_plan* pl = new (std::nothrow) mlff_plan;
+  if (!pl) { set_error("out of host memory"); return MLFF_ERR_INVALID; }
+  memset(pl, 0, sizeof(*pl));
+  pl->cfg = *cfg;
+  pl->mw = mw;
+  carve(pl);
+  int blocks = cfg->persistent_blocks;
+  if (blocks <= 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    }
+    blocks = 2 * cus;
+  }
+  pl->buf.blocks = (blocks + 7) / 8 * 8;
+  *out = pl;
+  return MLFF_OK;
+}
+
+int mlff_plan_destroy(mlff_plan* pl) {
+  if (!pl) return MLFF_OK;
+  if (pl->graph_exec) (void)hipGraphExecDestroy(pl->graph_exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  delete pl;
+  return MLFF_OK;
+}
+
+int64_t mlff_plan_workspace_bytes(const mlff_plan* pl) { return pl ? pl->ws_bytes : -1; }
+
+int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weights_dev,
+                   const int64_t* numbers_dev, const int32_t* mol_ptr_dev,
+                   const int32_t* atom_mol_dev, const float* cells_dev, const uint8_t* pbc_dev) {
+  if (!pl) { set_error("mlff_plan_bind: null plan"); return MLFF_ERR_INVALID; }
+  if (!ws || ws_bytes < pl->ws_bytes) {
+    set_error("mlff_plan_bind: workspace of %lld bytes, need %lld", (long long)ws_bytes,
+              (long long)pl->ws_bytes);
+    return MLFF_ERR_WORKSPACE;
+  }
+  if (!weights_dev || !numbers_dev || !mol_ptr_dev || !atom_mol_dev) {
+    set_error("mlff_plan_bind: null pointer");
+    return MLFF_ERR_INVALID;
+  }
+  if (pl->cfg.any_periodic && (!cells_dev || !pbc_dev)) {
+    set_error("mlff_plan_bind: periodic plan needs cells and pbc");
+    return MLFF_ERR_INVALID;
+  }
+  Buffers& b = pl->buf;
+  const mlff_config& c = pl->cfg;
+  b.N = c.n_atoms; b.B = c.n_structures; b.H = c.hidden_dim; b.L = c.num_interactions;
+  b.R = c.num_rbf; b.rc = c.cutoff; b.ecap = c.edge_capacity; b.pcap = c.pair_capacity;
+  b.w = weights_dev; b.mw = &pl->mw; b.numbers = numbers_dev; b.mol_ptr = mol_ptr_dev;
+  b.atom_mol = atom_mol_dev; b.cells = c.any_periodic ? cells_dev : nullptr;
+  b.counts = segp<int32_t>(pl, ws, S_COUNTS);
+  b.row_ptr = segp<int32_t>(pl, ws, S_ROWPTR);
+  b.col = segp<int32_t>(pl, ws, S_COL);
+  b.edge_pair = segp<int32_t>(pl, ws, S_EPAIR);
+  b.pair_lo = segp<int32_t>(pl, ws, S_PLO);
+  b.pair_hi = segp<int32_t>(pl, ws, S_PHI);
+  b.pair_shift = segp<int32_t>(pl, ws, S_PSHIFT);
+  b.geom = segp<float4>(pl, ws, S_GEOM);
+  b.pacc = segp<float4>(pl, ws, S_PACC);
+  b.gvec = segp<float4>(pl, ws, S_GVEC);
+  b.Wf = segp<float>(pl, ws, S_WF);
+  b.S = segp<float>(pl, ws, S_S);
+  b.V = segp<float>(pl, ws, S_V);
+  b.Smid = segp<float>(pl, ws, S_SMID);
+  b.Vmid = segp<float>(pl, ws, S_VMID);
+  b.Aupd = segp<float>(pl, ws, S_AUPD);
+  b.G12 = segp<float>(pl, ws, S_G12);
+  b.gS0 = segp<float>(pl, ws, S_GS0);
+  b.gV0 = segp<float>(pl, ws, S_GV0);
+  b.gS1 = segp<float>(pl, ws, S_GS1);
+  b.gV1 = segp<float>(pl, ws, S_GV1);
+  b.eps = segp<float>(pl, ws, S_EPS);
+  b.vir_atom = segp<float>(pl, ws, S_VIR);
+  NLArgs& a = pl->nl;
+  memset(&a, 0, sizeof(a));
+  a.N = b.N; a.B = b.B; a.mol_ptr = mol_ptr_dev; a.atom_mol = atom_mol_dev;
+  a.cells = c.any_periodic ? cells_dev : nullptr; a.pbc = c.any_periodic ? pbc_dev : nullptr;
+  a.rc = c.cutoff;
+  a.deg = segp<int32_t>(pl, ws, S_DEG); a.udeg = segp<int32_t>(pl, ws, S_UDEG);
+  a.row_ptr = segp<int32_t>(pl, ws, S_ROWPTR); a.pair_ptr = segp<int32_t>(pl, ws, S_PAIRPTR);
+  a.col = segp<int32_t>(pl, ws, S_COL); a.edge_shift = nullptr;
+  a.edge_pair = segp<int32_t>(pl, ws, S_EPAIR); a.ecap = b.ecap;
+  a.pair_lo = segp<int32_t>(pl, ws, S_PLO); a.pair_hi = segp<int32_t>(pl, ws, S_PHI);
+  a.pair_shift = segp<int32_t>(pl, ws, S_PSHIFT); a.geom = segp<float4>(pl, ws, S_GEOM);
+  a.pcap = b.pcap; a.counts = b.counts;
+  hipError_t e = hipMemset(b.counts, 0, 4 * MLFF_COUNT_SLOTS);
+  if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
+  pl->bound = true;
+  pl->md_bound = false;
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
+int32_t* mlff_plan_counts(mlff_plan* pl) { return (pl && pl->bound) ? pl->buf.counts : nullptr; }
+
+static int run_ef(mlff_plan* pl, hipStream_t s, const float* pos, double* energy, float* forces,
+                  double* virial, float* atom_energy) {
+  pl->nl.pos = pos;
+  hipError_t e = launch_neighbor_list(pl->nl, s, true);
+  if (e != hipSuccess) return hip_fail(e, "neighbour list");
+  e = launch_forward(pl->buf, s);
+  if (e != hipSuccess) return hip_fail(e, "forward");
+  if (forces != nullptr || virial != nullptr) {
+    e = launch_backward(pl->buf, s, forces, virial != nullptr);
+    if (e != hipSuccess) return hip_fail(e, "adjoint");
+  }
+  e = launch_reduce(pl->buf, s, energy, virial, atom_energy);
+  if (e != hipSuccess) return hip_fail(e, "reduce");
+  return MLFF_OK;
+}
+
+int mlff_energy_forces(mlff_plan* pl, void* stream, const float* pos_dev, double* energy_dev,
+                       float* forces_dev, double* virial_dev, float* atom_energy_dev) {
+  if (!pl || !pl->bound) { set_error("mlff_energy_forces: plan not bound"); return MLFF_ERR_STATE; }
+  if (!pos_dev) { set_error("mlff_energy_forces: null positions"); return MLFF_ERR_INVALID; }
+  return run_ef(pl, static_cast<hipStream_t>(stream), pos_dev, energy_dev, forces_dev, virial_dev,
+                atom_energy_dev);
+}
+
+int mlff_md_bind(mlff_plan* pl, double* x_dev, double* p_dev, const double* inv_mass_dev,
+                 float* pos32_dev, float* forces_dev, double* energy_dev, double* ledger_dev,
+                 int32_t ledger_len, double dt) {
+  if (!pl || !pl->bound) { set_error("mlff_md_bind: plan not bound"); return MLFF_ERR_STATE; }
+  if (!x_dev || !p_dev || !inv_mass_dev || !pos32_dev || !forces_dev || !energy_dev ||
+      ledger_len < 0 || (ledger_len > 0 && !ledger_dev)) {
+    set_error("mlff_md_bind: invalid argument");
+    return MLFF_ERR_INVALID;
+  }
+  pl->x = x_dev; pl->p = p_dev; pl->inv_mass = inv_mass_dev; pl->pos32 = pos32_dev;
+  pl->forces = forces_dev; pl->energy = energy_dev; pl->ledger = ledger_dev;
+  pl->ledger_len = ledger_len; pl->dt = dt;
+  pl->md_bound = true;
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
+int mlff_md_init_forces(mlff_plan* pl, void* stream) {
+  if (!pl || !pl->md_bound) { set_error("mlff_md_init_forces: MD not bound"); return MLFF_ERR_STATE; }
+  return run_ef(pl, static_cast<hipStream_t>(stream), pl->pos32, pl->energy, pl->forces, nullptr,
+                nullptr);
+}
+
+static int md_step(mlff_plan* pl, hipStream_t s) {
+  const Buffers& b = pl->buf;
+  hipError_t e = launch_vv_first(b.N, pl->x, pl->p, pl->inv_mass, pl->forces, pl->pos32, pl->dt, s);
+  if (e != hipSuccess) return hip_fail(e, "vv first half");
+  int rc = run_ef(pl, s, pl->pos32, pl->energy, pl->forces, nullptr, nullptr);
+  if (rc != MLFF_OK) return rc;
+  e = launch_vv_second(b.N, b.B, b.mol_ptr, pl->p, pl->inv_mass, pl->forces, pl->dt, pl->energy,
+                       pl->ledger, pl->ledger_len, b.counts, s);
+  if (e != hipSuccess) return hip_fail(e, "vv second half");
+  return MLFF_OK;
+}
+
+int mlff_md_run(mlff_plan* pl, void* stream, int n_steps) {
+  if (!pl || !pl->md_bound) { set_error("mlff_md_run: MD not bound"); return MLFF_ERR_STATE; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int k = 0; k < n_steps; ++k) {
+    int rc = md_step(pl, s);
+    if (rc != MLFF_OK) return rc;
+  }
+  return MLFF_OK;
+}
+
+int mlff_md_graph_run(mlff_plan* pl, void* stream, int steps_per_graph, int n_launches) {
+  if (!pl || !pl->md_bound) { set_error("mlff_md_graph_run: MD not bound"); return MLFF_ERR_STATE; }
+  if (steps_per_graph < 1 || n_launches < 0) {
+    set_error("mlff_md_graph_run: invalid counts");
+    return MLFF_ERR_INVALID;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (s == nullptr) {
+    set_error("mlff_md_graph_run: graph capture needs a non-default stream");
+    return MLFF_ERR_INVALID;
+  }
+  if (!pl->graph_exec || pl->graph_steps != steps_per_graph) {
+    if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+    if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int rc = MLFF_OK;
+    for (int k = 0; k < steps_per_graph && rc == MLFF_OK; ++k) rc = md_step(pl, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc != MLFF_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) { (void)hipGraphDestroy(g); return hip_fail(e, "hipGraphInstantiate"); }
+    pl->graph = g;
+    pl->graph_exec = ge;
+    pl->graph_steps = steps_per_graph;
+  }
+  for (int k = 0; k < n_launches; ++k) {
+    hipError_t e = hipGraphLaunch(pl->graph_exec, s);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  }
+  return MLFF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// Shared device helpers for the gfx950 PaiNN kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlff {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kEdgeEps = 1e-8f;   // edge_vector / (d + 1e-8), student_model.py:715
+
+__host__ __device__ inline int pad16(int x) { return (x + 15) & ~15; }
+
+// Distance rounded exactly as torch.norm(dim=-1) rounds a 3-vector on the
+// CPU: x*x first, then two fused multiply-adds, then a correctly rounded
+// square root.  Contraction is switched off so the compiler cannot fuse or
+// reorder anything else in here.
+__device__ __forceinline__ float norm3_exact(float x, float y, float z) {
+#pragma clang fp contract(off)
+  float acc = x * x;
+  acc = __builtin_fmaf(y, y, acc);
+  acc = __builtin_fmaf(z, z, acc);
+  return __builtin_sqrtf(acc);
+}
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
+// d/dx [x sigmoid(x)] = s (1 + x (1 - s))
+__device__ __forceinline__ float silu_grad(float x) {
+  float s = 1.0f / (1.0f + expf(-x));
+  return s * (1.0f + x * (1.0f - s));
+}
+
+// Gaussian RBF x cosine cutoff of one basis function (student_model.py:249-253,
+// 285-290, 720-722) and its derivative with respect to d.
+__device__ __forceinline__ float edge_feature(float d, float mu, float width, float rc) {
+  float diff = d - mu;
+  float gamma = 1.0f / (width * width);
+  float phi = expf(-gamma * (diff * diff));
+  float fc = 0.5f * (cosf(kPi * d / rc) + 1.0f);
+  fc = (d < rc) ? fc : 0.0f;
+  return phi * fc;
+}
+__device__ __forceinline__ float edge_feature_grad(float d, float mu, float width, float rc) {
+  float diff = d - mu;
+  float gamma = 1.0f / (width * width);
+  float phi = expf(-gamma * (diff * diff));
+  float arg = kPi * d / rc;
+  float fc = 0.5f * (cosf(arg) + 1.0f);
+  float dfc = -0.5f * (kPi / rc) * sinf(arg);
+  if (!(d < rc)) { fc = 0.0f; dfc = 0.0f; }
+  float dphi = -2.0f * gamma * diff * phi;
+  return dphi * fc + phi * dfc;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tile product on v_mfma_f32_16x16x4f32.
+//
+// acc[rt][ct] += A[rt*16 .. rt*16+15][0..K) * B[cb[ct] .. cb[ct]+15][0..K)^T
+//
+// A lives in LDS, row-major with row stride lda floats (lda % 4 == 0).
+// B is a row-major [rows][ldb] global matrix (a torch Linear weight, or its
+// transpose, zero-padded so K is a multiple of 16).
+//
+// Within each 16-wide K slab, lane group g = lane/16 feeds k = k0 + 4g + s in
+// sub-step s, for both operands.  The MFMA sums over its four k slots, so
+// this permutation of k leaves the product unchanged while letting every lane
+// fetch its four A values and four B values as single 16-byte loads.
+// Result layout (per wave): acc[rt][ct][i] holds
+//   D[row = rt*16 + 4*(lane/16) + i][col = cb[ct] + lane%16].
+// ---------------------------------------------------------------------------
+template <int RT, int NT>
+__device__ __forceinline__ void mma_tile(const float* __restrict__ A, int lda,
+                                         const float* __restrict__ B, int ldb, int K,
+                                         const int (&cb)[NT], f4 (&acc)[RT][NT]) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const float* arow = A + r * lda + 4 * g;
+  const float* brow[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) brow[ct] = B + (size_t)(cb[ct] + r) * ldb + 4 * g;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    f4 a[RT], b[NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) a[rt] = *reinterpret_cast<const f4*>(arow + rt * 16 * lda + k0);
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) b[ct] = *reinterpret_cast<const f4*>(brow[ct] + k0);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[ct][s], acc[rt][ct], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int RT, int NT>
+__device__ __forceinline__ void zero_acc(f4 (&acc)[RT][NT]) {
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+}
+
+// Contiguous range of work tiles for a persistent block.  Workgroups are
+// dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8); the
+// logical index below gives the blocks of one XCD consecutive tile ranges so
+// neighbouring tiles (same molecules, same node features) share that XCD's L2.
+__device__ __forceinline__ void persistent_range(int ntiles, int& t0, int& t1) {
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int logical = (G % 8 == 0) ? ((b % 8) * (G / 8) + b / 8) : b;
+  const int per = (ntiles + G - 1) / G;
+  t0 = logical * per;
+  t1 = min(ntiles, t0 + per);
+}
+
+// Sum over aligned groups of `width` lanes (width a power of two <= 64).
+template <int width>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = width / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace mlff

@@ -1,0 +1,86 @@
+// Internal declarations shared by the kernel translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mlff_amd.h"
+
+namespace mlff {
+
+// Per-interaction-block weight offsets (floats into the packed blob).
+struct LayerW {
+  int64_t f1w, f1b, f1t;   // filter Linear(R,H): W [H][RP], b [H], W^T [RP][H]
+  int64_t f2w, f2b, f2t;   // filter Linear(H,3H): W [3H][H], b [3H], W^T [H][3H]
+  int64_t u1w, u1b, u1t;   // update Linear(2H,H): W [H][2H], b [H], W^T [2H][H]
+  int64_t u2w, u2b, u2t;   // update Linear(H,3H): W [3H][H], b [3H], W^T [H][3H]
+  int64_t mix;             // mixing matrix [3][3]
+};
+
+struct ModelW {
+  int H, L, R, RP, HP2, HP4, max_z;
+  int64_t emb, centers, widths;
+  int64_t r1w, r1b, r1t;   // Linear(H, H/2): [HP2][H], b [HP2], W^T [H][HP2]
+  int64_t r2w, r2b, r2t;   // Linear(H/2, H/4): [HP4][HP2], b [HP4], W^T [HP2][HP4]
+  int64_t r3w, r3b;        // Linear(H/4, 1): w [HP4], b [1] (stored padded to 16)
+  LayerW layer[8];
+  int64_t total;
+};
+
+int build_layout(int H, int L, int R, int max_z, ModelW* mw, mlff_slot* slots, int max_slots,
+                 int* n_slots);
+
+// Neighbour-list launches (nl.hip).
+struct NLArgs {
+  const float* pos; int N; int B;
+  const int32_t* mol_ptr; const int32_t* atom_mol;
+  const float* cells; const uint8_t* pbc; float rc;
+  int32_t* deg; int32_t* udeg; int32_t* row_ptr; int32_t* pair_ptr;
+  int32_t* col; int32_t* edge_shift; int32_t* edge_pair; int32_t ecap;
+  int32_t* pair_lo; int32_t* pair_hi; int32_t* pair_shift; float4* geom; int32_t pcap;
+  int32_t* counts;
+};
+hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs);
+
+// Model launches (painn_fwd.hip / painn_bwd.hip).
+struct Buffers {
+  int N, B, H, L, R;
+  float rc;
+  int32_t ecap, pcap;
+  int blocks;                       // persistent grid for pair kernels
+  const float* w;                   // weight blob
+  const ModelW* mw;
+  const int64_t* numbers;
+  const int32_t* mol_ptr;
+  const int32_t* atom_mol;
+  const float* cells;
+  int32_t* counts;
+  const int32_t* row_ptr; const int32_t* col; const int32_t* edge_pair;
+  const int32_t* pair_lo; const int32_t* pair_hi; const int32_t* pair_shift;
+  const float4* geom;
+  float4* pacc;                     // per pair (gu.xyz, gd)
+  float4* gvec;                     // per pair dE/dvec
+  float* Wf;                        // [L][pcap][3H]
+  float* S; float* V;               // [L+1][N][H], [L+1][N][3][H]
+  float* Smid; float* Vmid;         // [L][N][H], [L][N][3][H]
+  float* Aupd; float* G12;          // [L][N][H], [L][N][2H]
+  float* gS0; float* gV0; float* gS1; float* gV1;  // gradient ping-pong
+  float* eps;                       // [N]
+  float* vir_atom;                  // [N][9]
+};
+hipError_t launch_forward(const Buffers& b, hipStream_t s);
+hipError_t launch_backward(const Buffers& b, hipStream_t s, float* forces, bool want_virial);
+hipError_t launch_reduce(const Buffers& b, hipStream_t s, double* energy, double* virial,
+                         float* atom_energy);
+
+// MD launches (md.hip).
+hipError_t launch_vv_first(int N, double* x, double* p, const double* inv_mass, const float* F,
+                           float* pos32, double dt, hipStream_t s);
+hipError_t launch_vv_second(int N, int B, const int32_t* mol_ptr, double* p,
+                            const double* inv_mass, const float* F, double dt,
+                            const double* epot, double* ledger, int ledger_len,
+                            int32_t* counts, hipStream_t s);
+
+void set_error(const char* fmt, ...);
+
+}  // namespace mlff

@@ -1,0 +1,812 @@
+// PaiNN student energy and hand-written adjoint (forces, virial) on gfx950.
+//
+// Forward follows StudentForceField.forward (student_model.py:634-757):
+//   embedding -> L x [message (PaiNNMessage :346-387), update (PaiNNUpdate
+//   :434-470)] -> energy_head (:599-605) -> per-structure sum (FP64 here).
+// Forces are -dE/dx from the adjoint below instead of torch.autograd
+// (student_model.py:788-793).
+//
+// Data layout (all float32, row-major): node scalars s [N][H], node vectors
+// v [N][3][H] (xyz in the middle, as the reference); per unordered pair p
+// (lo < hi): geometry {vec = p_hi - p_lo (+ image shift), d} and the filter
+// output W_p [3H] = (W_s | W_v1 | W_v2) of each block.
+//
+// Kernels and what bounds them:
+//   filter_fwd   pairs, MFMA (R->H, SiLU, H->3H); 2(RH + 3H^2) flop/pair
+//   message      receiver rows, gather of s_j, v_j, W_p (HBM/L2)
+//   update_fwd   atoms, MFMA (2H->H, SiLU, H->3H) + gated vector epilogue
+//   readout      atoms, MFMA head and its input gradient in one pass
+//   update_bwd   atoms, MFMA (3H->H, H->2H) transposed products
+//   filter_bwd   pairs, gathers both endpoints' features and gradients, MFMA
+//                (3H->H, H->R) back to dE/dd and dE/du
+//   force_gather receiver rows, forces and per-atom virial from dE/dvec
+//   reduce       per structure FP64 sums (energy, virial)
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace mlff {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
+                             const float* __restrict__ emb, float* __restrict__ S0,
+                             float* __restrict__ V0) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)N * H;
+  if (idx >= total) return;
+  const int i = (int)(idx / H), c = (int)(idx % H);
+  S0[idx] = emb[Z[i] * H + c];
+  V0[(int64_t)i * 3 * H + c] = 0.f;
+  V0[(int64_t)i * 3 * H + H + c] = 0.f;
+  V0[(int64_t)i * 3 * H + 2 * H + c] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for 64 pairs per tile
+// ---------------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(kBlock) void filter_fwd_kernel(
+    const float4* __restrict__ geom, const int32_t* __restrict__ counts, int32_t pcap,
+    const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
+    const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout) {
+  constexpr int TM = 64, RT = 4;
+  constexpr int LDE = 64 + 4;
+  constexpr int LDH = H + 4;
+  constexpr int CT1 = H / 16, NT1 = (CT1 + 3) / 4;
+  constexpr int CT2 = 3 * H / 16, NT2 = (CT2 + 3) / 4;
+  static_assert(CT1 % NT1 == 0 && CT2 % NT2 == 0, "tile split");
+  __shared__ __align__(16) float e_lds[TM * LDE];
+  __shared__ __align__(16) float h_lds[TM * LDH];
+  const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
+  const int ntiles = (P + TM - 1) / TM;
+  int t0, t1;
+  persistent_range(ntiles, t0, t1);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const float* f1w = w + lw.f1w;
+  const float* f1b = w + lw.f1b;
+  const float* f2w = w + lw.f2w;
+  const float* f2b = w + lw.f2b;
+  for (int tile = t0; tile < t1; ++tile) {
+    const int pbase = tile * TM;
+    for (int idx = threadIdx.x; idx < TM * RP; idx += kBlock) {
+      const int m = idx / RP, k = idx - m * RP;
+      const int p = pbase + m;
+      float val = 0.f;
+      if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
+      e_lds[m * LDE + k] = val;
+    }
+    __syncthreads();
+    if (wave < CT1 / NT1) {
+      int cb[NT1];
+      f4 acc[RT][NT1];
+#pragma unroll
+      for (int ct = 0; ct < NT1; ++ct) cb[ct] = (wave * NT1 + ct) * 16;
+      zero_acc(acc);
+      mma_tile<RT, NT1>(e_lds, LDE, f1w, RP, RP, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT1; ++ct) {
+          const int c = cb[ct] + r;
+          const float bias = f1b[c];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int m = rt * 16 + 4 * g + i;
+            h_lds[m * LDH + c] = silu(acc[rt][ct][i] + bias);
+          }
+        }
+    }
+    __syncthreads();
+    if (wave < CT2 / NT2) {
+      int cb[NT2];
+      f4 acc[RT][NT2];
+#pragma unroll
+      for (int ct = 0; ct < NT2; ++ct) cb[ct] = (wave * NT2 + ct) * 16;
+      zero_acc(acc);
+      mma_tile<RT, NT2>(h_lds, LDH, f2w, H, H, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT2; ++ct) {
+          const int c = cb[ct] + r;
+          const float bias = f2b[c];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int p = pbase + rt * 16 + 4 * g + i;
+            if (p < P) Wout[(size_t)p * 3 * H + c] = acc[rt][ct][i] + bias;
+          }
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// message (forward) and its node-gradient transpose (backward).
+// FWD:  s'_i = s_i + sum_j s_j * Ws_p ;  v'_i = v_i + sum_j v_j * Wv1_p + u_ji (x) Wv2_p
+// BWD:  gs_i = gs'_i + sum_j gs'_j * Ws_p ; gv_i = gv'_i + sum_j gv'_j * Wv1_p
+// One row per H/2 lanes, two channels per lane; neighbours in ascending j.
+// ---------------------------------------------------------------------------
+template <int H, bool FWD>
+__global__ __launch_bounds__(kBlock) void message_kernel(
+    int N, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ edge_pair, int32_t ecap, const float4* __restrict__ geom,
+    const float* __restrict__ Wf, const float* __restrict__ s_in, const float* __restrict__ v_in,
+    float* __restrict__ s_out, float* __restrict__ v_out) {
+  constexpr int LPR = H / 2;
+  constexpr int RPB = kBlock / LPR;
+  const int sub = threadIdx.x % LPR;
+  const int i = blockIdx.x * RPB + threadIdx.x / LPR;
+  if (i >= N) return;
+  const int c = 2 * sub;
+  float2 as = make_float2(0.f, 0.f), a0 = as, a1 = as, a2 = as;
+  const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
+  for (int e = e0; e < e1; ++e) {
+    const int j = col[e];
+    const int p = edge_pair[e];
+    if (p < 0) continue;
+    const float* wp = Wf + (size_t)p * 3 * H;
+    const float2 ws = *reinterpret_cast<const float2*>(wp + c);
+    const float2 wv1 = *reinterpret_cast<const float2*>(wp + H + c);
+    const float2 sj = *reinterpret_cast<const float2*>(s_in + (size_t)j * H + c);
+    const float* vj = v_in + (size_t)j * 3 * H;
+    const float2 v0 = *reinterpret_cast<const float2*>(vj + c);
+    const float2 v1 = *reinterpret_cast<const float2*>(vj + H + c);
+    const float2 v2 = *reinterpret_cast<const float2*>(vj + 2 * H + c);
+    as.x += sj.x * ws.x; as.y += sj.y * ws.y;
+    a0.x += v0.x * wv1.x; a0.y += v0.y * wv1.y;
+    a1.x += v1.x * wv1.x; a1.y += v1.y * wv1.y;
+    a2.x += v2.x * wv1.x; a2.y += v2.y * wv1.y;
+    if (FWD) {
+      const float2 wv2 = *reinterpret_cast<const float2*>(wp + 2 * H + c);
+      const float4 gm = geom[p];
+      const float den = gm.w + kEdgeEps;
+      // u_ji = (p_j - p_i)/(d + eps): +pair vector when j is the pair's hi atom
+      float ux = gm.x / den, uy = gm.y / den, uz = gm.z / den;
+      if (j < i) { ux = -ux; uy = -uy; uz = -uz; }
+      a0.x += ux * wv2.x; a0.y += ux * wv2.y;
+      a1.x += uy * wv2.x; a1.y += uy * wv2.y;
+      a2.x += uz * wv2.x; a2.y += uz * wv2.y;
+    }
+  }
+  const float2 si = *reinterpret_cast<const float2*>(s_in + (size_t)i * H + c);
+  const float* vi = v_in + (size_t)i * 3 * H;
+  const float2 w0 = *reinterpret_cast<const float2*>(vi + c);
+  const float2 w1 = *reinterpret_cast<const float2*>(vi + H + c);
+  const float2 w2 = *reinterpret_cast<const float2*>(vi + 2 * H + c);
+  *reinterpret_cast<float2*>(s_out + (size_t)i * H + c) = make_float2(si.x + as.x, si.y + as.y);
+  float* vo = v_out + (size_t)i * 3 * H;
+  *reinterpret_cast<float2*>(vo + c) = make_float2(w0.x + a0.x, w0.y + a0.y);
+  *reinterpret_cast<float2*>(vo + H + c) = make_float2(w1.x + a1.x, w1.y + a1.y);
+  *reinterpret_cast<float2*>(vo + 2 * H + c) = make_float2(w2.x + a2.x, w2.y + a2.y);
+}
+
+// ---------------------------------------------------------------------------
+// update forward, 32 atoms per tile
+// ---------------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(kBlock) void update_fwd_kernel(
+    int N, const float* __restrict__ w, LayerW lw, const float* __restrict__ s_mid,
+    const float* __restrict__ v_mid, float* __restrict__ s_out, float* __restrict__ v_out,
+    float* __restrict__ a_upd, float* __restrict__ g12) {
+  constexpr int TM = 32, RT = 2;
+  constexpr int LDX = 2 * H + 4, LDH = H + 4;
+  constexpr int CT1 = H / 16, NT1 = (CT1 + 3) / 4;
+  constexpr int NTC = NT1, NT2 = 3 * NTC;
+  __shared__ __align__(16) float x_lds[TM * LDX];
+  __shared__ __align__(16) float h_lds[TM * LDH];
+  const int base = blockIdx.x * TM;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+    const int m = idx / H, c = idx - m * H, i = base + m;
+    float sv = 0.f, nv = 0.f;
+    if (i < N) {
+      sv = s_mid[(size_t)i * H + c];
+      const float* vv = v_mid + (size_t)i * 3 * H;
+      nv = norm3_exact(vv[c], vv[H + c], vv[2 * H + c]);
+    }
+    x_lds[m * LDX + c] = sv;
+    x_lds[m * LDX + H + c] = nv;
+  }
+  __syncthreads();
+  if (wave < CT1 / NT1) {
+    int cb[NT1];
+    f4 acc[RT][NT1];
+#pragma unroll
+    for (int ct = 0; ct < NT1; ++ct) cb[ct] = (wave * NT1 + ct) * 16;
+    zero_acc(acc);
+    mma_tile<RT, NT1>(x_lds, LDX, w + lw.u1w, 2 * H, 2 * H, cb, acc);
+    const float* b1 = w + lw.u1b;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < NT1; ++ct) {
+        const int c = cb[ct] + r;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int m = rt * 16 + 4 * g + k, i = base + m;
+          const float a = acc[rt][ct][k] + b1[c];
+          if (i < N) a_upd[(size_t)i * H + c] = a;
+          h_lds[m * LDH + c] = silu(a);
+        }
+      }
+  }
+  __syncthreads();
+  if (wave < CT1 / NTC) {
+    int cb[NT2];
+    f4 acc[RT][NT2];
+#pragma unroll
+    for (int q = 0; q < NTC; ++q)
+#pragma unroll
+      for (int sp = 0; sp < 3; ++sp) cb[sp * NTC + q] = sp * H + (wave * NTC + q) * 16;
+    zero_acc(acc);
+    mma_tile<RT, NT2>(h_lds, LDH, w + lw.u2w, H, H, cb, acc);
+    const float* b2 = w + lw.u2b;
+    const float* U = w + lw.mix;
+    float u[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) u[k] = U[k];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int q = 0; q < NTC; ++q) {
+        const int c = (wave * NTC + q) * 16 + r;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int m = rt * 16 + 4 * g + k, i = base + m;
+          if (i >= N) continue;
+          const float ds = acc[rt][q][k] + b2[c];
+          const float gt1 = acc[rt][NTC + q][k] + b2[H + c];
+          const float gt2 = acc[rt][2 * NTC + q][k] + b2[2 * H + c];
+          s_out[(size_t)i * H + c] = x_lds[m * LDX + c] + ds;
+          const float* vv = v_mid + (size_t)i * 3 * H;
+          const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
+          float* vo = v_out + (size_t)i * 3 * H;
+          vo[c] = x0 * gt1 + (u[0] * x0 + u[1] * x1 + u[2] * x2) * gt2;
+          vo[H + c] = x1 * gt1 + (u[3] * x0 + u[4] * x1 + u[5] * x2) * gt2;
+          vo[2 * H + c] = x2 * gt1 + (u[6] * x0 + u[7] * x1 + u[8] * x2) * gt2;
+          g12[(size_t)i * 2 * H + c] = gt1;
+          g12[(size_t)i * 2 * H + H + c] = gt2;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// readout: eps_i = head(s_i) and gs_i = d eps_i / d s_i, 32 atoms per tile
+// ---------------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __restrict__ w,
+                                                         ModelW mw, const float* __restrict__ sL,
+                                                         float* __restrict__ eps,
+                                                         float* __restrict__ gS) {
+  constexpr int TM = 32, RT = 2;
+  constexpr int HP2 = ((H / 2) + 15) & ~15, HP4 = ((H / 4) + 15) & ~15;
+  constexpr int LDX = H + 4, LD2 = HP2 + 4, LD4 = HP4 + 4;
+  __shared__ __align__(16) float x_lds[TM * LDX];
+  __shared__ __align__(16) float a1_lds[TM * LD2];
+  __shared__ __align__(16) float h1_lds[TM * LD2];
+  __shared__ __align__(16) float a2_lds[TM * LD4];
+  __shared__ __align__(16) float g2_lds[TM * LD4];
+  const int base = blockIdx.x * TM;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+    const int m = idx / H, c = idx - m * H, i = base + m;
+    x_lds[m * LDX + c] = (i < N) ? sL[(size_t)i * H + c] : 0.f;
+  }
+  __syncthreads();
+  if (wave < HP2 / 16) {   // Linear(H, H/2) + SiLU, one column tile per wave
+    int cb[1] = {wave * 16};
+    f4 acc[RT][1];
+    zero_acc(acc);
+    mma_tile<RT, 1>(x_lds, LDX, w + mw.r1w, H, H, cb, acc);
+    const int c = cb[0] + r;
+    const float bias = w[mw.r1b + c];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int m = rt * 16 + 4 * g + k;
+        const float a = acc[rt][0][k] + bias;
+        a1_lds[m * LD2 + c] = a;
+        h1_lds[m * LD2 + c] = silu(a);
+      }
+  }
+  __syncthreads();
+  if (wave < HP4 / 16) {   // Linear(H/2, H/4)
+    int cb[1] = {wave * 16};
+    f4 acc[RT][1];
+    zero_acc(acc);
+    mma_tile<RT, 1>(h1_lds, LD2, w + mw.r2w, HP2, HP2, cb, acc);
+    const int c = cb[0] + r;
+    const float bias = w[mw.r2b + c];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a2_lds[(rt * 16 + 4 * g + k) * LD4 + c] = acc[rt][0][k] + bias;
+  }
+  __syncthreads();
+  const float* w3 = w + mw.r3w;
+  if (threadIdx.x < TM) {   // Linear(H/4, 1), its input gradient, and SiLU'
+    const int m = threadIdx.x, i = base + m;
+    float e = 0.f;
+    for (int c = 0; c < HP4; ++c) {
+      const float a = a2_lds[m * LD4 + c];
+      e += silu(a) * w3[c];
+      g2_lds[m * LD4 + c] = w3[c] * silu_grad(a);
+    }
+    if (i < N) eps[i] = e + w[mw.r3b];
+  }
+  __syncthreads();
+  if (wave < HP2 / 16) {   // d/dh1 = g2 W2, then SiLU'
+    int cb[1] = {wave * 16};
+    f4 acc[RT][1];
+    zero_acc(acc);
+    mma_tile<RT, 1>(g2_lds, LD4, w + mw.r2t, HP4, HP4, cb, acc);
+    const int c = cb[0] + r;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int m = rt * 16 + 4 * g + k;
+        h1_lds[m * LD2 + c] = acc[rt][0][k] * silu_grad(a1_lds[m * LD2 + c]);
+      }
+  }
+  __syncthreads();
+  {   // d/ds = g1 W1 over H output columns
+    constexpr int CT = H / 16, NT = (CT + 3) / 4;
+    if (wave < CT / NT) {
+      int cb[NT];
+      f4 acc[RT][NT];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+      zero_acc(acc);
+      mma_tile<RT, NT>(h1_lds, LD2, w + mw.r1t, HP2, HP2, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int i = base + rt * 16 + 4 * g + k;
+            if (i < N) gS[(size_t)i * H + cb[ct] + r] = acc[rt][ct][k];
+          }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update backward, 32 atoms per tile.  gv2 == nullptr means a zero gradient.
+// ---------------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(kBlock) void update_bwd_kernel(
+    int N, const float* __restrict__ w, LayerW lw, const float* __restrict__ gs2,
+    const float* __restrict__ gv2, const float* __restrict__ v_mid,
+    const float* __restrict__ a_upd, const float* __restrict__ g12, float* __restrict__ gs1,
+    float* __restrict__ gv1) {
+  constexpr int TM = 32, RT = 2;
+  constexpr int LDY = 3 * H + 4, LDA = H + 4, LDX = 2 * H + 4;
+  __shared__ __align__(16) float y_lds[TM * LDY];
+  __shared__ __align__(16) float a_lds[TM * LDA];
+  const int base = blockIdx.x * TM;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const float* U = w + lw.mix;
+  float u[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u[k] = U[k];
+  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+    const int m = idx / H, c = idx - m * H, i = base + m;
+    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+    if (i < N) {
+      y0 = gs2[(size_t)i * H + c];
+      if (gv2 != nullptr) {
+        const float* vv = v_mid + (size_t)i * 3 * H;
+        const float* gg = gv2 + (size_t)i * 3 * H;
+        const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
+        const float q0 = gg[c], q1 = gg[H + c], q2 = gg[2 * H + c];
+        y1 = q0 * x0 + q1 * x1 + q2 * x2;
+        y2 = q0 * (u[0] * x0 + u[1] * x1 + u[2] * x2) + q1 * (u[3] * x0 + u[4] * x1 + u[5] * x2) +
+             q2 * (u[6] * x0 + u[7] * x1 + u[8] * x2);
+      }
+    }
+    y_lds[m * LDY + c] = y0;
+    y_lds[m * LDY + H + c] = y1;
+    y_lds[m * LDY + 2 * H + c] = y2;
+  }
+  __syncthreads();
+  {
+    constexpr int CT = H / 16, NT = (CT + 3) / 4;
+    if (wave < CT / NT) {
+      int cb[NT];
+      f4 acc[RT][NT];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+      zero_acc(acc);
+      mma_tile<RT, NT>(y_lds, LDY, w + lw.u2t, 3 * H, 3 * H, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int m = rt * 16 + 4 * g + k, i = base + m, c = cb[ct] + r;
+            const float a = (i < N) ? a_upd[(size_t)i * H + c] : 0.f;
+            a_lds[m * LDA + c] = acc[rt][ct][k] * silu_grad(a);
+          }
+    }
+  }
+  __syncthreads();
+  float* x_lds = y_lds;   // y is dead after the barrier above
+  {
+    constexpr int CT = 2 * H / 16, NT = (CT + 3) / 4;
+    if (wave < CT / NT) {
+      int cb[NT];
+      f4 acc[RT][NT];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+      zero_acc(acc);
+      mma_tile<RT, NT>(a_lds, LDA, w + lw.u1t, H, H, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) x_lds[(rt * 16 + 4 * g + k) * LDX + cb[ct] + r] = acc[rt][ct][k];
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+    const int m = idx / H, c = idx - m * H, i = base + m;
+    if (i >= N) continue;
+    gs1[(size_t)i * H + c] = gs2[(size_t)i * H + c] + x_lds[m * LDX + c];
+    const float gn = x_lds[m * LDX + H + c];
+    const float* vv = v_mid + (size_t)i * 3 * H;
+    const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
+    const float n = norm3_exact(x0, x1, x2);
+    const float sc = (n > 0.f) ? gn / n : 0.f;
+    float o0 = sc * x0, o1 = sc * x1, o2 = sc * x2;
+    if (gv2 != nullptr) {
+      const float gt1 = g12[(size_t)i * 2 * H + c], gt2 = g12[(size_t)i * 2 * H + H + c];
+      const float* gg = gv2 + (size_t)i * 3 * H;
+      const float q0 = gg[c], q1 = gg[H + c], q2 = gg[2 * H + c];
+      o0 += q0 * gt1 + (u[0] * q0 + u[3] * q1 + u[6] * q2) * gt2;
+      o1 += q1 * gt1 + (u[1] * q0 + u[4] * q1 + u[7] * q2) * gt2;
+      o2 += q2 * gt1 + (u[2] * q0 + u[5] * q1 + u[8] * q2) * gt2;
+    }
+    float* go = gv1 + (size_t)i * 3 * H;
+    go[c] = o0;
+    go[H + c] = o1;
+    go[2 * H + c] = o2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// filter backward, 32 pairs per tile.
+// For pair p = (a < b): edge b->a carries s_b*Ws, v_b*Wv1 + u (x) Wv2 to a
+// and edge a->b carries s_a*Ws, v_a*Wv1 - u (x) Wv2 to b, with
+// u = (p_b - p_a)/(d + 1e-8).  The tile forms dL/dW_p from both directions,
+// pushes it through W2^T, SiLU', W1^T to dL/de, then to dL/dd, and adds
+// dL/du.  The running (dL/du, dL/dd) of all blocks is kept per pair; the
+// last block processed turns it into dE/dvec.
+// ---------------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(kBlock) void filter_bwd_kernel(
+    const int32_t* __restrict__ counts, int32_t pcap, const int32_t* __restrict__ pair_lo,
+    const int32_t* __restrict__ pair_hi, const float4* __restrict__ geom,
+    const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
+    const float* __restrict__ widths, int R, int RP, float rc, const float* __restrict__ Wf,
+    const float* __restrict__ s_in, const float* __restrict__ v_in,
+    const float* __restrict__ gs1, const float* __restrict__ gv1, float4* __restrict__ pacc,
+    float4* __restrict__ gvec, int first, int last) {
+  constexpr int TM = 32, RT = 2;
+  constexpr int LDE = 64 + 4, LDG = 3 * H + 4, LDA = H + 4;
+  constexpr int GW = H < 64 ? H : 64;          // lanes sharing one pair in the gather
+  constexpr int NPART = H / GW;                // partial sums per pair
+  __shared__ __align__(16) float e_lds[TM * LDE];
+  __shared__ __align__(16) float gw_lds[TM * LDG];
+  __shared__ __align__(16) float ga_lds[TM * LDA];
+  __shared__ float gu_lds[TM * NPART * 3];
+  const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
+  const int ntiles = (P + TM - 1) / TM;
+  int t0, t1;
+  persistent_range(ntiles, t0, t1);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  for (int tile = t0; tile < t1; ++tile) {
+    const int pbase = tile * TM;
+    for (int idx = threadIdx.x; idx < TM * RP; idx += kBlock) {
+      const int m = idx / RP, k = idx - m * RP, p = pbase + m;
+      float val = 0.f;
+      if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
+      e_lds[m * LDE + k] = val;
+    }
+    for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+      const int m = idx / H, c = idx - m * H, p = pbase + m;
+      float gws = 0.f, gwv1 = 0.f, gwv2 = 0.f, gu0 = 0.f, gu1 = 0.f, gu2 = 0.f;
+      if (p < P) {
+        const int a = pair_lo[p], b = pair_hi[p];
+        const float4 gm = geom[p];
+        const float den = gm.w + kEdgeEps;
+        const float ux = gm.x / den, uy = gm.y / den, uz = gm.z / den;
+        gws = gs1[(size_t)a * H + c] * s_in[(size_t)b * H + c] +
+              gs1[(size_t)b * H + c] * s_in[(size_t)a * H + c];
+        const float* va = v_in + (size_t)a * 3 * H;
+        const float* vb = v_in + (size_t)b * 3 * H;
+        const float* qa = gv1 + (size_t)a * 3 * H;
+        const float* qb = gv1 + (size_t)b * 3 * H;
+        const float qa0 = qa[c], qa1 = qa[H + c], qa2 = qa[2 * H + c];
+        const float qb0 = qb[c], qb1 = qb[H + c], qb2 = qb[2 * H + c];
+        gwv1 = qa0 * vb[c] + qa1 * vb[H + c] + qa2 * vb[2 * H + c] + qb0 * va[c] +
+               qb1 * va[H + c] + qb2 * va[2 * H + c];
+        const float d0 = qa0 - qb0, d1 = qa1 - qb1, d2 = qa2 - qb2;
+        gwv2 = ux * d0 + uy * d1 + uz * d2;
+        const float wv2 = Wf[(size_t)p * 3 * H + 2 * H + c];
+        gu0 = wv2 * d0;
+        gu1 = wv2 * d1;
+        gu2 = wv2 * d2;
+      }
+      gw_lds[m * LDG + c] = gws;
+      gw_lds[m * LDG + H + c] = gwv1;
+      gw_lds[m * LDG + 2 * H + c] = gwv2;
+      gu0 = group_sum<GW>(gu0);
+      gu1 = group_sum<GW>(gu1);
+      gu2 = group_sum<GW>(gu2);
+      if ((c % GW) == 0) {
+        const int part = c / GW;
+        gu_lds[(m * NPART + part) * 3 + 0] = gu0;
+        gu_lds[(m * NPART + part) * 3 + 1] = gu1;
+        gu_lds[(m * NPART + part) * 3 + 2] = gu2;
+      }
+    }
+    __syncthreads();
+    {
+      constexpr int CT = H / 16, NT = (CT + 3) / 4;
+      if (wave < CT / NT) {
+        int cb[NT];
+        f4 pre[RT][NT], acc[RT][NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+        zero_acc(pre);
+        zero_acc(acc);
+        mma_tile<RT, NT>(e_lds, LDE, w + lw.f1w, RP, RP, cb, pre);
+        mma_tile<RT, NT>(gw_lds, LDG, w + lw.f2t, 3 * H, 3 * H, cb, acc);
+        const float* b1 = w + lw.f1b;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct) {
+            const int c = cb[ct] + r;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int m = rt * 16 + 4 * g + k;
+              ga_lds[m * LDA + c] = acc[rt][ct][k] * silu_grad(pre[rt][ct][k] + b1[c]);
+            }
+          }
+      }
+    }
+    __syncthreads();
+    float* ge_lds = e_lds;   // RBF inputs are dead after the barrier above
+    if (wave < RP / 16) {
+      int cb[1] = {wave * 16};
+      f4 acc[RT][1];
+      zero_acc(acc);
+      mma_tile<RT, 1>(ga_lds, LDA, w + lw.f1t, H, H, cb, acc);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ge_lds[(rt * 16 + 4 * g + k) * LDE + cb[0] + r] = acc[rt][0][k];
+    }
+    __syncthreads();
+    if (threadIdx.x < TM) {
+      const int m = threadIdx.x, p = pbase + m;
+      if (p < P) {
+        const float4 gm = geom[p];
+        const float d = gm.w;
+        float gd = 0.f;
+        for (int k = 0; k < R; ++k)
+          gd += ge_lds[m * LDE + k] * edge_feature_grad(d, centers[k], widths[k], rc);
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+        for (int part = 0; part < NPART; ++part) {
+          gx += gu_lds[(m * NPART + part) * 3 + 0];
+          gy += gu_lds[(m * NPART + part) * 3 + 1];
+          gz += gu_lds[(m * NPART + part) * 3 + 2];
+        }
+        float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : pacc[p];
+        acc.x += gx; acc.y += gy; acc.z += gz; acc.w += gd;
+        if (last) {
+          // dE/dvec = gd vec/d + gu/(d+eps) - vec (vec.gu)/(d (d+eps)^2)
+          const float den = d + kEdgeEps;
+          const float vdot = gm.x * acc.x + gm.y * acc.y + gm.z * acc.z;
+          const float a1 = (d > 0.f) ? acc.w / d : 0.f;
+          const float a2 = (d > 0.f) ? vdot / (d * den * den) : 0.f;
+          gvec[p] = make_float4(a1 * gm.x + acc.x / den - a2 * gm.x,
+                                a1 * gm.y + acc.y / den - a2 * gm.y,
+                                a1 * gm.z + acc.z / den - a2 * gm.z, 0.f);
+        } else {
+          pacc[p] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// forces F_i = sum_{j>i} G_p - sum_{j<i} G_p; per-atom virial sum_{j>i} vec (x) G
+__global__ void force_gather_kernel(int N, const int32_t* __restrict__ row_ptr,
+                                    const int32_t* __restrict__ col,
+                                    const int32_t* __restrict__ edge_pair, int32_t ecap,
+                                    const float4* __restrict__ gvec,
+                                    const float4* __restrict__ geom, float* __restrict__ forces,
+                                    float* __restrict__ vir_atom) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  float vr[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
+  for (int e = e0; e < e1; ++e) {
+    const int j = col[e], p = edge_pair[e];
+    if (p < 0) continue;
+    const float4 G = gvec[p];
+    if (j > i) {
+      fx += G.x; fy += G.y; fz += G.z;
+      if (vir_atom != nullptr) {
+        const float4 v = geom[p];
+        vr[0] += v.x * G.x; vr[1] += v.x * G.y; vr[2] += v.x * G.z;
+        vr[3] += v.y * G.x; vr[4] += v.y * G.y; vr[5] += v.y * G.z;
+        vr[6] += v.z * G.x; vr[7] += v.z * G.y; vr[8] += v.z * G.z;
+      }
+    } else {
+      fx -= G.x; fy -= G.y; fz -= G.z;
+    }
+  }
+  if (forces != nullptr) {
+    forces[3 * i] = fx;
+    forces[3 * i + 1] = fy;
+    forces[3 * i + 2] = fz;
+  }
+  if (vir_atom != nullptr)
+    for (int k = 0; k < 9; ++k) vir_atom[9 * i + k] = vr[k];
+}
+
+// per-structure FP64 sums in a fixed order (one workgroup per structure)
+__global__ __launch_bounds__(kBlock) void reduce_kernel(int B, const int32_t* __restrict__ mol_ptr,
+                                                        const float* __restrict__ eps,
+                                                        const float* __restrict__ vir_atom,
+                                                        double* __restrict__ energy,
+                                                        double* __restrict__ virial) {
+  __shared__ double sh[kBlock][10];
+  const int b = blockIdx.x;
+  if (b >= B) return;
+  const int i0 = mol_ptr[b], i1 = mol_ptr[b + 1];
+  double acc[10];
+  for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
+    acc[0] += (double)eps[i];
+    if (vir_atom != nullptr)
+      for (int k = 0; k < 9; ++k) acc[1 + k] += (double)vir_atom[9 * i + k];
+  }
+  for (int k = 0; k < 10; ++k) sh[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int k = 0; k < 10; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + off][k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (energy != nullptr) energy[b] = sh[0][0];
+    if (virial != nullptr)
+      for (int k = 0; k < 9; ++k) virial[9 * b + k] = sh[0][1 + k];
+  }
+}
+
+__global__ void copy_f32_kernel(int n, const float* __restrict__ src, float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+inline size_t nh(const Buffers& b) { return (size_t)b.N * b.H; }
+
+template <int H>
+hipError_t forward_impl(const Buffers& b, hipStream_t s) {
+  const ModelW& mw = *b.mw;
+  const int N = b.N;
+  const float* centers = b.w + mw.centers;
+  const float* widths = b.w + mw.widths;
+  {
+    const int64_t total = (int64_t)N * H;
+    embed_kernel<<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        N, H, b.numbers, b.w + mw.emb, b.S, b.V);
+  }
+  const int rows_per_block = kBlock / (H / 2);
+  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int atom_grid = (N + 31) / 32;
+  for (int l = 0; l < b.L; ++l) {
+    const LayerW& lw = mw.layer[l];
+    float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+    filter_fwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+                                                     widths, b.R, mw.RP, b.rc, Wl);
+    float* Sl = b.S + (size_t)l * nh(b);
+    float* Vl = b.V + (size_t)l * 3 * nh(b);
+    float* Sm = b.Smid + (size_t)l * nh(b);
+    float* Vm = b.Vmid + (size_t)l * 3 * nh(b);
+    message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair, b.ecap,
+                                                       b.geom, Wl, Sl, Vl, Sm, Vm);
+    update_fwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+        N, b.w, lw, Sm, Vm, b.S + (size_t)(l + 1) * nh(b), b.V + (size_t)(l + 1) * 3 * nh(b),
+        b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b));
+  }
+  readout_kernel<H><<<atom_grid, kBlock, 0, s>>>(N, b.w, mw, b.S + (size_t)b.L * nh(b), b.eps,
+                                                 b.gS0);
+  return hipGetLastError();
+}
+
+template <int H>
+hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool want_virial) {
+  const ModelW& mw = *b.mw;
+  const int N = b.N;
+  const float* centers = b.w + mw.centers;
+  const float* widths = b.w + mw.widths;
+  const int rows_per_block = kBlock / (H / 2);
+  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int atom_grid = (N + 31) / 32;
+  // gradient w.r.t. the update outputs of the last block: (gS0, 0)
+  float* gs2 = b.gS0;
+  float* gv2 = nullptr;
+  for (int l = b.L - 1; l >= 0; --l) {
+    const LayerW& lw = mw.layer[l];
+    const float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+    update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+        N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
+        b.G12 + (size_t)l * 2 * nh(b), b.gS1, b.gV1);
+    filter_bwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(
+        b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
+        b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
+        b.gvec, l == b.L - 1, l == 0);
+    if (l > 0) {
+      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
+                                                          b.ecap, b.geom, Wl, b.gS1, b.gV1,
+                                                          b.gS0, b.gV0);
+      gs2 = b.gS0;
+      gv2 = b.gV0;
+    }
+  }
+  force_gather_kernel<<<(N + 127) / 128, 128, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair, b.ecap,
+                                                      b.gvec, b.geom, forces,
+                                                      want_virial ? b.vir_atom : nullptr);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_forward(const Buffers& b, hipStream_t s) {
+  switch (b.H) {
+    case 32: return forward_impl<32>(b, s);
+    case 64: return forward_impl<64>(b, s);
+    case 128: return forward_impl<128>(b, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_backward(const Buffers& b, hipStream_t s, float* forces, bool want_virial) {
+  switch (b.H) {
+    case 32: return backward_impl<32>(b, s, forces, want_virial);
+    case 64: return backward_impl<64>(b, s, forces, want_virial);
+    case 128: return backward_impl<128>(b, s, forces, want_virial);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_reduce(const Buffers& b, hipStream_t s, double* energy, double* virial,
+                         float* atom_energy) {
+  if (b.B > 0)
+    reduce_kernel<<<b.B, kBlock, 0, s>>>(b.B, b.mol_ptr, b.eps, virial ? b.vir_atom : nullptr,
+                                         energy, virial);
+  if (atom_energy != nullptr && b.N > 0)
+    copy_f32_kernel<<<(b.N + 255) / 256, 256, 0, s>>>(b.N, b.eps, atom_energy);
+  return hipGetLastError();
+}
+
+}  // namespace mlff

@@ -1,0 +1,237 @@
+"""Device-side engine: weight packing, plans and workspaces, launches.
+
+Everything here drives libmlff_amd.so through ctypes; tensors are plain
+device buffers (PyTorch's caching allocator), never arguments of the C ABI.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _capi as C
+
+# ---------------------------------------------------------------------------
+# weights
+# ---------------------------------------------------------------------------
+
+_NAMES = {
+    C.T_EMBEDDING: "embedding.weight",
+    C.T_RBF_CENTERS: "rbf.centers",
+    C.T_RBF_WIDTHS: "rbf.widths",
+    C.T_F1_W: "interactions.{l}.message.rbf_to_scalar.0.weight",
+    C.T_F1_B: "interactions.{l}.message.rbf_to_scalar.0.bias",
+    C.T_F2_W: "interactions.{l}.message.rbf_to_scalar.2.weight",
+    C.T_F2_B: "interactions.{l}.message.rbf_to_scalar.2.bias",
+    C.T_U1_W: "interactions.{l}.update.update_mlp.0.weight",
+    C.T_U1_B: "interactions.{l}.update.update_mlp.0.bias",
+    C.T_U2_W: "interactions.{l}.update.update_mlp.2.weight",
+    C.T_U2_B: "interactions.{l}.update.update_mlp.2.bias",
+    C.T_MIX: "interactions.{l}.update.mixing_matrix",
+    C.T_R1_W: "energy_head.0.weight",
+    C.T_R1_B: "energy_head.0.bias",
+    C.T_R2_W: "energy_head.2.weight",
+    C.T_R2_B: "energy_head.2.bias",
+    C.T_R3_W: "energy_head.4.weight",
+    C.T_R3_B: "energy_head.4.bias",
+}
+
+
+def hyper_from_state(sd: Dict[str, torch.Tensor]) -> Tuple[int, int, int, int]:
+    H = sd["embedding.weight"].shape[1]
+    max_z = sd["embedding.weight"].shape[0] - 1
+    L = len({int(k.split(".")[1]) for k in sd if k.startswith("interactions.")})
+    R = sd["rbf.centers"].shape[0]
+    return H, L, R, max_z
+
+
+def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
+    """Pack a StudentForceField state dict into the library's float32 blob."""
+    H, L, R, max_z = hyper_from_state(sd)
+    slots, total = C.weights_layout(H, L, R, max_z)
+    host = torch.zeros(total, dtype=torch.float32)
+    for s in slots:
+        name = _NAMES[s.tensor].format(l=s.layer)
+        t = sd[name].detach().to("cpu", torch.float32)
+        if t.dim() == 1:
+            t = t.unsqueeze(0)
+        if s.kind == 1:
+            t = t.t()
+        rows, cols = t.shape
+        if rows > s.rows or cols > s.ld:
+            raise ValueError(f"slot for {name} too small ({rows}x{cols} into {s.rows}x{s.ld})")
+        view = host[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)
+        view[:rows, :cols] = t
+    return host.to(device)
+
+
+# ---------------------------------------------------------------------------
+# batch description
+# ---------------------------------------------------------------------------
+
+@dataclass
+class BatchLayout:
+    """Host description of a flattened batch (atoms of a structure contiguous)."""
+    n_atoms: int
+    sizes: np.ndarray           # [B]
+    mol_ptr: torch.Tensor       # [B+1] int32, device
+    atom_mol: torch.Tensor      # [N] int32, device
+    cells: Optional[torch.Tensor]  # [B,3,3] float32 device or None
+    pbc: Optional[torch.Tensor]    # [B,3] uint8 device or None
+    periodic: bool
+
+    @property
+    def n_structures(self) -> int:
+        return len(self.sizes)
+
+
+def make_layout(sizes: Sequence[int], device, cells=None, pbc=None) -> BatchLayout:
+    sizes = np.asarray(sizes, dtype=np.int64)
+    mol_ptr = np.zeros(len(sizes) + 1, dtype=np.int32)
+    np.cumsum(sizes, out=mol_ptr[1:])
+    atom_mol = np.repeat(np.arange(len(sizes), dtype=np.int32), sizes)
+    periodic = False
+    ct = pt = None
+    if cells is not None and pbc is not None:
+        pbc_np = np.asarray(pbc, dtype=bool).reshape(len(sizes), 3)
+        if pbc_np.any():
+            periodic = True
+            cells_np = np.asarray(cells, dtype=np.float32).reshape(len(sizes), 3, 3)
+            ct = torch.from_numpy(np.ascontiguousarray(cells_np)).to(device)
+            pt = torch.from_numpy(pbc_np.astype(np.uint8)).to(device)
+    return BatchLayout(int(mol_ptr[-1]), sizes, torch.from_numpy(mol_ptr).to(device),
+                       torch.from_numpy(atom_mol).to(device), ct, pt, periodic)
+
+
+def check_cells(cells: np.ndarray, pbc: np.ndarray, cutoff: float):
+    """Minimum image needs the cutoff below half of every periodic width."""
+    cells = np.asarray(cells, dtype=np.float64).reshape(-1, 3, 3)
+    pbc = np.asarray(pbc, dtype=bool).reshape(-1, 3)
+    for c, p in zip(cells, pbc):
+        if not p.any():
+            continue
+        vol = abs(np.linalg.det(c))
+        if vol < 1e-9:
+            raise ValueError("periodic cell is degenerate")
+        for a in range(3):
+            if not p[a]:
+                continue
+            b, d = c[(a + 1) % 3], c[(a + 2) % 3]
+            width = vol / np.linalg.norm(np.cross(b, d))
+            if cutoff >= 0.5 * width:
+                raise ValueError(
+                    f"cutoff {cutoff} A is not below half the periodic cell width {width:.3f} A "
+                    "(minimum image would be ambiguous)")
+
+
+def edge_capacity_bound(sizes: Sequence[int], periodic: bool, max_neighbors: int = 256) -> int:
+    """Upper bound on directed edges: complete graphs for small structures,
+    ``max_neighbors`` per atom for large or periodic ones."""
+    tot = 0
+    for n in np.asarray(sizes).tolist():
+        tot += n * min(n - 1, max_neighbors) if not periodic else n * max_neighbors
+    return max(int(tot), 16)
+
+
+# ---------------------------------------------------------------------------
+# plans
+# ---------------------------------------------------------------------------
+
+class Plan:
+    """One bound plan: workspace + per-batch constants + weights."""
+
+    def __init__(self, H, L, R, max_z, cutoff, layout: BatchLayout, numbers: torch.Tensor,
+                 weights: torch.Tensor, edge_capacity: int, pair_capacity: Optional[int] = None,
+                 persistent_blocks: int = 0):
+        self.lib = C.lib()
+        self.layout = layout
+        self.H, self.L, self.R, self.max_z, self.cutoff = H, L, R, max_z, float(cutoff)
+        self.ecap = int(edge_capacity)
+        self.pcap = int(pair_capacity if pair_capacity is not None else (self.ecap + 1) // 2)
+        cfg = C.Config(H, L, R, max_z, float(cutoff), layout.n_atoms, layout.n_structures,
+                       self.ecap, self.pcap, int(layout.periodic), persistent_blocks)
+        handle = C.vp()
+        C.check("mlff_plan_create", self.lib.mlff_plan_create(ctypes_byref(handle), ctypes_byref(cfg)))
+        self.handle = handle
+        nbytes = self.lib.mlff_plan_workspace_bytes(handle)
+        dev = numbers.device
+        self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.numbers = numbers
+        self.weights = weights
+        C.check("mlff_plan_bind", self.lib.mlff_plan_bind(
+            handle, C.ptr(self.workspace), int(nbytes), C.ptr(weights), C.ptr(numbers),
+            C.ptr(layout.mol_ptr), C.ptr(layout.atom_mol), C.ptr(layout.cells),
+            C.ptr(layout.pbc)))
+        counts_addr = self.lib.mlff_plan_counts(handle)
+        # view of the counts slots inside the workspace
+        off = counts_addr - self.workspace.data_ptr()
+        self.counts = self.workspace[off:off + 4 * C.MLFF_COUNT_SLOTS].view(torch.int32)
+
+    def set_weights(self, weights: torch.Tensor):
+        """Rebind with a new weight blob (same layout)."""
+        self.weights = weights
+        lay = self.layout
+        C.check("mlff_plan_bind", self.lib.mlff_plan_bind(
+            self.handle, C.ptr(self.workspace), self.workspace.numel(), C.ptr(weights),
+            C.ptr(self.numbers), C.ptr(lay.mol_ptr), C.ptr(lay.atom_mol), C.ptr(lay.cells),
+            C.ptr(lay.pbc)))
+
+    def energy_forces(self, positions: torch.Tensor, energy: torch.Tensor,
+                      forces: Optional[torch.Tensor], virial: Optional[torch.Tensor] = None,
+                      atom_energy: Optional[torch.Tensor] = None, stream=None):
+        C.check("mlff_energy_forces", self.lib.mlff_energy_forces(
+            self.handle, C.stream_handle(stream), C.ptr(positions), C.ptr(energy),
+            C.ptr(forces), C.ptr(virial), C.ptr(atom_energy)))
+
+    def read_counts(self) -> np.ndarray:
+        return self.counts.cpu().numpy()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.mlff_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = C.vp()
+
+
+def ctypes_byref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+def neighbor_list(positions: torch.Tensor, layout: BatchLayout, cutoff: float,
+                  edge_capacity: Optional[int] = None, with_shifts: bool = False):
+    """Receiver-CSR neighbour list on the device.
+
+    Returns (row_ptr [N+1] int32, col [E] int32, shift [E,3] int32 or None).
+    """
+    lib = C.lib()
+    N = layout.n_atoms
+    dev = positions.device
+    cap = edge_capacity if edge_capacity is not None else edge_capacity_bound(layout.sizes,
+                                                                             layout.periodic)
+    for _ in range(2):
+        row_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        col = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        shift = torch.empty((max(cap, 1), 3), dtype=torch.int32, device=dev) if with_shifts else None
+        counts = torch.zeros(C.MLFF_COUNT_SLOTS, dtype=torch.int32, device=dev)
+        ws = torch.empty(int(lib.mlff_neighbor_workspace_bytes(N)), dtype=torch.uint8, device=dev)
+        C.check("mlff_neighbor_list", lib.mlff_neighbor_list(
+            C.stream_handle(), C.ptr(positions), N, layout.n_structures, C.ptr(layout.mol_ptr),
+            C.ptr(layout.atom_mol), C.ptr(layout.cells), C.ptr(layout.pbc), float(cutoff),
+            C.ptr(row_ptr), C.ptr(col), C.ptr(shift), cap, C.ptr(counts), C.ptr(ws)))
+        c = counts.cpu().numpy()
+        E = int(c[C.MLFF_COUNT_EDGES])
+        if E <= cap:
+            col = col[:E]
+            if shift is not None:
+                shift = shift[:E]
+            return row_ptr, col, shift
+        cap = E
+    raise RuntimeError("neighbour list capacity did not converge")

@@ -1,0 +1,417 @@
+"""Drop-in StudentForceField whose evaluation runs on the gfx950 kernels.
+
+Interface kept from src/mlff_distiller/models/student_model.py:
+  * constructor arguments and attributes (:566-583), submodule tree and
+    state-dict keys (embedding, rbf.centers/widths, interactions.k.message.
+    rbf_to_scalar.{0,2}, interactions.k.update.update_mlp.{0,2},
+    interactions.k.update.mixing_matrix, energy_head.{0,2,4}), and the order
+    in which __init__ consumes the random stream (:585-628), so
+    ``torch.manual_seed(s); StudentForceField()`` yields the reference's
+    weights;
+  * forward(atomic_numbers, positions, cell, pbc, batch) -> energy (0-d, or
+    [B] when batch has more than one structure) (:634-757), differentiable
+    with respect to positions, so ``-autograd.grad(E, positions)`` gives
+    forces exactly as the reference is used;
+  * predict_energy_and_forces (:759-795), forward_with_analytical_forces
+    (:797-932), num_parameters, save/load checkpoint format (:1075-1176).
+
+Differences, by design: the whole evaluation (neighbour list, message
+passing, readout, and forces from the hand-written adjoint) is one pipeline
+of HIP kernels; periodic cells are honoured (the reference ignores
+cell/pbc, :694-703); structure energies are accumulated in float64 and
+returned in the positions' dtype.  Tensors must live on a ROCm device: there
+is no CPU path.
+"""
+
+from __future__ import annotations
+
+import logging
+from collections import OrderedDict
+from pathlib import Path
+from typing import Optional, Tuple, Union
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from . import engine
+
+logger = logging.getLogger(__name__)
+
+
+class GaussianRBF(nn.Module):
+    """Gaussian radial basis (student_model.py:196-255); centers/widths buffers
+    (parameters when ``learnable``).  Its forward is a convenience for
+    inspecting the basis; StudentForceField evaluates the basis inside its
+    kernels."""
+
+    def __init__(self, num_rbf: int = 20, cutoff: float = 5.0, learnable: bool = False):
+        super().__init__()
+        self.num_rbf = num_rbf
+        self.cutoff = cutoff
+        centers = torch.linspace(0, cutoff, num_rbf)
+        widths = torch.ones(num_rbf) * (cutoff / num_rbf)
+        if learnable:
+            self.centers = nn.Parameter(centers)
+            self.widths = nn.Parameter(widths)
+        else:
+            self.register_buffer("centers", centers)
+            self.register_buffer("widths", widths)
+
+    def forward(self, distances: torch.Tensor) -> torch.Tensor:
+        diff = distances.unsqueeze(-1) - self.centers
+        return torch.exp(-(1.0 / self.widths ** 2) * diff ** 2)
+
+
+class CosineCutoff(nn.Module):
+    """Cosine cutoff (student_model.py:258-292), strict ``d < cutoff``."""
+
+    def __init__(self, cutoff: float = 5.0):
+        super().__init__()
+        self.cutoff = cutoff
+
+    def forward(self, distances: torch.Tensor) -> torch.Tensor:
+        f = 0.5 * (torch.cos(np.pi * distances / self.cutoff) + 1.0)
+        return f * (distances < self.cutoff).to(distances.dtype)
+
+
+class _FusedOnly:
+    def forward(self, *args, **kwargs):
+        raise NotImplementedError(
+            f"{type(self).__name__} holds parameters only; StudentForceField evaluates the "
+            "interaction blocks inside its fused HIP pipeline")
+
+
+class PaiNNMessage(_FusedOnly, nn.Module):
+    """Parameters of the message block (student_model.py:297-322)."""
+
+    def __init__(self, hidden_dim: int, num_rbf: int):
+        super().__init__()
+        self.hidden_dim = hidden_dim
+        self.rbf_to_scalar = nn.Sequential(nn.Linear(num_rbf, hidden_dim), nn.SiLU(),
+                                           nn.Linear(hidden_dim, hidden_dim * 3))
+
+
+class PaiNNUpdate(_FusedOnly, nn.Module):
+    """Parameters of the update block (student_model.py:390-416)."""
+
+    def __init__(self, hidden_dim: int):
+        super().__init__()
+        self.hidden_dim = hidden_dim
+        self.update_mlp = nn.Sequential(nn.Linear(hidden_dim * 2, hidden_dim), nn.SiLU(),
+                                        nn.Linear(hidden_dim, hidden_dim * 3))
+        self.mixing_matrix = nn.Parameter(torch.randn(3, 3) / np.sqrt(3))
+
+
+class PaiNNInteraction(_FusedOnly, nn.Module):
+    """Message + update parameters (student_model.py:473-488)."""
+
+    def __init__(self, hidden_dim: int, num_rbf: int):
+        super().__init__()
+        self.message = PaiNNMessage(hidden_dim, num_rbf)
+        self.update = PaiNNUpdate(hidden_dim)
+
+
+class _EnergyFunction(torch.autograd.Function):
+    """E(positions) from the fused pipeline; dE/dx = -F from the adjoint
+    computed in the same launch sequence."""
+
+    @staticmethod
+    def forward(ctx, positions, runner, batch_index):
+        need = ctx.needs_input_grad[0]
+        energy, forces = runner(positions.detach(), need)
+        ctx.batch_index = batch_index
+        if need:
+            ctx.save_for_backward(forces)
+        return energy
+
+    @staticmethod
+    def backward(ctx, grad_energy):
+        (forces,) = ctx.saved_tensors
+        if grad_energy.dim() == 0:
+            g = grad_energy.to(forces.dtype)
+            return -forces * g, None, None
+        g = grad_energy.to(forces.dtype)[ctx.batch_index].unsqueeze(1)
+        return -forces * g, None, None
+
+
+class StudentForceField(nn.Module):
+    """PaiNN student force field (student_model.py:532-1176) on MI355X."""
+
+    def __init__(self, hidden_dim: int = 128, num_interactions: int = 3, num_rbf: int = 20,
+                 cutoff: float = 5.0, max_z: int = 118, learnable_rbf: bool = False,
+                 use_torch_cluster: bool = True):
+        super().__init__()
+        self.hidden_dim = hidden_dim
+        self.num_interactions = num_interactions
+        self.num_rbf = num_rbf
+        self.cutoff = cutoff
+        self.max_z = max_z
+        self.use_torch_cluster = use_torch_cluster
+        self.embedding = nn.Embedding(max_z + 1, hidden_dim)
+        self.rbf = GaussianRBF(num_rbf, cutoff, learnable_rbf)
+        self.cutoff_fn = CosineCutoff(cutoff)
+        self.interactions = nn.ModuleList([PaiNNInteraction(hidden_dim, num_rbf)
+                                           for _ in range(num_interactions)])
+        self.energy_head = nn.Sequential(
+            nn.Linear(hidden_dim, hidden_dim // 2), nn.SiLU(),
+            nn.Linear(hidden_dim // 2, hidden_dim // 4), nn.SiLU(),
+            nn.Linear(hidden_dim // 4, 1))
+        self._initialize_parameters()
+        self._packed = None
+        self._packed_sig = None
+        self._plans: "OrderedDict[tuple, dict]" = OrderedDict()
+        self.last_energy_f64: Optional[torch.Tensor] = None
+        logger.info("Initialized StudentForceField: %s parameters, %dD hidden, %d interactions, "
+                    "%sA cutoff", f"{self.num_parameters():,}", hidden_dim, num_interactions,
+                    cutoff)
+
+    def _initialize_parameters(self):
+        """Same scheme and order as student_model.py:618-628."""
+        nn.init.uniform_(self.embedding.weight, -np.sqrt(3), np.sqrt(3))
+        for module in self.modules():
+            if isinstance(module, nn.Linear):
+                nn.init.xavier_uniform_(module.weight)
+                if module.bias is not None:
+                    nn.init.zeros_(module.bias)
+
+    def num_parameters(self) -> int:
+        return sum(p.numel() for p in self.parameters() if p.requires_grad)
+
+    # ------------------------------------------------------------------
+    # device plumbing
+    # ------------------------------------------------------------------
+    def packed_weights(self, device) -> torch.Tensor:
+        sig = tuple((t.data_ptr(), t._version) for t in self.state_dict().values())
+        if self._packed is None or self._packed_sig != sig or self._packed.device != device:
+            self._packed = engine.pack_weights(self.state_dict(), device)
+            self._packed_sig = sig
+            for entry in self._plans.values():
+                entry["plan"].set_weights(self._packed)
+        return self._packed
+
+    def _plan_for(self, sizes: np.ndarray, device, periodic: bool, cells, pbc) -> dict:
+        key = (tuple(int(x) for x in sizes), periodic, str(device))
+        entry = self._plans.get(key)
+        if entry is not None:
+            self._plans.move_to_end(key)
+            if periodic:
+                entry["layout"].cells.copy_(torch.as_tensor(np.asarray(cells, dtype=np.float32)
+                                                            .reshape(-1, 3, 3)))
+                entry["layout"].pbc.copy_(torch.as_tensor(np.asarray(pbc, dtype=np.uint8)
+                                                          .reshape(-1, 3)))
+            return entry
+        layout = engine.make_layout(sizes, device, cells if periodic else None,
+                                    pbc if periodic else None)
+        weights = self.packed_weights(device)
+        numbers = torch.zeros(layout.n_atoms, dtype=torch.int64, device=device)
+        cap = engine.edge_capacity_bound(sizes, periodic)
+        plan = engine.Plan(self.hidden_dim, self.num_interactions, self.num_rbf, self.max_z,
+                           self.cutoff, layout, numbers, weights, cap)
+        entry = {"layout": layout, "plan": plan, "numbers": numbers}
+        self._plans[key] = entry
+        while len(self._plans) > 8:
+            self._plans.popitem(last=False)
+        return entry
+
+    def _grow(self, key_entry: dict, sizes, device, periodic, cells, pbc, edges: int):
+        key = (tuple(int(x) for x in sizes), periodic, str(device))
+        layout = key_entry["layout"]
+        plan = engine.Plan(self.hidden_dim, self.num_interactions, self.num_rbf, self.max_z,
+                           self.cutoff, layout, key_entry["numbers"], self.packed_weights(device),
+                           int(edges * 1.25) + 16)
+        key_entry["plan"] = plan
+        self._plans[key] = key_entry
+        return key_entry
+
+    def evaluate(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
+                 sizes: np.ndarray, cells=None, pbc=None, need_forces: bool = True,
+                 need_virial: bool = False):
+        """Run the fused pipeline on a flattened, structure-contiguous batch.
+
+        Returns (energy float64 [B], forces float32 [N,3] or None,
+        virial float64 [B,3,3] or None).
+        """
+        device = positions.device
+        if device.type != "cuda":
+            raise RuntimeError("StudentForceField (MI355X build) needs tensors on a ROCm device; "
+                               f"got {device}")
+        periodic = pbc is not None and bool(np.asarray(pbc).any())
+        if periodic:
+            engine.check_cells(cells, pbc, self.cutoff)
+        zmin = int(atomic_numbers.min()) if atomic_numbers.numel() else 1
+        zmax = int(atomic_numbers.max()) if atomic_numbers.numel() else 1
+        if zmin < 0 or zmax > self.max_z:
+            raise ValueError(f"atomic numbers must lie in [0, {self.max_z}]")
+        entry = self._plan_for(sizes, device, periodic, cells, pbc)
+        self.packed_weights(device)
+        entry["numbers"].copy_(atomic_numbers.to(torch.int64))
+        pos = positions.detach().to(torch.float32).contiguous()
+        B = len(sizes)
+        N = pos.shape[0]
+        for _attempt in range(2):
+            plan = entry["plan"]
+            energy = torch.empty(B, dtype=torch.float64, device=device)
+            forces = torch.empty(N, 3, dtype=torch.float32, device=device) if need_forces else None
+            virial = torch.empty(B, 9, dtype=torch.float64, device=device) if need_virial else None
+            plan.counts[C_OVERFLOW].zero_()
+            plan.energy_forces(pos, energy, forces, virial)
+            counts = plan.read_counts()
+            if counts[C_OVERFLOW] == 0:
+                return energy, forces, (virial.view(B, 3, 3) if virial is not None else None)
+            entry = self._grow(entry, sizes, device, periodic, cells, pbc,
+                               int(counts[C_EDGES]))
+        raise RuntimeError("neighbour-list capacity overflow persisted after regrowth")
+
+    # ------------------------------------------------------------------
+    # reference API
+    # ------------------------------------------------------------------
+    @staticmethod
+    def _batch_sizes(batch: Optional[torch.Tensor], n: int):
+        if batch is None:
+            return np.array([n], dtype=np.int64), None, None, False
+        b = batch.detach().to("cpu", torch.int64).numpy()
+        nstruct = int(b.max()) + 1 if b.size else 1
+        if b.size and np.any(b[1:] < b[:-1]):
+            order = np.argsort(b, kind="stable")
+        else:
+            order = None
+        sizes = np.bincount(b, minlength=nstruct).astype(np.int64)
+        return sizes, order, b, nstruct > 1 or False
+
+    def forward(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
+                cell: Optional[torch.Tensor] = None, pbc: Optional[torch.Tensor] = None,
+                batch: Optional[torch.Tensor] = None) -> torch.Tensor:
+        n = atomic_numbers.shape[0]
+        sizes, order, bnp, multi = self._batch_sizes(batch, n)
+        cells_np, pbc_np = self._cell_args(cell, pbc, len(sizes))
+        Z = atomic_numbers
+        perm = None
+        if order is not None:
+            perm = torch.as_tensor(order, device=positions.device)
+            Z = atomic_numbers[perm]
+        batch_index = None
+        if multi:
+            batch_index = torch.as_tensor(bnp, device=positions.device)
+
+        def runner(pos_detached, need):
+            p = pos_detached[perm] if perm is not None else pos_detached
+            e64, f, _ = self.evaluate(Z, p, sizes, cells_np, pbc_np, need_forces=need)
+            self.last_energy_f64 = e64
+            if f is not None and perm is not None:
+                fo = torch.empty_like(f)
+                fo[perm] = f
+                f = fo
+            e = e64.to(positions.dtype)
+            if not multi:
+                e = e.reshape(())
+            return e, f
+
+        return _EnergyFunction.apply(positions, runner, batch_index)
+
+    @staticmethod
+    def _cell_args(cell, pbc, nstruct):
+        if cell is None or pbc is None:
+            return None, None
+        pbc_np = np.asarray(pbc.detach().cpu() if torch.is_tensor(pbc) else pbc, dtype=bool)
+        if not pbc_np.any():
+            return None, None
+        cell_np = np.asarray(cell.detach().cpu() if torch.is_tensor(cell) else cell,
+                             dtype=np.float64)
+        cell_np = np.broadcast_to(cell_np.reshape(-1, 3, 3), (nstruct, 3, 3))
+        pbc_np = np.broadcast_to(pbc_np.reshape(-1, 3), (nstruct, 3))
+        return np.ascontiguousarray(cell_np), np.ascontiguousarray(pbc_np)
+
+    def predict_energy_and_forces(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
+                                  cell: Optional[torch.Tensor] = None,
+                                  pbc: Optional[torch.Tensor] = None
+                                  ) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Energy and forces (student_model.py:759-795) in one fused launch."""
+        if self.training and torch.is_grad_enabled():
+            positions = positions.requires_grad_(True)
+            energy = self.forward(atomic_numbers, positions, cell, pbc)
+            forces = -torch.autograd.grad(energy, positions, create_graph=True,
+                                          retain_graph=True)[0]
+            return energy, forces
+        n = atomic_numbers.shape[0]
+        cells_np, pbc_np = self._cell_args(cell, pbc, 1)
+        e64, f, _ = self.evaluate(atomic_numbers, positions, np.array([n]), cells_np, pbc_np)
+        self.last_energy_f64 = e64
+        return e64.to(positions.dtype).reshape(()), f.to(positions.dtype)
+
+    def forward_with_analytical_forces(self, atomic_numbers: torch.Tensor,
+                                       positions: torch.Tensor,
+                                       cell: Optional[torch.Tensor] = None,
+                                       pbc: Optional[torch.Tensor] = None,
+                                       batch: Optional[torch.Tensor] = None
+                                       ) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Energy and forces from the hand-written adjoint (student_model.py:797-932
+        names this path; here it really is analytical, with no autograd)."""
+        n = atomic_numbers.shape[0]
+        sizes, order, _, multi = self._batch_sizes(batch, n)
+        cells_np, pbc_np = self._cell_args(cell, pbc, len(sizes))
+        Z, p = atomic_numbers, positions
+        if order is not None:
+            perm = torch.as_tensor(order, device=positions.device)
+            Z, p = atomic_numbers[perm], positions[perm]
+        e64, f, _ = self.evaluate(Z, p, sizes, cells_np, pbc_np)
+        if order is not None:
+            fo = torch.empty_like(f)
+            fo[perm] = f
+            f = fo
+        e = e64.to(positions.dtype)
+        return (e if multi else e.reshape(())), f.to(positions.dtype)
+
+    def energy_forces_virial(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
+                             cell, pbc, sizes: Optional[np.ndarray] = None):
+        """Energy (float64), forces and virial dE/d(strain) (float64 [B,3,3])."""
+        n = atomic_numbers.shape[0]
+        sizes = np.array([n]) if sizes is None else np.asarray(sizes)
+        cells_np, pbc_np = self._cell_args(cell, pbc, len(sizes))
+        return self.evaluate(atomic_numbers, positions, sizes, cells_np, pbc_np, True, True)
+
+    # ------------------------------------------------------------------
+    # checkpoints (student_model.py:1075-1176)
+    # ------------------------------------------------------------------
+    def save(self, path: Union[str, Path]):
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        torch.save({
+            "model_state_dict": self.state_dict(),
+            "config": {"hidden_dim": self.hidden_dim, "num_interactions": self.num_interactions,
+                       "num_rbf": self.num_rbf, "cutoff": self.cutoff, "max_z": self.max_z,
+                       "use_torch_cluster": self.use_torch_cluster},
+            "num_parameters": self.num_parameters(),
+        }, path)
+
+    @classmethod
+    def load(cls, path: Union[str, Path], device: str = "cpu"):
+        checkpoint = torch.load(path, map_location=device, weights_only=True)
+        config = dict(checkpoint.get("config", {}))
+        config.setdefault("use_torch_cluster", True)
+        keep = {"hidden_dim", "num_interactions", "num_rbf", "cutoff", "max_z", "learnable_rbf",
+                "use_torch_cluster"}
+        cfg = {k: v for k, v in config.items() if k in keep}
+        sd = checkpoint["model_state_dict"]
+        if "hidden_dim" not in cfg and "embedding.weight" in sd:
+            cfg["hidden_dim"] = sd["embedding.weight"].shape[1]
+        if "num_rbf" not in cfg and "rbf.centers" in sd:
+            cfg["num_rbf"] = sd["rbf.centers"].shape[0]
+        if "num_interactions" not in cfg:
+            ks = {int(k.split(".")[1]) for k in sd if k.startswith("interactions.")}
+            if ks:
+                cfg["num_interactions"] = len(ks)
+        cfg.setdefault("cutoff", 5.0)
+        if "max_z" not in cfg and "embedding.weight" in sd:
+            cfg["max_z"] = sd["embedding.weight"].shape[0] - 1
+        model = cls(**cfg)
+        model.load_state_dict(sd)
+        model.to(device)
+        return model
+
+
+C_EDGES = 0
+C_OVERFLOW = 2
+
+__all__ = ["StudentForceField", "PaiNNInteraction", "PaiNNMessage", "PaiNNUpdate",
+           "GaussianRBF", "CosineCutoff"]

@@ -1,0 +1,120 @@
+/*
+ * ORACLE -- test infrastructure only.  CPU restatement of the radius-graph
+ * neighbour search in float32.
+ *
+ * Non-periodic: radius_graph_native (student_model.py:63-109): edge (i, j),
+ * i != j, same structure, when the float32 norm of p_j - p_i is <= rc.
+ * torch.norm on the CPU rounds a 3-vector as
+ * sqrtf(fmaf(z, z, fmaf(y, y, x*x))) (checked against torch in the tests),
+ * and that is the rounding restated here.
+ *
+ * Periodic (the reference has none; this is the build's definition): the
+ * fractional difference d @ inv(cell) is rounded to the nearest lattice
+ * vector along periodic axes; orthorhombic cells take that image, other
+ * cells the shortest of its 27 neighbouring images.  Every float operation
+ * is spelled out (fmaf where fused) so the result is reproducible.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared nl_ref.c -o _build/libnl_ref.so -lm
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+float nlr_norm3(float x, float y, float z) {
+  float acc = x * x;
+  acc = fmaf(y, y, acc);
+  acc = fmaf(z, z, acc);
+  return sqrtf(acc);
+}
+
+void nlr_norm3_many(const float* v, int64_t n, float* out) {
+  for (int64_t k = 0; k < n; ++k) out[k] = nlr_norm3(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+}
+
+typedef struct {
+  float c[9], inv[9];
+  int per[3], ortho, periodic;
+} cellinfo;
+
+static void load_cell(const float* cells, const uint8_t* pbc, int b, cellinfo* ci) {
+  memset(ci, 0, sizeof(*ci));
+  if (!cells || !pbc) return;
+  ci->per[0] = pbc[3 * b]; ci->per[1] = pbc[3 * b + 1]; ci->per[2] = pbc[3 * b + 2];
+  ci->periodic = ci->per[0] | ci->per[1] | ci->per[2];
+  if (!ci->periodic) return;
+  const float* c = cells + 9 * b;
+  for (int k = 0; k < 9; ++k) ci->c[k] = c[k];
+  double a = c[0], bb = c[1], cc = c[2], d = c[3], e = c[4], f = c[5], g = c[6], h = c[7], k = c[8];
+  double det = a * (e * k - f * h) - bb * (d * k - f * g) + cc * (d * h - e * g);
+  double id = 1.0 / det;
+  ci->inv[0] = (float)((e * k - f * h) * id); ci->inv[1] = (float)((cc * h - bb * k) * id);
+  ci->inv[2] = (float)((bb * f - cc * e) * id); ci->inv[3] = (float)((f * g - d * k) * id);
+  ci->inv[4] = (float)((a * k - cc * g) * id); ci->inv[5] = (float)((cc * d - a * f) * id);
+  ci->inv[6] = (float)((d * h - e * g) * id); ci->inv[7] = (float)((bb * g - a * h) * id);
+  ci->inv[8] = (float)((a * e - bb * d) * id);
+  ci->ortho = (c[1] == 0.f && c[2] == 0.f && c[3] == 0.f && c[5] == 0.f && c[6] == 0.f &&
+               c[7] == 0.f);
+}
+
+static void shift_vec(const cellinfo* ci, int t0, int t1, int t2, float* s) {
+  float f0 = (float)t0, f1 = (float)t1, f2 = (float)t2;
+  s[0] = fmaf(f2, ci->c[6], fmaf(f1, ci->c[3], f0 * ci->c[0]));
+  s[1] = fmaf(f2, ci->c[7], fmaf(f1, ci->c[4], f0 * ci->c[1]));
+  s[2] = fmaf(f2, ci->c[8], fmaf(f1, ci->c[5], f0 * ci->c[2]));
+}
+
+static int within(float dx, float dy, float dz, float rc, const cellinfo* ci, int* sh) {
+  sh[0] = sh[1] = sh[2] = 0;
+  if (!ci->periodic) return nlr_norm3(dx, dy, dz) <= rc;
+  float fr0 = fmaf(dz, ci->inv[6], fmaf(dy, ci->inv[3], dx * ci->inv[0]));
+  float fr1 = fmaf(dz, ci->inv[7], fmaf(dy, ci->inv[4], dx * ci->inv[1]));
+  float fr2 = fmaf(dz, ci->inv[8], fmaf(dy, ci->inv[5], dx * ci->inv[2]));
+  int n0 = ci->per[0] ? -(int)rintf(fr0) : 0;
+  int n1 = ci->per[1] ? -(int)rintf(fr1) : 0;
+  int n2 = ci->per[2] ? -(int)rintf(fr2) : 0;
+  float s[3];
+  if (ci->ortho) {
+    shift_vec(ci, n0, n1, n2, s);
+    sh[0] = n0; sh[1] = n1; sh[2] = n2;
+    return nlr_norm3(dx + s[0], dy + s[1], dz + s[2]) <= rc;
+  }
+  float best = 3.4e38f;
+  for (int a = -ci->per[0]; a <= ci->per[0]; ++a)
+    for (int b = -ci->per[1]; b <= ci->per[1]; ++b)
+      for (int c = -ci->per[2]; c <= ci->per[2]; ++c) {
+        shift_vec(ci, n0 + a, n1 + b, n2 + c, s);
+        float dd = nlr_norm3(dx + s[0], dy + s[1], dz + s[2]);
+        if (dd < best) { best = dd; sh[0] = n0 + a; sh[1] = n1 + b; sh[2] = n2 + c; }
+      }
+  return best <= rc;
+}
+
+/* Receiver CSR: row i lists j ascending.  Returns the edge count; writes at
+ * most cap entries of col/shift (shift may be NULL). */
+int64_t nlr_build(const float* pos, int n, int B, const int32_t* mol_ptr, const float* cells,
+                  const uint8_t* pbc, float rc, int64_t* row_ptr, int32_t* col, int32_t* shift,
+                  int64_t cap) {
+  int64_t e = 0;
+  for (int b = 0; b < B; ++b) {
+    cellinfo ci;
+    load_cell(cells, pbc, b, &ci);
+    for (int i = mol_ptr[b]; i < mol_ptr[b + 1]; ++i) {
+      row_ptr[i] = e;
+      for (int j = mol_ptr[b]; j < mol_ptr[b + 1]; ++j) {
+        if (j == i) continue;
+        int sh[3];
+        if (within(pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1],
+                   pos[3 * j + 2] - pos[3 * i + 2], rc, &ci, sh)) {
+          if (e < cap) {
+            col[e] = j;
+            if (shift) { shift[3 * e] = sh[0]; shift[3 * e + 1] = sh[1]; shift[3 * e + 2] = sh[2]; }
+          }
+          ++e;
+        }
+      }
+    }
+  }
+  (void)n;
+  row_ptr[mol_ptr[B]] = e;
+  return e;
+}

@@ -1,0 +1,242 @@
+"""ORACLE -- test infrastructure only.
+
+CPU restatement of the reference PaiNN student force field
+(src/mlff_distiller/models/student_model.py), written functionally over a
+state dict so it can run in float64 or float32.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline/checker leg may
+import this module, and only as the checker or the timed CPU baseline; the
+product path never calls it.
+
+Each function cites the reference lines it restates.  Pinning (what stops a
+plausible mistake from passing):
+
+* ``radius_graph_dense``: golden edge lists written by the reference itself
+  (tests/golden), brute force on tiny inputs, and the FP32 predicate
+  ``sqrtf(fmaf(z,z,fmaf(y,y,x*x))) <= rc`` checked against torch.norm.
+* ``energy``/``energy_and_forces``: golden energies/forces of the reference
+  model (tiny and full-size configurations, seed-0 weights) plus invariants
+  (translation, permutation, additivity of far-apart fragments, F = -dE/dx by
+  central differences, batched == per-molecule).
+* PBC (``minimum_image_graph``, ``cell`` arguments): the reference has no PBC
+  (student_model.py:694-703 ignores cell/pbc).  The build's own definition is
+  pinned by a supercell-replica identity (a periodic box equals the matching
+  non-periodic cluster's interior energies only in special cases, so the test
+  uses: edge set of a box == edge set from explicit replication of the 27
+  neighbouring images, and virial == d E / d strain by central differences).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+Tensor = torch.Tensor
+
+
+# --------------------------------------------------------------------------
+# neighbour search
+# --------------------------------------------------------------------------
+
+def radius_graph_dense(positions: Tensor, r: float, batch: Optional[Tensor] = None) -> Tensor:
+    """All ordered pairs (i != j, same structure) with ||p_j - p_i|| <= r.
+
+    Restates radius_graph_native, student_model.py:63-109: the [N,N,3]
+    difference ``p[None] - p[:, None]``, ``torch.norm(dim=2)``, the ``<= r``
+    mask, the same-batch mask, the diagonal removed, and ``torch.where`` order
+    (sorted by source, then destination).  Returns [2, E] int64 (src, dst).
+    """
+    n = positions.shape[0]
+    if batch is None:
+        batch = torch.zeros(n, dtype=torch.long)
+    diff = positions.unsqueeze(0) - positions.unsqueeze(1)
+    dist = torch.norm(diff, dim=2)
+    mask = (batch.unsqueeze(0) == batch.unsqueeze(1)) & (dist <= r)
+    mask &= ~torch.eye(n, dtype=torch.bool)
+    src, dst = torch.where(mask)
+    return torch.stack([src, dst], dim=0)
+
+
+def minimum_image_graph(positions: np.ndarray, cell: np.ndarray, r: float,
+                        dtype=np.float64) -> Tuple[np.ndarray, np.ndarray]:
+    """Brute-force periodic neighbour list (the build's PBC definition).
+
+    For every ordered pair (i, j), i != j, and every lattice shift
+    S in {-1,0,1}^3 applied to the fractionally wrapped difference, the edge
+    (src=j, dst=i) exists when ||p_j - p_i + S_wrapped . cell|| <= r.  With
+    r below half the smallest perpendicular cell width at most one image
+    qualifies.  Returns edge_index [2, E] (src, dst) sorted by (dst, src) and
+    the integer shift [E, 3] such that
+    edge_vector = p[src] - p[dst] + shift @ cell.
+    """
+    pos = np.asarray(positions, dtype=dtype)
+    cell = np.asarray(cell, dtype=dtype)
+    inv = np.linalg.inv(cell)
+    n = len(pos)
+    src_l, dst_l, sh_l = [], [], []
+    shifts = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)])
+    for i in range(n):
+        d = pos - pos[i]                       # p_j - p_i
+        frac = d @ inv
+        base = -np.round(frac)                 # wrap to nearest image
+        for j in range(n):
+            if j == i:
+                continue
+            for s in shifts:
+                tot = base[j] + s
+                v = d[j] + tot @ cell
+                if math.sqrt(float(v @ v)) <= r:
+                    src_l.append(j)
+                    dst_l.append(i)
+                    sh_l.append(tot)
+    ei = np.array([src_l, dst_l], dtype=np.int64).reshape(2, -1)
+    sh = np.array(sh_l, dtype=np.int64).reshape(-1, 3)
+    order = np.lexsort((ei[0], ei[1]))
+    return ei[:, order], sh[order]
+
+
+# --------------------------------------------------------------------------
+# model pieces
+# --------------------------------------------------------------------------
+
+def _linear(x: Tensor, p: Dict[str, Tensor], name: str) -> Tensor:
+    return x @ p[name + ".weight"].T + p[name + ".bias"]
+
+
+def _silu(x: Tensor) -> Tensor:
+    return x * torch.sigmoid(x)
+
+
+def rbf_expand(d: Tensor, centers: Tensor, widths: Tensor) -> Tensor:
+    """GaussianRBF.forward, student_model.py:249-253: exp(-(1/w^2)(d-mu)^2)."""
+    diff = d.unsqueeze(-1) - centers
+    gamma = 1.0 / (widths ** 2)
+    return torch.exp(-gamma * diff ** 2)
+
+
+def cosine_cutoff(d: Tensor, rc: float) -> Tensor:
+    """CosineCutoff.forward, student_model.py:285-290 (strict d < rc)."""
+    f = 0.5 * (torch.cos(math.pi * d / rc) + 1.0)
+    return f * (d < rc).to(d.dtype)
+
+
+def message(p: Dict[str, Tensor], k: int, H: int, s: Tensor, v: Tensor,
+            src: Tensor, dst: Tensor, erbf: Tensor, unit: Tensor) -> Tuple[Tensor, Tensor]:
+    """PaiNNMessage.forward, student_model.py:346-387."""
+    pre = f"interactions.{k}.message.rbf_to_scalar"
+    w = _linear(_silu(_linear(erbf, p, pre + ".0")), p, pre + ".2")
+    ws, wv1, wv2 = torch.split(w, H, dim=-1)
+    s_out = torch.zeros_like(s).index_add_(0, dst, s[src] * ws)
+    vm = v[src] * wv1.unsqueeze(1) + unit.unsqueeze(-1) * wv2.unsqueeze(1)
+    v_out = torch.zeros_like(v).index_add_(0, dst, vm)
+    return s + s_out, v + v_out
+
+
+def update(p: Dict[str, Tensor], k: int, H: int, s: Tensor, v: Tensor) -> Tuple[Tensor, Tensor]:
+    """PaiNNUpdate.forward, student_model.py:434-470 (no residual on v)."""
+    pre = f"interactions.{k}.update"
+    nrm = torch.norm(v, dim=1)
+    a = _linear(_silu(_linear(torch.cat([s, nrm], dim=-1), p, pre + ".update_mlp.0")),
+                p, pre + ".update_mlp.2")
+    ds, g1, g2 = torch.split(a, H, dim=-1)
+    mixed = torch.einsum("ij,njk->nik", p[pre + ".mixing_matrix"], v)
+    return s + ds, v * g1.unsqueeze(1) + mixed * g2.unsqueeze(1)
+
+
+def readout(p: Dict[str, Tensor], s: Tensor) -> Tensor:
+    """energy_head, student_model.py:599-605 and :736."""
+    h = _silu(_linear(s, p, "energy_head.0"))
+    h = _silu(_linear(h, p, "energy_head.2"))
+    return _linear(h, p, "energy_head.4")
+
+
+def hyper_from_state(p: Dict[str, Tensor]) -> Dict[str, int]:
+    H = p["embedding.weight"].shape[1]
+    L = len({int(k.split(".")[1]) for k in p if k.startswith("interactions.")})
+    R = p["rbf.centers"].shape[0]
+    return {"hidden_dim": H, "num_interactions": L, "num_rbf": R}
+
+
+def energy(p: Dict[str, Tensor], numbers: Tensor, positions: Tensor, cutoff: float = 5.0,
+           batch: Optional[Tensor] = None, n_structures: Optional[int] = None,
+           edge_index: Optional[Tensor] = None, shift_vec: Optional[Tensor] = None,
+           return_atomic: bool = False):
+    """StudentForceField.forward, student_model.py:634-757.
+
+    ``edge_index`` overrides the dense search (to evaluate on a given edge
+    set, e.g. a periodic one); ``shift_vec`` [E,3] is added to the edge
+    vector p[src] - p[dst] (periodic images).  Energies per structure are
+    returned as a [B] tensor (0-d when ``batch`` is None).
+    """
+    hp = hyper_from_state(p)
+    H, L = hp["hidden_dim"], hp["num_interactions"]
+    dt = positions.dtype
+    n = numbers.shape[0]
+    s = p["embedding.weight"][numbers].to(dt)
+    v = torch.zeros(n, 3, H, dtype=dt)
+    if edge_index is None:
+        edge_index = radius_graph_dense(positions, cutoff, batch)
+    src, dst = edge_index[0], edge_index[1]
+    vec = positions[src] - positions[dst]
+    if shift_vec is not None:
+        vec = vec + shift_vec
+    d = torch.norm(vec, dim=1)
+    unit = vec / (d.unsqueeze(1) + 1e-8)
+    erbf = rbf_expand(d, p["rbf.centers"].to(dt), p["rbf.widths"].to(dt))
+    erbf = erbf * cosine_cutoff(d, cutoff).unsqueeze(-1)
+    for k in range(L):
+        s, v = message(p, k, H, s, v, src, dst, erbf, unit)
+        s, v = update(p, k, H, s, v)
+    eps = readout(p, s)[:, 0]
+    if batch is None:
+        tot = eps.sum()
+    else:
+        B = int(batch.max()) + 1 if n_structures is None else n_structures
+        tot = torch.zeros(B, dtype=dt).index_add_(0, batch, eps)
+    return (tot, eps) if return_atomic else tot
+
+
+def state_to(p: Dict[str, Tensor], dtype) -> Dict[str, Tensor]:
+    return {k: (t.to(dtype) if t.is_floating_point() else t) for k, t in p.items()}
+
+
+def energy_and_forces(p: Dict[str, Tensor], numbers, positions, cutoff: float = 5.0,
+                      batch=None, n_structures=None, dtype=torch.float64,
+                      edge_index=None, shifts=None, cell=None, virial: bool = False):
+    """Energy and F = -dE/dx by autograd (student_model.py:759-795).
+
+    Batched: gradient of sum of structure energies, as
+    src/mlff_distiller/inference/ase_calculator.py:757-763.  With ``cell``
+    and integer ``shifts`` the edge vector gets ``shifts @ cell`` and, when
+    ``virial`` is set, the virial dE/d(strain) [3,3] per structure is
+    returned (strain applied as x -> x (I + eps) to positions and cell).
+    """
+    pd = state_to(p, dtype)
+    pos = torch.as_tensor(np.asarray(positions), dtype=dtype).clone()
+    Z = torch.as_tensor(np.asarray(numbers), dtype=torch.long)
+    bt = None if batch is None else torch.as_tensor(np.asarray(batch), dtype=torch.long)
+    ei = None if edge_index is None else torch.as_tensor(np.asarray(edge_index), dtype=torch.long)
+    B = 1 if bt is None else (int(bt.max()) + 1 if n_structures is None else n_structures)
+    eps = torch.zeros(B, 3, 3, dtype=dtype, requires_grad=virial)
+    pos.requires_grad_(True)
+    if virial:
+        bidx = torch.zeros(len(Z), dtype=torch.long) if bt is None else bt
+        strained = torch.einsum("na,nab->nb", pos, torch.eye(3, dtype=dtype) + eps[bidx])
+    else:
+        strained = pos
+    sv = None
+    if shifts is not None:
+        c = torch.as_tensor(np.asarray(cell), dtype=dtype)
+        sv = torch.as_tensor(np.asarray(shifts), dtype=dtype) @ c
+        if virial:
+            bsrc = torch.zeros(ei.shape[1], dtype=torch.long) if bt is None else bt[ei[1]]
+            sv = torch.einsum("ea,eab->eb", sv, torch.eye(3, dtype=dtype) + eps[bsrc])
+    E = energy(pd, Z, strained, cutoff, bt, n_structures, ei, sv)
+    Esum = E.sum()
+    grads = torch.autograd.grad(Esum, [pos, eps] if virial else [pos])
+    F = -grads[0]
+    if virial:
+        return E.detach(), F.detach(), grads[1].detach()
+    return E.detach(), F.detach()

@@ -1,0 +1,94 @@
+"""The C ABI library loads, exports every declared symbol, and its host-only
+entry points (layout, plan bookkeeping, argument checks) behave (CPU only)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+from conftest import ROOT, pkg_mod
+
+C = pkg_mod("_capi")
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "mlff_amd.h")).read()
+    return set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(mlff_\w+)\s*\(", text, flags=re.M))
+
+
+def test_header_and_binding_agree():
+    assert _declared_symbols() == set(C.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    lib = C.lib()
+    for name in C.EXPORTS:
+        assert hasattr(lib, name), name
+    assert lib.mlff_version() >= 100
+
+
+def test_weights_layout_full_model():
+    slots, total = C.weights_layout(128, 3, 20, 118)
+    # 3 model tensors, 13 slots per block, 8 readout slots
+    assert len(slots) == 3 + 13 * 3 + 8
+    offs = sorted(s.offset for s in slots)
+    assert offs[0] == 0 and len(set(offs)) == len(offs)
+    for s in slots:
+        assert s.offset % 64 == 0
+        # MFMA operands need 16-byte rows; the 3x3 mixing matrix is read as scalars
+        assert s.ld % 4 == 0 or s.rows == 1 or s.tensor == C.T_MIX
+        assert s.offset + s.rows * s.ld <= total
+    f1 = [s for s in slots if s.tensor == C.T_F1_W and s.kind == 0][0]
+    assert (f1.rows, f1.cols, f1.ld) == (128, 20, 32)
+
+
+def test_layout_rejects_unsupported_hidden():
+    n = ctypes.c_int(0)
+    tot = ctypes.c_int64(0)
+    rc = C.lib().mlff_weights_layout(48, 3, 20, 118, None, 0, ctypes.byref(n), ctypes.byref(tot))
+    assert rc == -2
+    assert b"hidden_dim 48" in C.lib().mlff_last_error()
+
+
+def test_plan_workspace_accounting():
+    cfg = C.Config(128, 3, 20, 118, 5.0, 1000, 10, 40000, 20000, 0, 64)
+    h = C.vp()
+    C.check("create", C.lib().mlff_plan_create(ctypes.byref(h), ctypes.byref(cfg)))
+    nbytes = C.lib().mlff_plan_workspace_bytes(h)
+    # W per block (3H floats per pair) dominates: 3 * 20000 * 384 * 4 bytes
+    assert nbytes > 3 * 20000 * 384 * 4
+    rc = C.lib().mlff_plan_bind(h, None, 0, None, None, None, None, None, None)
+    assert rc == -4
+    assert C.lib().mlff_energy_forces(h, None, None, None, None, None, None) == -5
+    C.lib().mlff_plan_destroy(h)
+
+
+def test_plan_create_rejects_bad_sizes():
+    cfg = C.Config(128, 3, 20, 118, 5.0, 0, 1, 10, 10, 0, 0)
+    h = C.vp()
+    assert C.lib().mlff_plan_create(ctypes.byref(h), ctypes.byref(cfg)) == -1
+
+
+def test_weight_packing_places_tensors():
+    engine = pkg_mod("engine")
+    model = pkg_mod("model")
+    torch.manual_seed(0)
+    m = model.StudentForceField(hidden_dim=32, num_interactions=2, num_rbf=16)
+    blob = engine.pack_weights(m.state_dict(), "cpu")
+    slots, total = C.weights_layout(32, 2, 16, 118)
+    assert blob.numel() == total
+    sd = m.state_dict()
+    for s in slots:
+        if s.tensor == C.T_F2_W and s.layer == 1:
+            view = blob[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)
+            ref = sd["interactions.1.message.rbf_to_scalar.2.weight"]
+            if s.kind == 0:
+                assert torch.equal(view[:, :32], ref)
+            else:
+                assert torch.equal(view[:32, :96], ref.t())
+        if s.tensor == C.T_R3_W:
+            view = blob[s.offset:s.offset + s.ld]
+            assert torch.equal(view[:8], sd["energy_head.4.weight"][0])
+            assert torch.all(view[8:] == 0)
