@@ -169,6 +169,27 @@ int mlff_energy_forces(mlff_plan* plan, void* stream, const float* pos_dev,
                        double* energy_dev, float* forces_dev, double* virial_dev,
                        float* atom_energy_dev);
 
+/* Re-launch one stage of the last mlff_energy_forces call on its workspace
+ * contents (for timing individual kernels and for the launch-shape tuner;
+ * no counterpart in the reference).  Stages are idempotent: they recompute
+ * the same outputs from the same inputs.  layer selects the interaction
+ * block for per-block stages. */
+enum mlff_stage {
+  MLFF_STAGE_NEIGHBOR = 0,
+  MLFF_STAGE_FILTER_FWD = 1,
+  MLFF_STAGE_MESSAGE_FWD = 2,
+  MLFF_STAGE_UPDATE_FWD = 3,
+  MLFF_STAGE_READOUT = 4,
+  MLFF_STAGE_UPDATE_BWD = 5,
+  MLFF_STAGE_FILTER_BWD = 6,
+  MLFF_STAGE_MESSAGE_BWD = 7,
+  MLFF_STAGE_FORCES = 8
+};
+int mlff_launch_stage(mlff_plan* plan, void* stream, int stage, int layer);
+
+/* Set the persistent grid (workgroups) used by the pair kernels. */
+int mlff_plan_set_blocks(mlff_plan* plan, int blocks);
+
 /* ---------------------------------------------------------------------------
  * Molecular dynamics.  Velocity Verlet as ASE's VelocityVerlet.step
  * (called from nve_harness.py:214-235): p += dt/2 F; x += dt p/m; F = F(x);

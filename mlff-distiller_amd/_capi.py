@@ -38,8 +38,12 @@ EXPORTS = (
     "mlff_version", "mlff_last_error", "mlff_weights_layout", "mlff_neighbor_workspace_bytes",
     "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
-    "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
+    "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run", "mlff_launch_stage",
+    "mlff_plan_set_blocks",
 )
+
+STAGE_NEIGHBOR, STAGE_FILTER_FWD, STAGE_MESSAGE_FWD, STAGE_UPDATE_FWD = 0, 1, 2, 3
+STAGE_READOUT, STAGE_UPDATE_BWD, STAGE_FILTER_BWD, STAGE_MESSAGE_BWD, STAGE_FORCES = 4, 5, 6, 7, 8
 
 
 class Slot(ctypes.Structure):
@@ -106,6 +110,10 @@ def _declare(lib):
     lib.mlff_md_run.argtypes = [vp, vp, ctypes.c_int]
     lib.mlff_md_graph_run.restype = ctypes.c_int
     lib.mlff_md_graph_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.mlff_launch_stage.restype = ctypes.c_int
+    lib.mlff_launch_stage.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.mlff_plan_set_blocks.restype = ctypes.c_int
+    lib.mlff_plan_set_blocks.argtypes = [vp, ctypes.c_int]
 
 
 def lib():

@@ -352,6 +352,28 @@ int mlff_energy_forces(mlff_plan* pl, void* stream, const float* pos_dev, double
                 atom_energy_dev);
 }
 
+int mlff_launch_stage(mlff_plan* pl, void* stream, int stage, int layer) {
+  if (!pl || !pl->bound) { set_error("mlff_launch_stage: plan not bound"); return MLFF_ERR_STATE; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e;
+  if (stage == MLFF_STAGE_NEIGHBOR) {
+    if (!pl->nl.pos) { set_error("mlff_launch_stage: no evaluation yet"); return MLFF_ERR_STATE; }
+    e = launch_neighbor_list(pl->nl, s, true);
+  } else {
+    e = launch_stage(pl->buf, s, stage, layer);
+  }
+  if (e != hipSuccess) return hip_fail(e, "mlff_launch_stage");
+  return MLFF_OK;
+}
+
+int mlff_plan_set_blocks(mlff_plan* pl, int blocks) {
+  if (!pl || blocks < 1) { set_error("mlff_plan_set_blocks: invalid"); return MLFF_ERR_INVALID; }
+  pl->buf.blocks = blocks;
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
 int mlff_md_bind(mlff_plan* pl, double* x_dev, double* p_dev, const double* inv_mass_dev,
                  float* pos32_dev, float* forces_dev, double* energy_dev, double* ledger_dev,
                  int32_t ledger_len, double dt) {

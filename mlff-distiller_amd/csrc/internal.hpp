@@ -70,6 +70,9 @@ struct Buffers {
 };
 hipError_t launch_forward(const Buffers& b, hipStream_t s);
 hipError_t launch_backward(const Buffers& b, hipStream_t s, float* forces, bool want_virial);
+// One model stage (mlff_stage 1..8) of block `layer`, on the workspace as the
+// last full evaluation left it.
+hipError_t launch_stage(const Buffers& b, hipStream_t s, int stage, int layer);
 hipError_t launch_reduce(const Buffers& b, hipStream_t s, double* energy, double* virial,
                          float* atom_energy);
 

@@ -779,7 +779,77 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
   return hipGetLastError();
 }
 
+template <int H>
+hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
+  const ModelW& mw = *b.mw;
+  const int N = b.N;
+  const float* centers = b.w + mw.centers;
+  const float* widths = b.w + mw.widths;
+  const int rows_per_block = kBlock / (H / 2);
+  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int atom_grid = (N + 31) / 32;
+  const LayerW& lw = mw.layer[l];
+  float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+  switch (stage) {
+    case MLFF_STAGE_FILTER_FWD:
+      filter_fwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+                                                       widths, b.R, mw.RP, b.rc, Wl);
+      break;
+    case MLFF_STAGE_MESSAGE_FWD:
+      message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(
+          N, b.row_ptr, b.col, b.edge_pair, b.ecap, b.geom, Wl, b.S + (size_t)l * nh(b),
+          b.V + (size_t)l * 3 * nh(b), b.Smid + (size_t)l * nh(b), b.Vmid + (size_t)l * 3 * nh(b));
+      break;
+    case MLFF_STAGE_UPDATE_FWD:
+      update_fwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+          N, b.w, lw, b.Smid + (size_t)l * nh(b), b.Vmid + (size_t)l * 3 * nh(b),
+          b.S + (size_t)(l + 1) * nh(b), b.V + (size_t)(l + 1) * 3 * nh(b),
+          b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b));
+      break;
+    case MLFF_STAGE_READOUT:
+      readout_kernel<H><<<atom_grid, kBlock, 0, s>>>(N, b.w, mw, b.S + (size_t)b.L * nh(b), b.eps,
+                                                     b.gS0);
+      break;
+    case MLFF_STAGE_UPDATE_BWD:
+      // reads gS0/gV0, writes gS1/gV1 (the buffers of the backward sweep)
+      update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+          N, b.w, lw, b.gS0, (l == b.L - 1) ? nullptr : b.gV0, b.Vmid + (size_t)l * 3 * nh(b),
+          b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b), b.gS1, b.gV1);
+      break;
+    case MLFF_STAGE_FILTER_BWD:
+      // first=1 overwrites the pair accumulator, so repeated launches agree
+      filter_bwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(
+          b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
+          b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
+          b.gvec, 1, 0);
+      break;
+    case MLFF_STAGE_MESSAGE_BWD:
+      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
+                                                          b.ecap, b.geom, Wl, b.gS1, b.gV1,
+                                                          b.gS0, b.gV0);
+      break;
+    case MLFF_STAGE_FORCES:
+      force_gather_kernel<<<(N + 127) / 128, 128, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
+                                                          b.ecap, b.gvec, b.geom, nullptr,
+                                                          b.vir_atom);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_stage(const Buffers& b, hipStream_t s, int stage, int layer) {
+  if (layer < 0 || layer >= b.L) return hipErrorInvalidValue;
+  switch (b.H) {
+    case 32: return stage_impl<32>(b, s, stage, layer);
+    case 64: return stage_impl<64>(b, s, stage, layer);
+    case 128: return stage_impl<128>(b, s, stage, layer);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_forward(const Buffers& b, hipStream_t s) {
   switch (b.H) {
