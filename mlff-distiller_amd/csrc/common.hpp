@@ -83,12 +83,18 @@ __device__ __forceinline__ void mma_tile(const float* __restrict__ A, int lda,
   const float* brow[NT];
 #pragma unroll
   for (int ct = 0; ct < NT; ++ct) brow[ct] = B + (size_t)(cb[ct] + r) * ldb + 4 * g;
+  // The global B operand of slab k+1 is fetched before the MFMAs of slab k
+  // are issued, so its latency hides behind them.
+  f4 b[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) b[ct] = *reinterpret_cast<const f4*>(brow[ct]);
   for (int k0 = 0; k0 < K; k0 += 16) {
-    f4 a[RT], b[NT];
+    f4 a[RT], bn[NT];
+    const int kn = (k0 + 16 < K) ? k0 + 16 : k0;
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) bn[ct] = *reinterpret_cast<const f4*>(brow[ct] + kn);
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) a[rt] = *reinterpret_cast<const f4*>(arow + rt * 16 * lda + k0);
-#pragma unroll
-    for (int ct = 0; ct < NT; ++ct) b[ct] = *reinterpret_cast<const f4*>(brow[ct] + k0);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
 #pragma unroll
@@ -98,7 +104,19 @@ __device__ __forceinline__ void mma_tile(const float* __restrict__ A, int lda,
           acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[ct][s], acc[rt][ct], 0, 0, 0);
       }
     }
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) b[ct] = bn[ct];
   }
+}
+
+// Row index of a row-parallel kernel with XCD-aware placement: consecutive
+// rows (atoms of one molecule) go to blocks on the same XCD, so the node
+// features and filters they share stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  if (G % 8 != 0) return b;
+  return (b % 8) * (G / 8) + b / 8;
 }
 
 template <int RT, int NT>

@@ -229,16 +229,22 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   }
 }
 
+// One thread per edge (grid-stride over the device-side edge count).
 __global__ void nl_link_kernel(int N, const int32_t* __restrict__ row_ptr,
                                const int32_t* __restrict__ pair_ptr,
                                const int32_t* __restrict__ col, int32_t* __restrict__ edge_pair,
-                               int32_t ecap, int32_t pcap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
-  for (int e = e0; e < e1; ++e) {
+                               int32_t ecap, int32_t pcap, const int32_t* __restrict__ counts) {
+  const int E = min(counts[MLFF_COUNT_EDGES], ecap);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+    // row of edge e: last i with row_ptr[i] <= e
+    int lo_r = 0, hi_r = N;
+    while (hi_r - lo_r > 1) {
+      const int mid = (lo_r + hi_r) >> 1;
+      if (row_ptr[mid] <= e) lo_r = mid; else hi_r = mid;
+    }
+    const int i = lo_r;
     const int j = col[e];
-    if (j >= i) break;   // sorted: the rest are upper neighbours
+    if (j >= i) continue;   // upper neighbours got their pair in the fill pass
     // binary search for i among row j (sorted ascending)
     int lo = row_ptr[j], hi = min(row_ptr[j + 1], ecap);
     const int rstart = lo;
@@ -276,9 +282,11 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
                                   build_pairs ? a.edge_pair : nullptr, a.ecap,
                                   build_pairs ? a.pair_lo : nullptr, a.pair_hi, a.pair_shift,
                                   a.geom, a.pcap);
-  if (build_pairs)
-    nl_link_kernel<<<g, tb, 0, s>>>(a.N, a.row_ptr, a.pair_ptr, a.col, a.edge_pair, a.ecap,
-                                    a.pcap);
+  if (build_pairs) {
+    const int lg = min(4096, max(1, (a.ecap + 255) / 256));
+    nl_link_kernel<<<lg, 256, 0, s>>>(a.N, a.row_ptr, a.pair_ptr, a.col, a.edge_pair, a.ecap,
+                                      a.pcap, a.counts);
+  }
   return hipGetLastError();
 }
 

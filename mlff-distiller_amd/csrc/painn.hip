@@ -30,6 +30,8 @@ namespace mlff {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
+constexpr int kPairWaves = kPairBlock / 64;
 
 __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
                              const float* __restrict__ emb, float* __restrict__ S0,
@@ -48,15 +50,15 @@ __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
 // filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for 64 pairs per tile
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kBlock) void filter_fwd_kernel(
+__global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
     const float4* __restrict__ geom, const int32_t* __restrict__ counts, int32_t pcap,
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
     const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout) {
   constexpr int TM = 64, RT = 4;
   constexpr int LDE = 64 + 4;
   constexpr int LDH = H + 4;
-  constexpr int CT1 = H / 16, NT1 = (CT1 + 3) / 4;
-  constexpr int CT2 = 3 * H / 16, NT2 = (CT2 + 3) / 4;
+  constexpr int CT1 = H / 16, NT1 = (CT1 + kPairWaves - 1) / kPairWaves;
+  constexpr int CT2 = 3 * H / 16, NT2 = (CT2 + kPairWaves - 1) / kPairWaves;
   static_assert(CT1 % NT1 == 0 && CT2 % NT2 == 0, "tile split");
   __shared__ __align__(16) float e_lds[TM * LDE];
   __shared__ __align__(16) float h_lds[TM * LDH];
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void filter_fwd_kernel(
   const float* f2b = w + lw.f2b;
   for (int tile = t0; tile < t1; ++tile) {
     const int pbase = tile * TM;
-    for (int idx = threadIdx.x; idx < TM * RP; idx += kBlock) {
+    for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP;
       const int p = pbase + m;
       float val = 0.f;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
   constexpr int LPR = H / 2;
   constexpr int RPB = kBlock / LPR;
   const int sub = threadIdx.x % LPR;
-  const int i = blockIdx.x * RPB + threadIdx.x / LPR;
+  const int i = xcd_block() * RPB + threadIdx.x / LPR;
   if (i >= N) return;
   const int c = 2 * sub;
   float2 as = make_float2(0.f, 0.f), a0 = as, a1 = as, a2 = as;
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 // last block processed turns it into dE/dvec.
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kBlock) void filter_bwd_kernel(
+__global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
     const int32_t* __restrict__ counts, int32_t pcap, const int32_t* __restrict__ pair_lo,
     const int32_t* __restrict__ pair_hi, const float4* __restrict__ geom,
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
@@ -516,13 +518,13 @@ __global__ __launch_bounds__(kBlock) void filter_bwd_kernel(
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   for (int tile = t0; tile < t1; ++tile) {
     const int pbase = tile * TM;
-    for (int idx = threadIdx.x; idx < TM * RP; idx += kBlock) {
+    for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP, p = pbase + m;
       float val = 0.f;
       if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
       e_lds[m * LDE + k] = val;
     }
-    for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+    for (int idx = threadIdx.x; idx < TM * H; idx += kPairBlock) {
       const int m = idx / H, c = idx - m * H, p = pbase + m;
       float gws = 0.f, gwv1 = 0.f, gwv2 = 0.f, gu0 = 0.f, gu1 = 0.f, gu2 = 0.f;
       if (p < P) {
@@ -562,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void filter_bwd_kernel(
     }
     __syncthreads();
     {
-      constexpr int CT = H / 16, NT = (CT + 3) / 4;
+      constexpr int CT = H / 16, NT = (CT + kPairWaves - 1) / kPairWaves;
       if (wave < CT / NT) {
         int cb[NT];
         f4 pre[RT][NT], acc[RT][NT];
@@ -721,12 +723,12 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
         N, H, b.numbers, b.w + mw.emb, b.S, b.V);
   }
   const int rows_per_block = kBlock / (H / 2);
-  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   for (int l = 0; l < b.L; ++l) {
     const LayerW& lw = mw.layer[l];
     float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
-    filter_fwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+    filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
                                                      widths, b.R, mw.RP, b.rc, Wl);
     float* Sl = b.S + (size_t)l * nh(b);
     float* Vl = b.V + (size_t)l * 3 * nh(b);
@@ -750,7 +752,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
   const float* centers = b.w + mw.centers;
   const float* widths = b.w + mw.widths;
   const int rows_per_block = kBlock / (H / 2);
-  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   // gradient w.r.t. the update outputs of the last block: (gS0, 0)
   float* gs2 = b.gS0;
@@ -761,7 +763,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
     update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
         b.G12 + (size_t)l * 2 * nh(b), b.gS1, b.gV1);
-    filter_bwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(
+    filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
         b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
         b.gvec, l == b.L - 1, l == 0);
@@ -786,13 +788,13 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   const float* centers = b.w + mw.centers;
   const float* widths = b.w + mw.widths;
   const int rows_per_block = kBlock / (H / 2);
-  const int msg_grid = (N + rows_per_block - 1) / rows_per_block;
+  const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   const LayerW& lw = mw.layer[l];
   float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
   switch (stage) {
     case MLFF_STAGE_FILTER_FWD:
-      filter_fwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+      filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
                                                        widths, b.R, mw.RP, b.rc, Wl);
       break;
     case MLFF_STAGE_MESSAGE_FWD:
@@ -818,7 +820,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
       break;
     case MLFF_STAGE_FILTER_BWD:
       // first=1 overwrites the pair accumulator, so repeated launches agree
-      filter_bwd_kernel<H><<<b.blocks, kBlock, 0, s>>>(
+      filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(
           b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
           b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
           b.gvec, 1, 0);
