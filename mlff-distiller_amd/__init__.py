@@ -14,7 +14,9 @@ building the library is required (``build_lib.build()``).
 from .model import (CosineCutoff, GaussianRBF, PaiNNInteraction, PaiNNMessage, PaiNNUpdate,
                     StudentForceField)
 from .graph import radius_graph, radius_graph_native, radius_graph_csr
+from .calculator import StudentForceFieldCalculator
+from .md import BatchedNVE, NVEMDHarness
 
 __all__ = ["StudentForceField", "PaiNNInteraction", "PaiNNMessage", "PaiNNUpdate",
            "GaussianRBF", "CosineCutoff", "radius_graph", "radius_graph_native",
-           "radius_graph_csr"]
+           "radius_graph_csr", "StudentForceFieldCalculator", "BatchedNVE", "NVEMDHarness"]

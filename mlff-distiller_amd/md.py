@@ -192,3 +192,123 @@ class BatchedNVE:
 
     def velocities(self) -> np.ndarray:
         return (self.p * self.inv_mass[:, None]).cpu().numpy()
+
+
+class NVEMDHarness:
+    """The reference's NVE harness API (nve_harness.py:58-446) on the device
+    integrator: same constructor, initialize_velocities, run_simulation and
+    summary keys.  Energies/temperatures are recorded every step (from the
+    device ledger); positions, velocities and forces every save_interval
+    steps (the reference copies them every step through the host)."""
+
+    def __init__(self, atoms, calculator, temperature: float = 300.0, timestep: float = 0.5,
+                 trajectory_file=None, log_interval: int = 10, remove_com_motion: bool = True,
+                 seed: Optional[int] = None):
+        if isinstance(atoms, (str,)) or hasattr(atoms, "__fspath__"):
+            from ase.io import read  # only for file inputs
+            atoms = read(str(atoms))
+        self.atoms = atoms.copy()
+        self.calculator = calculator
+        if not hasattr(calculator, "model"):
+            raise TypeError("NVEMDHarness needs a StudentForceFieldCalculator (its model runs "
+                            "the device integrator)")
+        self.temperature = temperature
+        self.timestep_fs = timestep
+        self.timestep = timestep * synth.FS
+        self.trajectory_file = trajectory_file
+        self.log_interval = log_interval
+        self.remove_com_motion = remove_com_motion
+        self._rng = np.random.Generator(np.random.PCG64(seed))
+        self._momenta = None
+        self.trajectory_data = {k: [] for k in ("time", "positions", "velocities", "forces",
+                                                "potential_energy", "kinetic_energy",
+                                                "total_energy", "temperature")}
+        self._n_steps = 0
+        self._wall_time = 0.0
+        self._simulation_complete = False
+
+    def _structure(self):
+        a = self.atoms
+        pbc = np.asarray(a.get_pbc(), dtype=bool)
+        return synth.Structure(np.asarray(a.get_atomic_numbers(), dtype=np.int64),
+                               np.asarray(a.get_positions(), dtype=np.float64),
+                               np.asarray(a.get_cell(), dtype=np.float64) if pbc.any() else None,
+                               pbc if pbc.any() else None)
+
+    def initialize_velocities(self, temperature: Optional[float] = None,
+                              remove_com_translation: bool = True,
+                              remove_com_rotation: bool = True):
+        s = self._structure()
+        T = self.temperature if temperature is None else temperature
+        self._momenta = synth.maxwell_boltzmann(s.numbers, s.positions, T, self._rng,
+                                                remove_com_translation, remove_com_rotation)
+
+    def run_simulation(self, steps: int, initialize_velocities: bool = True,
+                       save_interval: Optional[int] = None):
+        import time as _time
+        if steps <= 0:
+            raise ValueError(f"steps must be positive, got {steps}")
+        save_interval = save_interval or self.log_interval
+        t0 = _time.perf_counter()
+        try:
+            if initialize_velocities or self._momenta is None:
+                self.initialize_velocities()
+            md = BatchedNVE(self.calculator.model, [self._structure()],
+                            timestep_fs=self.timestep_fs, momenta=self._momenta,
+                            max_steps=steps, device=self.calculator.device)
+            self._record(md)
+            done = 0
+            while done < steps:
+                k = min(save_interval, steps - done)
+                md.run(k, graph=k >= 10, steps_per_graph=10 if k >= 10 else 1)
+                done += k
+                self._record(md)
+            epot, ekin = md.energies()
+            n = md.n_atoms
+            td = self.trajectory_data
+            td["time"] = list(np.arange(steps + 1) * self.timestep_fs / 1000.0)
+            td["potential_energy"] = list(epot[:, 0])
+            td["kinetic_energy"] = list(ekin[:, 0])
+            td["total_energy"] = list(epot[:, 0] + ekin[:, 0])
+            td["temperature"] = list(2 * ekin[:, 0] / (3 * n * synth.KB_EV))
+            self.atoms.set_positions(md.positions()) if hasattr(self.atoms, "set_positions") else None
+            self._momenta = md.p.cpu().numpy()
+            self._n_steps = steps
+            self._wall_time = _time.perf_counter() - t0
+            self._simulation_complete = True
+            return self._compute_summary()
+        except Exception as e:
+            raise RuntimeError(f"NVE MD simulation failed after "
+                               f"{len(self.trajectory_data['time'])} steps: {e}") from e
+
+    def _record(self, md):
+        self.trajectory_data["positions"].append(md.positions())
+        self.trajectory_data["velocities"].append(md.velocities())
+        self.trajectory_data["forces"].append(md.forces.cpu().numpy().astype(np.float64))
+
+    def _compute_summary(self):
+        """nve_harness.py:311-360."""
+        td = self.trajectory_data
+        if not td["time"]:
+            raise RuntimeError("No trajectory data collected")
+        times = np.array(td["time"])
+        etot = np.array(td["total_energy"])
+        temps = np.array(td["temperature"])
+        e0, ef = etot[0], etot[-1]
+        return {
+            "trajectory_data": td, "n_steps": self._n_steps,
+            "total_time_ps": times[-1] - times[0], "timestep_fs": self.timestep_fs,
+            "wall_time_s": self._wall_time,
+            "energy_drift_pct": 100.0 * (ef - e0) / abs(e0) if e0 != 0 else 0.0,
+            "energy_drift_abs": ef - e0, "energy_std": np.std(etot),
+            "energy_range": np.ptp(etot), "avg_temperature": np.mean(temps),
+            "std_temperature": np.std(temps), "initial_energy": e0, "final_energy": ef,
+            "avg_potential_energy": np.mean(td["potential_energy"]),
+            "avg_kinetic_energy": np.mean(td["kinetic_energy"]),
+        }
+
+    def reset(self):
+        self.trajectory_data = {k: [] for k in self.trajectory_data}
+        self._simulation_complete = False
+        self._n_steps = 0
+        self._wall_time = 0.0

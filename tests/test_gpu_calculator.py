@@ -1,0 +1,112 @@
+"""Calculator and NVE-harness drop-ins on the GPU (needs an MI355X)."""
+
+import numpy as np
+import pytest
+import torch
+
+from atoms_standin import Atoms
+from conftest import pkg, pkg_mod
+from oracle import nl_ref
+from oracle import painn_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def ckpt(tmp_path_factory):
+    torch.manual_seed(0)
+    m = pkg().StudentForceField()
+    path = tmp_path_factory.mktemp("ck") / "student.pt"
+    m.save(path)
+    return path, {k: v.detach().cpu() for k, v in m.state_dict().items()}
+
+
+def _mol(n, seed):
+    s = pkg_mod("synth").single_molecule(n, seed)
+    return Atoms(s.numbers, s.positions)
+
+
+def test_calculate_matches_oracle_and_caches(gpu, ckpt):
+    note("calculator single molecule")
+    path, sd = ckpt
+    calc = pkg().StudentForceFieldCalculator(path, device="cuda", enable_timing=True)
+    atoms = _mol(30, 4)
+    atoms.calc = calc
+    e = atoms.get_potential_energy()
+    f = atoms.get_forces()
+    E, F = R.energy_and_forces(sd, atoms.numbers, atoms.positions.astype(np.float32).astype(np.float64))
+    assert abs(e - E.item()) / len(atoms) < 1e-5
+    assert np.abs(f - F.numpy()).max() < 1e-4
+    assert calc.n_calls == 1                    # forces came from the cached results
+    atoms.positions[0, 0] += 0.01
+    atoms.get_forces()
+    assert calc.n_calls == 2
+    stats = calc.get_timing_stats()
+    assert stats["n_calls"] == 2 and stats["avg_time"] > 0
+
+
+def test_invalid_atoms_raise_value_error(gpu, ckpt):
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    bad = Atoms([0, 1], [[0, 0, 0], [1, 0, 0]])
+    with pytest.raises(ValueError):
+        calc.calculate(bad)
+
+
+def test_calculate_batch_equals_single(gpu, ckpt):
+    note("calculate_batch")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    mols = [_mol(n, 10 + n) for n in (12, 40, 25, 60)]
+    res = calc.calculate_batch(mols)
+    for a, r in zip(mols, res):
+        single = pkg().StudentForceFieldCalculator(path)
+        a.calc = single
+        assert abs(a.get_potential_energy() - r["energy"]) < 1e-5 * len(a)
+        assert np.abs(a.get_forces() - r["forces"]).max() < 1e-5
+    assert calc.calculate_batch([]) == []
+
+
+def test_periodic_stress_is_virial_over_volume(gpu, ckpt):
+    note("stress periodic water")
+    path, sd = ckpt
+    synth = pkg_mod("synth")
+    box = synth.water_box(5, seed=3)
+    atoms = Atoms(box.numbers, box.positions, box.cell, True)
+    calc = pkg().StudentForceFieldCalculator(path, enable_stress=True)
+    atoms.calc = calc
+    stress = atoms.get_stress()
+    P32 = box.positions.astype(np.float32)
+    rp, cl, sh = nl_ref.neighbor_csr(P32, np.array([0, len(P32)]), 5.0, box.cell[None], box.pbc[None])
+    ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(len(P32)), np.diff(rp))])
+    _, _, W = R.energy_and_forces(sd, box.numbers, P32.astype(np.float64), edge_index=ei,
+                                  shifts=sh, cell=box.cell.astype(np.float32).astype(np.float64),
+                                  virial=True)
+    W = W.numpy()[0]
+    s = 0.5 * (W + W.T) / abs(np.linalg.det(box.cell))
+    want = np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
+    assert np.abs(stress - want).max() < 1e-5 + 1e-3 * np.abs(want).max()
+    mol = _mol(10, 1)
+    mol.calc = calc
+    assert np.all(mol.get_stress() == 0)     # non-periodic: zeros, as the reference
+
+
+def test_nve_harness_api(gpu, ckpt):
+    note("NVEMDHarness 40 steps")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    h = pkg().NVEMDHarness(_mol(20, 2), calc, temperature=300.0, timestep=0.5, log_interval=10,
+                           seed=0)
+    out = h.run_simulation(40)
+    for k in ("energy_drift_pct", "energy_drift_abs", "avg_temperature", "wall_time_s",
+              "initial_energy", "final_energy", "trajectory_data", "n_steps"):
+        assert k in out
+    td = out["trajectory_data"]
+    assert len(td["total_energy"]) == 41 and len(td["positions"]) == 5
+    assert abs(out["energy_drift_abs"]) / 20 < 1e-3
+    with pytest.raises(ValueError):
+        h.run_simulation(0)
