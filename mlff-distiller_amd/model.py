@@ -135,6 +135,122 @@ class _EnergyFunction(torch.autograd.Function):
         return -forces * g, None, None
 
 
+class _Session:
+    """One differentiable evaluation: energies/forces from the kernels, and the
+    hand-derived parameter-gradient passes (train.py) evaluated lazily on the
+    same positions and neighbour list."""
+
+    def __init__(self, model, Z, pos, sizes, cells, pbc, batch_index, multi, perm):
+        self.model, self.Z, self.pos, self.sizes = model, Z, pos, sizes
+        self.cells, self.pbc, self.batch_index, self.multi, self.perm = cells, pbc, batch_index, multi, perm
+        self.forces = None
+        self._prep = None
+
+    def run(self):
+        e64, f, _ = self.model.evaluate(self.Z, self.pos, self.sizes, self.cells, self.pbc,
+                                        need_forces=True)
+        self.model.last_energy_f64 = e64
+        if self.perm is not None:
+            fo = torch.empty_like(f)
+            fo[self.perm] = f
+            f = fo
+        self.forces = f
+        e = e64.to(self.pos.dtype)
+        return e if self.multi else e.reshape(())
+
+    def _prepared(self):
+        if self._prep is None:
+            from . import train
+            p = {k: v.detach() for k, v in self.model.state_dict().items()}
+            g = train.device_graph(self.Z, self.pos, self.sizes, self.model.cutoff, self.cells,
+                                   self.pbc)
+            _, cache = train.forward(p, self.Z, self.pos.detach(), g, self.model.cutoff)
+            self._prep = (p, g, cache)
+        return self._prep
+
+    def _to_sorted(self, t):
+        return t[self.perm] if self.perm is not None else t
+
+    def param_grads(self, gE):
+        from . import train
+        p, g, c = self._prepared()
+        grads, _, _ = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype), want_forces=False)
+        return grads
+
+    def tangent_grads(self, gE, lam):
+        from . import train
+        p, g, c = self._prepared()
+        _, _, tg = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype),
+                                 lam=self._to_sorted(lam.to(self.pos.dtype)), want_forces=False)
+        return tg
+
+
+def _grads_tuple(names, grads, params):
+    out = []
+    for n, prm in zip(names, params):
+        gr = grads.get(n)
+        out.append(None if gr is None else gr.to(prm.dtype).reshape(prm.shape))
+    return tuple(out)
+
+
+class _TrainableEnergy(torch.autograd.Function):
+    """E(positions, parameters): first derivatives from the adjoint (positions)
+    and the hand-derived reverse pass (parameters); when the position gradient
+    is itself differentiated (create_graph), _EnergyGradient supplies the
+    hand-derived second-order pass."""
+
+    @staticmethod
+    def forward(ctx, positions, session, names, *params):
+        ctx.session, ctx.names = session, names
+        return session.run()
+
+    @staticmethod
+    def backward(ctx, gE):
+        s = ctx.session
+        gE_vec = gE.reshape(-1)
+        g_pos = None
+        if ctx.needs_input_grad[0]:
+            if torch.is_grad_enabled():
+                g_pos = _EnergyGradient.apply(gE_vec, s, ctx.names, *[
+                    p for _, p in s.model.named_parameters()])
+            else:
+                g = gE_vec.to(s.forces.dtype)
+                g = g[s.batch_index] if s.batch_index is not None else g.expand(s.forces.shape[0])
+                g_pos = -s.forces * g.unsqueeze(1)
+        pg = (None,) * len(ctx.names)
+        if any(ctx.needs_input_grad[3:]):
+            params = [p for _, p in s.model.named_parameters()]
+            pg = _grads_tuple(ctx.names, s.param_grads(gE_vec.detach()), params)
+        return (g_pos, None, None) + pg
+
+
+class _EnergyGradient(torch.autograd.Function):
+    """grad_x (sum_b gE_b E_b) as a function of gE and the parameters."""
+
+    @staticmethod
+    def forward(ctx, gE, session, names, *params):
+        ctx.session, ctx.names = session, names
+        ctx.save_for_backward(gE)
+        g = gE.to(session.forces.dtype)
+        g = g[session.batch_index] if session.batch_index is not None else g.expand(
+            session.forces.shape[0])
+        return -session.forces * g.unsqueeze(1)
+
+    @staticmethod
+    def backward(ctx, gg):
+        (gE,) = ctx.saved_tensors
+        s = ctx.session
+        # d/dgE_b of < gg, grad_x E_b >
+        dot = -(gg * s.forces).sum(1)
+        if s.batch_index is not None:
+            g_gE = torch.zeros_like(gE).index_add_(0, s.batch_index, dot.to(gE.dtype))
+        else:
+            g_gE = dot.sum().reshape(gE.shape).to(gE.dtype)
+        params = [p for _, p in s.model.named_parameters()]
+        tg = _grads_tuple(ctx.names, s.tangent_grads(gE.detach(), gg.detach()), params)
+        return (g_gE, None, None) + tg
+
+
 class StudentForceField(nn.Module):
     """PaiNN student force field (student_model.py:532-1176) on MI355X."""
 
@@ -293,6 +409,15 @@ class StudentForceField(nn.Module):
         batch_index = None
         if multi:
             batch_index = torch.as_tensor(bnp, device=positions.device)
+
+        params = [p for _, p in self.named_parameters()]
+        if torch.is_grad_enabled() and any(p.requires_grad for p in params):
+            # differentiable in the parameters (training): hand-derived passes
+            pos_sorted = positions.detach()[perm] if perm is not None else positions.detach()
+            session = _Session(self, Z, pos_sorted, sizes, cells_np, pbc_np, batch_index, multi,
+                               perm)
+            names = tuple(n_ for n_, _ in self.named_parameters())
+            return _TrainableEnergy.apply(positions, session, names, *params)
 
         def runner(pos_detached, need):
             p = pos_detached[perm] if perm is not None else pos_detached

@@ -149,7 +149,11 @@ def test_golden_full_seed0_molecule(gpu):
     E, F = model.predict_energy_and_forces(Z, pos)
     n = len(fx["full_eval/numbers"])
     assert abs(model.last_energy_f64.item() - float(fx["full_eval/energy"])) / n < E_TOL
-    assert np.abs(F.cpu().numpy() - fx["full_eval/forces"]).max() < F_TOL
+    assert np.abs(F.detach().cpu().numpy() - fx["full_eval/forces"]).max() < F_TOL
+    model.eval()
+    E2, F2 = model.predict_energy_and_forces(Z, pos)
+    assert not F2.requires_grad
+    assert np.abs(F2.cpu().numpy() - fx["full_eval/forces"]).max() < F_TOL
 
 
 def test_autograd_forces_and_batched_api(gpu):

@@ -1,0 +1,108 @@
+"""Hand-derived training gradients against the oracle's autograd double
+backward and central finite differences, in float64 on the CPU."""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import pkg_mod
+from oracle import painn_ref as R
+
+
+def _setup(H=16, L=2, Rb=6, rc=3.5, n_mol=3, seed=0, bias=0.2):
+    model_mod = pkg_mod("model")
+    torch.manual_seed(seed)
+    m = model_mod.StudentForceField(hidden_dim=H, num_interactions=L, num_rbf=Rb, cutoff=rc,
+                                    max_z=16)
+    gen = torch.Generator().manual_seed(seed + 7)
+    sd = {}
+    for k, v in m.state_dict().items():
+        v = v.double().clone()
+        if k.endswith(".bias"):
+            v = torch.randn(v.shape, generator=gen, dtype=torch.float64) * bias
+        sd[k] = v
+    rng = np.random.default_rng(seed)
+    Zs, Ps = [], []
+    for b in range(n_mol):
+        n = int(rng.integers(4, 9))
+        Zs.append(rng.choice([1, 6, 7, 8, 16], size=n))
+        Ps.append(rng.normal(size=(n, 3)) * 1.3 + np.array([30.0 * b, 0, 0]))
+    Z = torch.tensor(np.concatenate(Zs))
+    x = torch.tensor(np.concatenate(Ps))
+    batch = torch.tensor(np.repeat(np.arange(n_mol), [len(z) for z in Zs]))
+    return sd, Z, x, batch, rc
+
+
+def _graph(x, batch, rc, n_mol):
+    train = pkg_mod("train")
+    ei = R.radius_graph_dense(x, rc, batch)
+    keep = ei[1] > ei[0]                       # src > dst: one entry per unordered pair
+    lo, hi = ei[1][keep], ei[0][keep]
+    return train.PairGraph(lo, hi, None, batch, n_mol)
+
+
+def test_forward_and_forces_match_oracle():
+    train = pkg_mod("train")
+    sd, Z, x, batch, rc = _setup()
+    g = _graph(x, batch, rc, 3)
+    E, cache = train.forward(sd, Z, x, g, rc)
+    _, F, _ = train.reverse(sd, cache, g, torch.ones_like(E))
+    Eref, Fref = R.energy_and_forces(sd, Z.numpy(), x.numpy(), cutoff=rc, batch=batch.numpy())
+    assert torch.allclose(E, Eref, atol=1e-12, rtol=0)
+    assert torch.allclose(F, Fref, atol=1e-11, rtol=0)
+
+
+def _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, wE, wF):
+    p = {k: v.clone().requires_grad_(True) for k, v in sd.items()}
+    pos = x.clone().requires_grad_(True)
+    E = R.energy(p, Z, pos, rc, batch=batch, n_structures=int(batch.max()) + 1)
+    F = -torch.autograd.grad(E.sum(), pos, create_graph=True)[0]
+    loss = wE * ((E - El) ** 2).mean() + wF * ((F - Fl) ** 2).mean()
+    names = [k for k in p if p[k].requires_grad and k not in ("rbf.centers", "rbf.widths")]
+    gr = torch.autograd.grad(loss, [p[k] for k in names], allow_unused=True)
+    return loss.detach(), dict(zip(names, gr))
+
+
+def test_loss_gradient_matches_autograd_double_backward():
+    train = pkg_mod("train")
+    sd, Z, x, batch, rc = _setup()
+    g = _graph(x, batch, rc, 3)
+    gen = torch.Generator().manual_seed(5)
+    El = torch.randn(3, generator=gen, dtype=torch.float64)
+    Fl = torch.randn(x.shape, generator=gen, dtype=torch.float64)
+    loss, E, F, grads = train.force_matching_grads(sd, Z, x, g, rc, El, Fl, 1.0, 100.0)
+    loss_ref, ref = _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, 1.0, 100.0)
+    assert abs(loss.item() - loss_ref.item()) < 1e-10 * max(1, abs(loss_ref.item()))
+    for k, r in ref.items():
+        if r is None:
+            continue
+        assert k in grads, k
+        err = (grads[k] - r).abs().max().item()
+        scale = max(1.0, r.abs().max().item())
+        assert err < 1e-9 * scale, (k, err, scale)
+
+
+def test_loss_gradient_central_differences():
+    """Pins the derivation independently of autograd."""
+    train = pkg_mod("train")
+    sd, Z, x, batch, rc = _setup(H=8, L=2, Rb=4, n_mol=2, seed=3)
+    g = _graph(x, batch, rc, 2)
+    El = torch.tensor([0.3, -0.2], dtype=torch.float64)
+    Fl = torch.zeros(x.shape, dtype=torch.float64)
+    _, _, _, grads = train.force_matching_grads(sd, Z, x, g, rc, El, Fl, 1.0, 100.0)
+    picks = [("interactions.0.message.rbf_to_scalar.2.weight", (3, 2)),
+             ("interactions.1.update.update_mlp.0.weight", (1, 5)),
+             ("interactions.0.update.mixing_matrix", (2, 0)),
+             ("energy_head.2.bias", (1,)),
+             ("embedding.weight", (int(Z[0]), 4))]
+    h = 1e-6
+    for name, idx in picks:
+        vals = []
+        for sgn in (1, -1):
+            q = {k: v.clone() for k, v in sd.items()}
+            q[name][idx] += sgn * h
+            loss, _, _, _ = train.force_matching_grads(q, Z, x, g, rc, El, Fl, 1.0, 100.0)
+            vals.append(loss.item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - grads[name][idx].item()) < 1e-6 * max(1.0, abs(fd)), (name, fd,
+                                                                            grads[name][idx].item())
