@@ -211,6 +211,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--stage-reps", type=int, default=20)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied by a plain device copy at the end (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every box here has one "
@@ -251,7 +254,7 @@ def main():
     ef_s = a_ev.elapsed_time(b_ev) / 1e3 / reps
     log(f"static E+F at t=0: {ef_s * 1e3:.3f} ms ({n_atoms / ef_s:.4g} atom-evals/s), "
         f"E={E0}, P={P0}")
-    st = stage_times(md)
+    st = stage_times(md, reps=args.stage_reps)
     for k, v in st.items():
         log(f"  {k:12s} {v['avg_s'] * 1e6:9.1f} us/launch x{v['launches_per_step']}")
     log("warmup")
@@ -299,6 +302,22 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": None,
                 "algorithmic_per_launch": nbytes, "avg_launch_us": st[dom]["avg_s"] * 1e6}
+
+    traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            tr = json.load(fh)
+        if tr.get("kernel") == dom and abs(tr.get("pairs_t0", -1) - P0) <= 0.01 * P0:
+            roof["traffic"] = tr["traffic_bytes"]
+            roof["traffic_source"] = "profiles/pmc_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+
+    if args.copy_probe:
+        nb = args.copy_probe * (1 << 20)
+        src = torch.empty(nb, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nb} bytes read + {nb} bytes written")
 
     plain = None
     cpu = None

@@ -119,6 +119,50 @@ __device__ __forceinline__ int xcd_block() {
   return (b % 8) * (G / 8) + b / 8;
 }
 
+// Same product with K fixed at compile time: the slab loop unrolls fully and
+// the weight fetches run PF slabs ahead of the MFMAs that consume them.
+template <int RT, int NT, int K, int PF = 2>
+__device__ __forceinline__ void mma_tile_k(const float* __restrict__ A, int lda,
+                                           const float* __restrict__ B, int ldb,
+                                           const int (&cb)[NT], f4 (&acc)[RT][NT]) {
+  static_assert(K % 16 == 0, "K must be a multiple of 16");
+  constexpr int S = K / 16;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const float* arow = A + r * lda + 4 * g;
+  const float* brow[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) brow[ct] = B + (size_t)(cb[ct] + r) * ldb + 4 * g;
+  f4 b[PF + 1][NT];
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct)
+      if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(brow[ct] + 16 * q);
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    if (sl + PF < S) {
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct)
+        b[(sl + PF) % (PF + 1)][ct] = *reinterpret_cast<const f4*>(brow[ct] + 16 * (sl + PF));
+    }
+    f4 a[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+      a[rt] = *reinterpret_cast<const f4*>(arow + rt * 16 * lda + 16 * sl);
+    const int cur = sl % (PF + 1);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[cur][ct][s], acc[rt][ct],
+                                                             0, 0, 0);
+  }
+}
+
 template <int RT, int NT>
 __device__ __forceinline__ void zero_acc(f4 (&acc)[RT][NT]) {
 #pragma unroll
@@ -127,17 +171,24 @@ __device__ __forceinline__ void zero_acc(f4 (&acc)[RT][NT]) {
     for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
 }
 
-// Contiguous range of work tiles for a persistent block.  Workgroups are
-// dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8); the
-// logical index below gives the blocks of one XCD consecutive tile ranges so
-// neighbouring tiles (same molecules, same node features) share that XCD's L2.
-__device__ __forceinline__ void persistent_range(int ntiles, int& t0, int& t1) {
+// Work tiles of a persistent block: tiles t0, t0 + step, ... below t1.
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Each XCD owns one contiguous eighth of the tiles, and its blocks
+// stride through that eighth together, so at any moment an XCD works on a
+// few neighbouring tiles (the same molecules, the same node features) that
+// fit its L2, instead of one far-apart range per block.
+__device__ __forceinline__ void persistent_range(int ntiles, int& t0, int& t1, int& step) {
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int logical = (G % 8 == 0) ? ((b % 8) * (G / 8) + b / 8) : b;
-  const int per = (ntiles + G - 1) / G;
-  t0 = logical * per;
-  t1 = min(ntiles, t0 + per);
+  if (G % 8 != 0) {
+    t0 = b; t1 = ntiles; step = G;
+    return;
+  }
+  const int per_xcd = (ntiles + 7) / 8;
+  const int x = b % 8, k = b / 8;
+  t0 = x * per_xcd + k;
+  t1 = min(ntiles, (x + 1) * per_xcd);
+  step = G / 8;
 }
 
 // Sum over aligned groups of `width` lanes (width a power of two <= 64).

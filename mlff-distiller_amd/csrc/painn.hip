@@ -65,13 +65,14 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
   const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
   const int ntiles = (P + TM - 1) / TM;
   int t0, t1;
-  persistent_range(ntiles, t0, t1);
+  int tstep;
+  persistent_range(ntiles, t0, t1, tstep);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const float* f1w = w + lw.f1w;
   const float* f1b = w + lw.f1b;
   const float* f2w = w + lw.f2w;
   const float* f2b = w + lw.f2b;
-  for (int tile = t0; tile < t1; ++tile) {
+  for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
     for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT2; ++ct) cb[ct] = (wave * NT2 + ct) * 16;
       zero_acc(acc);
-      mma_tile<RT, NT2>(h_lds, LDH, f2w, H, H, cb, acc);
+      mma_tile_k<RT, NT2, H>(h_lds, LDH, f2w, H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -138,52 +139,49 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
     const int32_t* __restrict__ edge_pair, int32_t ecap, const float4* __restrict__ geom,
     const float* __restrict__ Wf, const float* __restrict__ s_in, const float* __restrict__ v_in,
     float* __restrict__ s_out, float* __restrict__ v_out) {
-  constexpr int LPR = H / 2;
+  constexpr int LPR = H / 4;
   constexpr int RPB = kBlock / LPR;
   const int sub = threadIdx.x % LPR;
   const int i = xcd_block() * RPB + threadIdx.x / LPR;
   if (i >= N) return;
-  const int c = 2 * sub;
-  float2 as = make_float2(0.f, 0.f), a0 = as, a1 = as, a2 = as;
+  const int c = 4 * sub;
+  f4 as = {0.f, 0.f, 0.f, 0.f}, a0 = as, a1 = as, a2 = as;
   const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
   for (int e = e0; e < e1; ++e) {
     const int j = col[e];
     const int p = edge_pair[e];
     if (p < 0) continue;
-    const float* wp = Wf + (size_t)p * 3 * H;
-    const float2 ws = *reinterpret_cast<const float2*>(wp + c);
-    const float2 wv1 = *reinterpret_cast<const float2*>(wp + H + c);
-    const float2 sj = *reinterpret_cast<const float2*>(s_in + (size_t)j * H + c);
-    const float* vj = v_in + (size_t)j * 3 * H;
-    const float2 v0 = *reinterpret_cast<const float2*>(vj + c);
-    const float2 v1 = *reinterpret_cast<const float2*>(vj + H + c);
-    const float2 v2 = *reinterpret_cast<const float2*>(vj + 2 * H + c);
-    as.x += sj.x * ws.x; as.y += sj.y * ws.y;
-    a0.x += v0.x * wv1.x; a0.y += v0.y * wv1.y;
-    a1.x += v1.x * wv1.x; a1.y += v1.y * wv1.y;
-    a2.x += v2.x * wv1.x; a2.y += v2.y * wv1.y;
+    const float* wp = Wf + (size_t)p * 3 * H + c;
+    const float* vj = v_in + (size_t)j * 3 * H + c;
+    const f4 ws = *reinterpret_cast<const f4*>(wp);
+    const f4 wv1 = *reinterpret_cast<const f4*>(wp + H);
+    const f4 sj = *reinterpret_cast<const f4*>(s_in + (size_t)j * H + c);
+    const f4 v0 = *reinterpret_cast<const f4*>(vj);
+    const f4 v1 = *reinterpret_cast<const f4*>(vj + H);
+    const f4 v2 = *reinterpret_cast<const f4*>(vj + 2 * H);
+    as += sj * ws;
+    a0 += v0 * wv1;
+    a1 += v1 * wv1;
+    a2 += v2 * wv1;
     if (FWD) {
-      const float2 wv2 = *reinterpret_cast<const float2*>(wp + 2 * H + c);
+      const f4 wv2 = *reinterpret_cast<const f4*>(wp + 2 * H);
       const float4 gm = geom[p];
       const float den = gm.w + kEdgeEps;
       // u_ji = (p_j - p_i)/(d + eps): +pair vector when j is the pair's hi atom
       float ux = gm.x / den, uy = gm.y / den, uz = gm.z / den;
       if (j < i) { ux = -ux; uy = -uy; uz = -uz; }
-      a0.x += ux * wv2.x; a0.y += ux * wv2.y;
-      a1.x += uy * wv2.x; a1.y += uy * wv2.y;
-      a2.x += uz * wv2.x; a2.y += uz * wv2.y;
+      a0 += ux * wv2;
+      a1 += uy * wv2;
+      a2 += uz * wv2;
     }
   }
-  const float2 si = *reinterpret_cast<const float2*>(s_in + (size_t)i * H + c);
-  const float* vi = v_in + (size_t)i * 3 * H;
-  const float2 w0 = *reinterpret_cast<const float2*>(vi + c);
-  const float2 w1 = *reinterpret_cast<const float2*>(vi + H + c);
-  const float2 w2 = *reinterpret_cast<const float2*>(vi + 2 * H + c);
-  *reinterpret_cast<float2*>(s_out + (size_t)i * H + c) = make_float2(si.x + as.x, si.y + as.y);
-  float* vo = v_out + (size_t)i * 3 * H;
-  *reinterpret_cast<float2*>(vo + c) = make_float2(w0.x + a0.x, w0.y + a0.y);
-  *reinterpret_cast<float2*>(vo + H + c) = make_float2(w1.x + a1.x, w1.y + a1.y);
-  *reinterpret_cast<float2*>(vo + 2 * H + c) = make_float2(w2.x + a2.x, w2.y + a2.y);
+  const float* vi = v_in + (size_t)i * 3 * H + c;
+  float* vo = v_out + (size_t)i * 3 * H + c;
+  *reinterpret_cast<f4*>(s_out + (size_t)i * H + c) =
+      *reinterpret_cast<const f4*>(s_in + (size_t)i * H + c) + as;
+  *reinterpret_cast<f4*>(vo) = *reinterpret_cast<const f4*>(vi) + a0;
+  *reinterpret_cast<f4*>(vo + H) = *reinterpret_cast<const f4*>(vi + H) + a1;
+  *reinterpret_cast<f4*>(vo + 2 * H) = *reinterpret_cast<const f4*>(vi + 2 * H) + a2;
 }
 
 // ---------------------------------------------------------------------------
@@ -220,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void update_fwd_kernel(
 #pragma unroll
     for (int ct = 0; ct < NT1; ++ct) cb[ct] = (wave * NT1 + ct) * 16;
     zero_acc(acc);
-    mma_tile<RT, NT1>(x_lds, LDX, w + lw.u1w, 2 * H, 2 * H, cb, acc);
+    mma_tile_k<RT, NT1, 2 * H>(x_lds, LDX, w + lw.u1w, 2 * H, cb, acc);
     const float* b1 = w + lw.u1b;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -245,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void update_fwd_kernel(
 #pragma unroll
       for (int sp = 0; sp < 3; ++sp) cb[sp * NTC + q] = sp * H + (wave * NTC + q) * 16;
     zero_acc(acc);
-    mma_tile<RT, NT2>(h_lds, LDH, w + lw.u2w, H, H, cb, acc);
+    mma_tile_k<RT, NT2, H>(h_lds, LDH, w + lw.u2w, H, cb, acc);
     const float* b2 = w + lw.u2b;
     const float* U = w + lw.mix;
     float u[9];
@@ -304,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
     int cb[1] = {wave * 16};
     f4 acc[RT][1];
     zero_acc(acc);
-    mma_tile<RT, 1>(x_lds, LDX, w + mw.r1w, H, H, cb, acc);
+    mma_tile_k<RT, 1, H>(x_lds, LDX, w + mw.r1w, H, cb, acc);
     const int c = cb[0] + r;
     const float bias = w[mw.r1b + c];
 #pragma unroll
@@ -322,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
     int cb[1] = {wave * 16};
     f4 acc[RT][1];
     zero_acc(acc);
-    mma_tile<RT, 1>(h1_lds, LD2, w + mw.r2w, HP2, HP2, cb, acc);
+    mma_tile_k<RT, 1, HP2>(h1_lds, LD2, w + mw.r2w, HP2, cb, acc);
     const int c = cb[0] + r;
     const float bias = w[mw.r2b + c];
 #pragma unroll
@@ -347,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
     int cb[1] = {wave * 16};
     f4 acc[RT][1];
     zero_acc(acc);
-    mma_tile<RT, 1>(g2_lds, LD4, w + mw.r2t, HP4, HP4, cb, acc);
+    mma_tile_k<RT, 1, HP4>(g2_lds, LD4, w + mw.r2t, HP4, cb, acc);
     const int c = cb[0] + r;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -366,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
       zero_acc(acc);
-      mma_tile<RT, NT>(h1_lds, LD2, w + mw.r1t, HP2, HP2, cb, acc);
+      mma_tile_k<RT, NT, HP2>(h1_lds, LD2, w + mw.r1t, HP2, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -427,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
       zero_acc(acc);
-      mma_tile<RT, NT>(y_lds, LDY, w + lw.u2t, 3 * H, 3 * H, cb, acc);
+      mma_tile_k<RT, NT, 3 * H>(y_lds, LDY, w + lw.u2t, 3 * H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -450,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
       zero_acc(acc);
-      mma_tile<RT, NT>(a_lds, LDA, w + lw.u1t, H, H, cb, acc);
+      mma_tile_k<RT, NT, H>(a_lds, LDA, w + lw.u1t, H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -505,8 +503,8 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
     float4* __restrict__ gvec, int first, int last) {
   constexpr int TM = 32, RT = 2;
   constexpr int LDE = 64 + 4, LDG = 3 * H + 4, LDA = H + 4;
-  constexpr int GW = H < 64 ? H : 64;          // lanes sharing one pair in the gather
-  constexpr int NPART = H / GW;                // partial sums per pair
+  constexpr int C4 = H / 4;                    // lanes per pair in the gather (4 channels each)
+  constexpr int NPART = 1;                     // one dE/du partial per pair
   __shared__ __align__(16) float e_lds[TM * LDE];
   __shared__ __align__(16) float gw_lds[TM * LDG];
   __shared__ __align__(16) float ga_lds[TM * LDA];
@@ -514,9 +512,10 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
   const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
   const int ntiles = (P + TM - 1) / TM;
   int t0, t1;
-  persistent_range(ntiles, t0, t1);
+  int tstep;
+  persistent_range(ntiles, t0, t1, tstep);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  for (int tile = t0; tile < t1; ++tile) {
+  for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
     for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP, p = pbase + m;
@@ -524,42 +523,56 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
       if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
       e_lds[m * LDE + k] = val;
     }
-    for (int idx = threadIdx.x; idx < TM * H; idx += kPairBlock) {
-      const int m = idx / H, c = idx - m * H, p = pbase + m;
-      float gws = 0.f, gwv1 = 0.f, gwv2 = 0.f, gu0 = 0.f, gu1 = 0.f, gu2 = 0.f;
+    // gather both endpoints, four channels per lane, every load issued up front
+    for (int idx = threadIdx.x; idx < TM * C4; idx += kPairBlock) {
+      const int m = idx / C4, c = 4 * (idx - m * C4), p = pbase + m;
+      f4 gws = {0.f, 0.f, 0.f, 0.f}, gwv1 = gws, gwv2 = gws;
+      float gu0 = 0.f, gu1 = 0.f, gu2 = 0.f;
       if (p < P) {
         const int a = pair_lo[p], b = pair_hi[p];
         const float4 gm = geom[p];
+        const float* va = v_in + (size_t)a * 3 * H + c;
+        const float* vb = v_in + (size_t)b * 3 * H + c;
+        const float* qa = gv1 + (size_t)a * 3 * H + c;
+        const float* qb = gv1 + (size_t)b * 3 * H + c;
+        const f4 sa = *reinterpret_cast<const f4*>(s_in + (size_t)a * H + c);
+        const f4 sb = *reinterpret_cast<const f4*>(s_in + (size_t)b * H + c);
+        const f4 ga = *reinterpret_cast<const f4*>(gs1 + (size_t)a * H + c);
+        const f4 gb = *reinterpret_cast<const f4*>(gs1 + (size_t)b * H + c);
+        const f4 va0 = *reinterpret_cast<const f4*>(va);
+        const f4 va1 = *reinterpret_cast<const f4*>(va + H);
+        const f4 va2 = *reinterpret_cast<const f4*>(va + 2 * H);
+        const f4 vb0 = *reinterpret_cast<const f4*>(vb);
+        const f4 vb1 = *reinterpret_cast<const f4*>(vb + H);
+        const f4 vb2 = *reinterpret_cast<const f4*>(vb + 2 * H);
+        const f4 qa0 = *reinterpret_cast<const f4*>(qa);
+        const f4 qa1 = *reinterpret_cast<const f4*>(qa + H);
+        const f4 qa2 = *reinterpret_cast<const f4*>(qa + 2 * H);
+        const f4 qb0 = *reinterpret_cast<const f4*>(qb);
+        const f4 qb1 = *reinterpret_cast<const f4*>(qb + H);
+        const f4 qb2 = *reinterpret_cast<const f4*>(qb + 2 * H);
+        const f4 wv2 = *reinterpret_cast<const f4*>(Wf + (size_t)p * 3 * H + 2 * H + c);
         const float den = gm.w + kEdgeEps;
         const float ux = gm.x / den, uy = gm.y / den, uz = gm.z / den;
-        gws = gs1[(size_t)a * H + c] * s_in[(size_t)b * H + c] +
-              gs1[(size_t)b * H + c] * s_in[(size_t)a * H + c];
-        const float* va = v_in + (size_t)a * 3 * H;
-        const float* vb = v_in + (size_t)b * 3 * H;
-        const float* qa = gv1 + (size_t)a * 3 * H;
-        const float* qb = gv1 + (size_t)b * 3 * H;
-        const float qa0 = qa[c], qa1 = qa[H + c], qa2 = qa[2 * H + c];
-        const float qb0 = qb[c], qb1 = qb[H + c], qb2 = qb[2 * H + c];
-        gwv1 = qa0 * vb[c] + qa1 * vb[H + c] + qa2 * vb[2 * H + c] + qb0 * va[c] +
-               qb1 * va[H + c] + qb2 * va[2 * H + c];
-        const float d0 = qa0 - qb0, d1 = qa1 - qb1, d2 = qa2 - qb2;
+        gws = ga * sb + gb * sa;
+        gwv1 = qa0 * vb0 + qa1 * vb1 + qa2 * vb2 + qb0 * va0 + qb1 * va1 + qb2 * va2;
+        const f4 d0 = qa0 - qb0, d1 = qa1 - qb1, d2 = qa2 - qb2;
         gwv2 = ux * d0 + uy * d1 + uz * d2;
-        const float wv2 = Wf[(size_t)p * 3 * H + 2 * H + c];
-        gu0 = wv2 * d0;
-        gu1 = wv2 * d1;
-        gu2 = wv2 * d2;
+        const f4 w0 = wv2 * d0, w1 = wv2 * d1, w2 = wv2 * d2;
+        gu0 = (w0[0] + w0[1]) + (w0[2] + w0[3]);
+        gu1 = (w1[0] + w1[1]) + (w1[2] + w1[3]);
+        gu2 = (w2[0] + w2[1]) + (w2[2] + w2[3]);
       }
-      gw_lds[m * LDG + c] = gws;
-      gw_lds[m * LDG + H + c] = gwv1;
-      gw_lds[m * LDG + 2 * H + c] = gwv2;
-      gu0 = group_sum<GW>(gu0);
-      gu1 = group_sum<GW>(gu1);
-      gu2 = group_sum<GW>(gu2);
-      if ((c % GW) == 0) {
-        const int part = c / GW;
-        gu_lds[(m * NPART + part) * 3 + 0] = gu0;
-        gu_lds[(m * NPART + part) * 3 + 1] = gu1;
-        gu_lds[(m * NPART + part) * 3 + 2] = gu2;
+      *reinterpret_cast<f4*>(gw_lds + m * LDG + c) = gws;
+      *reinterpret_cast<f4*>(gw_lds + m * LDG + H + c) = gwv1;
+      *reinterpret_cast<f4*>(gw_lds + m * LDG + 2 * H + c) = gwv2;
+      gu0 = group_sum<C4>(gu0);
+      gu1 = group_sum<C4>(gu1);
+      gu2 = group_sum<C4>(gu2);
+      if (c == 0) {
+        gu_lds[m * 3 + 0] = gu0;
+        gu_lds[m * 3 + 1] = gu1;
+        gu_lds[m * 3 + 2] = gu2;
       }
     }
     __syncthreads();
@@ -573,7 +586,7 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
         zero_acc(pre);
         zero_acc(acc);
         mma_tile<RT, NT>(e_lds, LDE, w + lw.f1w, RP, RP, cb, pre);
-        mma_tile<RT, NT>(gw_lds, LDG, w + lw.f2t, 3 * H, 3 * H, cb, acc);
+        mma_tile_k<RT, NT, 3 * H>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
         const float* b1 = w + lw.f1b;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -594,7 +607,7 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
       int cb[1] = {wave * 16};
       f4 acc[RT][1];
       zero_acc(acc);
-      mma_tile<RT, 1>(ga_lds, LDA, w + lw.f1t, H, H, cb, acc);
+      mma_tile_k<RT, 1, H>(ga_lds, LDA, w + lw.f1t, H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -722,7 +735,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
     embed_kernel<<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         N, H, b.numbers, b.w + mw.emb, b.S, b.V);
   }
-  const int rows_per_block = kBlock / (H / 2);
+  const int rows_per_block = kBlock / (H / 4);
   const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   for (int l = 0; l < b.L; ++l) {
@@ -751,7 +764,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
   const int N = b.N;
   const float* centers = b.w + mw.centers;
   const float* widths = b.w + mw.widths;
-  const int rows_per_block = kBlock / (H / 2);
+  const int rows_per_block = kBlock / (H / 4);
   const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   // gradient w.r.t. the update outputs of the last block: (gS0, 0)
@@ -787,7 +800,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   const int N = b.N;
   const float* centers = b.w + mw.centers;
   const float* widths = b.w + mw.widths;
-  const int rows_per_block = kBlock / (H / 2);
+  const int rows_per_block = kBlock / (H / 4);
   const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
   const LayerW& lw = mw.layer[l];
