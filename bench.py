@@ -199,8 +199,236 @@ def stage_times(md, reps=20):
     return out
 
 
+# ---------------------------------------------------------------------------
+# the other configurations (one JSON line each)
+# ---------------------------------------------------------------------------
+
+def _common(metric, value, unit, steps, warmup, seconds, dtype, workload, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": seconds / max(steps, 1) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded structures, random-init weights)",
+           "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def config1(args, pkg, synth, mdmod, dev):
+    """One 60-atom molecule, 1000 NVE steps at 0.5 fs (seed 1, weights seed 0)."""
+    torch.manual_seed(0)
+    model = pkg.StudentForceField().to(dev).eval()
+    mol = synth.single_molecule(60, seed=1)
+    steps = args.steps if args.steps != 500 else 1000
+    md = mdmod.BatchedNVE(model, [mol], seed=1, max_steps=steps + args.warmup + 1)
+    md.run(args.warmup, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(steps, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = 60 * steps / wall
+    sd_cpu = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    df, de = accuracy(sd_cpu, md, 1)
+    s = md.summary()
+    sd_dev = {k: v.detach() for k, v in model.state_dict().items()}
+    plain = plain_md_rate(sd_dev, 128, 3, [mol], synth.masses_of, 0.5 * synth.FS, 50, dev)
+    cpu_steps = args.cpu_steps if args.cpu_steps != 2 else 300
+    cpu = cpu_baseline(sd_cpu, [mol], synth.masses_of, 0.5 * synth.FS, cpu_steps)
+    return _common("atom-steps/s (E+F), single 60-atom molecule NVE", value, "atom-steps/s",
+                   steps, args.warmup, wall, "f32",
+                   "1 molecule, 60 atoms (seed 1), PaiNN H128/L3/R20 seed 0, hipGraph NVE dt 0.5 fs",
+                   accuracy={"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de},
+                   plain={"value": plain, "unit": "atom-steps/s", "speedup": value / plain},
+                   nve={"drift_per_atom_eV": float(s["energy_drift_per_atom"][0]),
+                        "drift_pct": float(s["energy_drift_pct"][0])},
+                   cpu_baseline={"value": cpu, "unit": "atom-steps/s",
+                                 "cores": torch.get_num_threads(), "kind": "port",
+                                 "sample": f"the same molecule, {cpu_steps} MD steps"})
+
+
+def config3(args, pkg, synth, mdmod, dev):
+    """1000 waters (3000 atoms) in a periodic box, 10 ps NVE with drift and stress."""
+    from oracle import nl_ref
+    from oracle import painn_ref as R
+    torch.manual_seed(0)
+    model = pkg.StudentForceField().to(dev).eval()
+    box = synth.water_box(10, seed=3)
+    steps = args.steps if args.steps != 500 else 20000
+    md = mdmod.BatchedNVE(model, [box], seed=3, max_steps=steps + args.warmup + 1)
+    md.run(args.warmup, graph=True)
+    torch.cuda.synchronize()
+    c = md.plan.read_counts()
+    t0 = time.perf_counter()
+    md.run(steps, graph=True, steps_per_graph=20)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = 3000 * steps / wall
+    s = md.summary()
+    # accuracy and virial at the final positions vs the float64 oracle on the same edge set
+    P32 = md.pos32.cpu().numpy()
+    rp, cl, sh = nl_ref.neighbor_csr(P32, np.array([0, 3000]), 5.0, box.cell[None], box.pbc[None])
+    ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(3000), np.diff(rp))])
+    sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    t1 = time.perf_counter()
+    E, F, W = R.energy_and_forces(sd, box.numbers, P32.astype(np.float64), edge_index=ei,
+                                  shifts=sh, cell=box.cell.astype(np.float32).astype(np.float64),
+                                  virial=True)
+    e64, f, vir = model.energy_forces_virial(torch.tensor(box.numbers, device=dev), md.pos32,
+                                             torch.tensor(box.cell), torch.tensor(box.pbc))
+    df = float(np.abs(f.cpu().numpy() - F.numpy()).max())
+    de = abs(e64.item() - E.item()) / 3000
+    dvir = float(np.abs(vir.cpu().numpy()[0] - W.numpy()[0]).max())
+    # CPU baseline: float32 oracle E+F on the host (same edge set), a few evaluations
+    sd32 = {k: v.float() for k, v in sd.items()}
+    t2 = time.perf_counter()
+    nrep = 3
+    for _ in range(nrep):
+        R.energy_and_forces(sd32, box.numbers, P32.astype(np.float64), edge_index=ei, shifts=sh,
+                            cell=box.cell, dtype=torch.float32)
+    cpu_rate = 3000 * nrep / (time.perf_counter() - t2)
+    vol = abs(np.linalg.det(box.cell))
+    return _common("atom-steps/s (E+F), periodic water box NVE", value, "atom-steps/s", steps,
+                   args.warmup, wall, "f32",
+                   "1000 waters (3000 atoms, seed 3) in a 31 A periodic box, PaiNN H128/L3/R20 "
+                   "seed 0, cell-free minimum image, hipGraph NVE dt 0.5 fs",
+                   edges_start=int(c[0]),
+                   accuracy={"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de,
+                             "max_abs_dvirial_eV": dvir},
+                   nve={"simulated_ps": steps * 0.5e-3,
+                        "drift_per_atom_eV": float(s["energy_drift_per_atom"][0]),
+                        "drift_rate_eV_per_atom_ps": float(s["drift_rate_eV_per_atom_ps"][0]),
+                        "drift_pct": float(s["energy_drift_pct"][0]),
+                        "avg_temperature_K": float(s["avg_temperature"][0])},
+                   pressure_eV_per_A3=float(-np.trace(W.numpy()[0]) / (3 * vol)),
+                   cpu_baseline={"value": cpu_rate, "unit": "atom-evals/s (E+F only)",
+                                 "cores": torch.get_num_threads(), "kind": "port",
+                                 "sample": f"{nrep} float32 E+F evaluations of the final frame"})
+
+
+def config4(args, pkg, synth, dev):
+    """Force-matching training step: 64 molecules of 20-60 atoms (seed 4),
+    labels from the float64 model with seed-1 weights, loss E-MSE + 100 F-MSE."""
+    from oracle import painn_ref as R
+    train = importlib.import_module("mlff-distiller_amd.train")
+    torch.manual_seed(1)
+    teacher = pkg.StudentForceField()
+    torch.manual_seed(0)
+    student = pkg.StudentForceField().to(dev)
+    mols = synth.molecule_set(64, 20, 60, seed=4)
+    Z, P, mp = synth.flatten(mols)
+    sizes = np.diff(mp)
+    b = np.repeat(np.arange(64), sizes)
+    P32 = P.astype(np.float32)
+    sdt = {k: v.double() for k, v in teacher.state_dict().items()}
+    El, Fl = R.energy_and_forces(sdt, Z, P32.astype(np.float64), batch=b)
+    Zt = torch.tensor(Z, device=dev)
+    Pt = torch.tensor(P32, device=dev)
+    Elt, Flt = El.float().to(dev), Fl.float().to(dev)
+    tr = train.ForceMatchingTrainer(student, lr=1e-4)
+    # gradient check against float64 autograd on the starting weights
+    loss, _, _, grads = tr.loss_and_grads(Zt, Pt, sizes, Elt, Flt)
+    sd = {k: v.detach().cpu().double().requires_grad_(True) for k, v in student.state_dict().items()}
+    pos = torch.tensor(P32, dtype=torch.float64, requires_grad=True)
+    E = R.energy(sd, torch.tensor(Z), pos, 5.0, batch=torch.tensor(b), n_structures=64)
+    F = -torch.autograd.grad(E.sum(), pos, create_graph=True)[0]
+    Lref = ((E - El) ** 2).mean() + 100.0 * ((F - Fl) ** 2).mean()
+    names = [k for k in sd if not k.startswith("rbf.")]
+    gref = torch.autograd.grad(Lref, [sd[k] for k in names], allow_unused=True)
+    rel = max(((grads[k].double().cpu() - g).norm() / (g.norm() + 1e-30)).item()
+              for k, g in zip(names, gref) if g is not None)
+    steps = args.steps if args.steps != 500 else 50
+    for _ in range(args.warmup if args.warmup != 20 else 3):
+        tr.step(Zt, Pt, sizes, Elt, Flt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tr.step(Zt, Pt, sizes, Elt, Flt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = len(Z)
+    # CPU baseline: float32 autograd double backward of the oracle, one step
+    sd32 = {k: v.detach().cpu().float().requires_grad_(True)
+            for k, v in student.state_dict().items()}
+    t1 = time.perf_counter()
+    pos32 = torch.tensor(P32, requires_grad=True)
+    E = R.energy(sd32, torch.tensor(Z), pos32, 5.0, batch=torch.tensor(b), n_structures=64)
+    F = -torch.autograd.grad(E.sum(), pos32, create_graph=True)[0]
+    L = ((E - El.float()) ** 2).mean() + 100.0 * ((F - Fl.float()) ** 2).mean()
+    L.backward()
+    cpu_s = time.perf_counter() - t1
+    return _common("force-matching training atoms/s (E+F loss, exact weight gradient)",
+                   n * steps / wall, "atoms/s", steps, 3, wall, "f32",
+                   "64 molecules N~U{20..60} (seed 4), PaiNN H128/L3/R20 student seed 0, "
+                   "labels from the float64 seed-1 model, loss MSE(E) + 100 MSE(F), AdamW",
+                   n_atoms=n, train_steps_per_s=steps / wall,
+                   accuracy={"max_rel_grad_err_vs_fp64_autograd": rel,
+                             "loss_fp32": float(loss), "loss_fp64": Lref.item()},
+                   cpu_baseline={"value": n / cpu_s, "unit": "atoms/s", "cores":
+                                 torch.get_num_threads(), "kind": "port",
+                                 "sample": "one float32 autograd double-backward step of the "
+                                           "oracle on the same batch"})
+
+
+def config5(args, pkg, synth, mdmod, dev):
+    """Compact student trained on full-model MD frames, then MTS vs full MD."""
+    mts = importlib.import_module("mlff-distiller_amd.mts")
+    torch.manual_seed(0)
+    full = pkg.StudentForceField().to(dev).eval()
+    t0 = time.perf_counter()
+    frames = mts.md_frames(full, synth.molecule_set(256, 10, 100, seed=2), n_frames=args.frames,
+                           stride=5, seed=2)
+    t_frames = time.perf_counter() - t0
+    torch.manual_seed(0)
+    student = pkg.StudentForceField(hidden_dim=32, num_interactions=2, num_rbf=16).to(dev)
+    t0 = time.perf_counter()
+    hist = mts.train_student(student, frames, steps=args.train_steps, batch_molecules=64,
+                             lr=2e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    student.eval()
+    mols = synth.molecule_set(512, 10, 100, seed=5)
+    n_atoms = sum(len(m) for m in mols)
+    outer = args.mts_outer
+    ref = mdmod.BatchedNVE(full, mols, seed=5, max_steps=4 * outer + 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref.run(4 * outer, graph=True, steps_per_graph=20)
+    torch.cuda.synchronize()
+    t_full = time.perf_counter() - t0
+    m = mts.MTSNVE(full, student, mols, seed=5, threshold=args.mts_threshold,
+                   max_outer=outer + 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.run(outer, chunk=10)
+    torch.cuda.synchronize()
+    t_mts = time.perf_counter() - t0
+    sf, sm = ref.summary(), m.summary()
+    full_rate = n_atoms * 4 * outer / t_full
+    mts_rate = n_atoms * 4 * outer / t_mts
+    return _common("MTS atom-steps/s (0.5 fs equivalent) vs full-model MD", mts_rate,
+                   "atom-steps/s", 4 * outer, 0, t_mts, "f32",
+                   f"512 molecules N~U{{10..100}} (seed 5), {outer * 2e-3:.2f} ps NVE; inner "
+                   "compact student H32/L2/R16 at 0.5 fs, outer full H128/L3/R20 at 2 fs",
+                   n_atoms=n_atoms,
+                   full_md={"value": full_rate, "unit": "atom-steps/s",
+                            "max_abs_drift_eV_per_atom": float(np.abs(sf["energy_drift_per_atom"]).max()),
+                            "median_abs_drift_eV_per_atom": float(np.median(np.abs(sf["energy_drift_per_atom"])))},
+                   mts={"speedup_vs_full": mts_rate / full_rate,
+                        "max_abs_drift_eV_per_atom": float(np.abs(sm["energy_drift_per_atom"]).max()),
+                        "median_abs_drift_eV_per_atom": float(np.median(np.abs(sm["energy_drift_per_atom"]))),
+                        "fallback_structure_steps": sm["fallback_structure_steps"],
+                        "replayed_chunks": sm["replays"], "threshold_eV_per_A": args.mts_threshold},
+                   student={"frames": frames.n_frames, "frame_seconds": t_frames,
+                            "train_steps": args.train_steps, "train_seconds": t_train,
+                            "loss_first": hist[0], "loss_last_mean10": float(np.mean(hist[-10:]))})
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--frames", type=int, default=80)
+    ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--mts-outer", type=int, default=5000)
+    ap.add_argument("--mts-threshold", type=float, default=5.0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
@@ -224,6 +452,12 @@ def main():
     synth = importlib.import_module("mlff-distiller_amd.synth")
     mdmod = importlib.import_module("mlff-distiller_amd.md")
     dev = torch.device("cuda:0")
+    if args.config != 2:
+        fn = {1: config1, 3: config3, 5: config5}.get(args.config)
+        out = config4(args, pkg, synth, dev) if args.config == 4 else fn(args, pkg, synth, mdmod,
+                                                                          dev)
+        print(json.dumps(out), flush=True)
+        return
 
     torch.manual_seed(0)
     model = pkg.StudentForceField(hidden_dim=128, num_interactions=3, num_rbf=20, cutoff=5.0)
