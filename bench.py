@@ -439,6 +439,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by a plain device copy at the end (byte-counter calibration)")
@@ -469,6 +470,10 @@ def main():
     total_steps = args.warmup + args.steps
     md = mdmod.BatchedNVE(model, mols, timestep_fs=0.5, temperature_K=300.0, seed=args.seed,
                           max_steps=total_steps + 1)
+    tuning = md.autotune() if not args.no_autotune else None
+    if tuning:
+        log(f"autotune: persistent grid {tuning['blocks']} workgroups "
+            f"{ {k: round(v, 1) for k, v in tuning['timings_us'].items()} }")
     # E+F alone on the starting (compact) structures, and per-kernel timings there
     c0 = md.plan.read_counts()
     E0, P0 = int(c0[0]), int(c0[1])
@@ -590,6 +595,10 @@ def main():
                    "note": "random-init forces are not restoring: molecules expand during "
                            "the run and the edge count falls over the window"},
         "static_ef_t0": {"ms": ef_s * 1e3, "atom_evals_per_s": n_atoms / ef_s},
+        "autotune": ({"persistent_blocks": tuning["blocks"],
+                      "pair_kernels_us": {str(k): round(v, 1)
+                                          for k, v in tuning["timings_us"].items()}}
+                     if tuning else None),
         "accuracy": {"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de,
                      "sample_molecules": 16},
         "plain": plain,

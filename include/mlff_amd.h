@@ -140,7 +140,8 @@ typedef struct mlff_config {
   int32_t edge_capacity;      /* directed edges */
   int32_t pair_capacity;      /* unordered pairs */
   int32_t any_periodic;       /* non-zero when some structure is periodic */
-  int32_t persistent_blocks;  /* 0 = library default (8 per XCD-aware CU group) */
+  int32_t persistent_blocks;  /* 0 = library default (2 workgroups per CU) */
+  int32_t cell_capacity;      /* cells reserved for cell-list neighbour search (0 = none) */
 } mlff_config;
 
 int mlff_plan_create(mlff_plan** plan, const mlff_config* cfg);
@@ -155,6 +156,15 @@ int mlff_plan_bind(mlff_plan* plan, void* workspace_dev, int64_t workspace_bytes
                    const float* weights_dev, const int64_t* numbers_dev,
                    const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
                    const float* cells_dev, const uint8_t* pbc_dev);
+
+/* Cell-list neighbour search for periodic structures (replaces the O(N^2)
+ * scan of radius_graph_native, student_model.py:89-109, for large boxes).
+ * grid_host [n_structures][4] = {nc0, nc1, nc2, first_cell} per structure;
+ * nc0 = 0 keeps the brute-force scan.  Every used axis needs nc >= 3 and
+ * cells at least cutoff wide (the 27-cell stencil then holds every neighbour);
+ * the cells must fit cell_capacity.  The edge set is identical to the brute
+ * force one.  Synchronous (copies the grid to the device); not for capture. */
+int mlff_plan_set_cell_grid(mlff_plan* plan, const int32_t* grid_host, int n_structures);
 
 /* Device pointer to the plan's int32 counts[MLFF_COUNT_SLOTS]. */
 int32_t* mlff_plan_counts(mlff_plan* plan);

@@ -39,7 +39,7 @@ EXPORTS = (
     "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
     "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run", "mlff_launch_stage",
-    "mlff_plan_set_blocks",
+    "mlff_plan_set_blocks", "mlff_plan_set_cell_grid",
 )
 
 STAGE_NEIGHBOR, STAGE_FILTER_FWD, STAGE_MESSAGE_FWD, STAGE_UPDATE_FWD = 0, 1, 2, 3
@@ -58,7 +58,8 @@ class Config(ctypes.Structure):
                 ("cutoff", ctypes.c_float),
                 ("n_atoms", ctypes.c_int32), ("n_structures", ctypes.c_int32),
                 ("edge_capacity", ctypes.c_int32), ("pair_capacity", ctypes.c_int32),
-                ("any_periodic", ctypes.c_int32), ("persistent_blocks", ctypes.c_int32)]
+                ("any_periodic", ctypes.c_int32), ("persistent_blocks", ctypes.c_int32),
+                ("cell_capacity", ctypes.c_int32)]
 
 
 class MlffError(RuntimeError):
@@ -114,6 +115,8 @@ def _declare(lib):
     lib.mlff_launch_stage.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.mlff_plan_set_blocks.restype = ctypes.c_int
     lib.mlff_plan_set_blocks.argtypes = [vp, ctypes.c_int]
+    lib.mlff_plan_set_cell_grid.restype = ctypes.c_int
+    lib.mlff_plan_set_cell_grid.argtypes = [vp, vp, ctypes.c_int]
 
 
 def lib():

@@ -1,0 +1,85 @@
+"""Launch-shape autotuner for the persistent pair kernels.
+
+The pair kernels (filter forward and backward) run a persistent grid whose
+size trades load balance against L2 locality and occupancy.  The tuner times
+each candidate grid on the plan's own workload with HIP events (one stage
+launch per repetition, on the stream the stage runs on), keeps the fastest,
+and remembers the choice per (device, hidden size, pair-count bucket).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+from typing import Dict, Iterable, Optional
+
+import torch
+
+from . import _capi as C
+
+_CACHE: Dict[str, int] = {}
+
+
+def _key(plan) -> str:
+    name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    counts = plan.read_counts()
+    pairs = max(int(counts[C.MLFF_COUNT_PAIRS]), 1)
+    return f"{name}|H{plan.H}|L{plan.L}|P2^{int(math.log2(pairs))}"
+
+
+def default_candidates(cus: Optional[int] = None) -> Iterable[int]:
+    if cus is None:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return [cus, 2 * cus, 3 * cus, 4 * cus, 6 * cus]
+
+
+def time_stage(plan, stage: int, layer: int = 0, reps: int = 10) -> float:
+    lib = C.lib()
+    s = torch.cuda.current_stream()
+    C.check("stage", lib.mlff_launch_stage(plan.handle, s.cuda_stream, stage, layer))
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        C.check("stage", lib.mlff_launch_stage(plan.handle, s.cuda_stream, stage, layer))
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
+
+
+def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = None,
+         reps: int = 10, cache_file: Optional[str] = None) -> Dict[str, object]:
+    """Pick the persistent grid for ``plan`` (evaluated once at ``positions``).
+
+    Returns {"blocks": best, "timings_us": {blocks: us}}; the plan keeps the
+    best grid.  With ``cache_file`` the choice is read from / written to a
+    JSON file so later runs skip the search.
+    """
+    lib = C.lib()
+    B = plan.layout.n_structures
+    e = torch.empty(B, dtype=torch.float64, device=positions.device)
+    f = torch.empty(plan.layout.n_atoms, 3, dtype=torch.float32, device=positions.device)
+    plan.energy_forces(positions, e, f)
+    key = _key(plan)
+    if cache_file and os.path.exists(cache_file):
+        with open(cache_file) as fh:
+            _CACHE.update(json.load(fh))
+    if key in _CACHE:
+        C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, int(_CACHE[key])))
+        return {"blocks": int(_CACHE[key]), "timings_us": {}, "cached": True}
+    cands = sorted({(int(c) + 7) // 8 * 8 for c in (candidates or default_candidates())})
+    timings = {}
+    for blocks in cands:
+        C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, blocks))
+        t = (time_stage(plan, C.STAGE_FILTER_FWD, 0, reps) +
+             time_stage(plan, C.STAGE_FILTER_BWD, 0, reps))
+        timings[blocks] = t * 1e6
+    best = min(timings, key=timings.get)
+    C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, best))
+    _CACHE[key] = best
+    if cache_file:
+        os.makedirs(os.path.dirname(os.path.abspath(cache_file)), exist_ok=True)
+        with open(cache_file, "w") as fh:
+            json.dump(_CACHE, fh, indent=1)
+    return {"blocks": best, "timings_us": timings, "cached": False}

@@ -104,6 +104,7 @@ struct mlff_plan {
   double* x; double* p; const double* inv_mass; float* pos32; float* forces; double* energy;
   double* ledger; int ledger_len; double dt;
   hipGraph_t graph; hipGraphExec_t graph_exec; int graph_steps; hipStream_t graph_stream;
+  int4* grid_dev;
   // workspace carve-up (byte offsets)
   struct { int64_t off, bytes; } seg[32];
   int nseg;
@@ -114,7 +115,7 @@ namespace {
 enum Seg {
   S_COUNTS, S_DEG, S_UDEG, S_ROWPTR, S_PAIRPTR, S_COL, S_EPAIR, S_PLO, S_PHI, S_PSHIFT, S_GEOM,
   S_PACC, S_GVEC, S_WF, S_S, S_V, S_SMID, S_VMID, S_AUPD, S_G12, S_GS0, S_GV0, S_GS1, S_GV1,
-  S_EPS, S_VIR, S_COUNT_
+  S_EPS, S_VIR, S_GRID, S_CELLCNT, S_CELLSTART, S_CELLATOMS, S_ATOMCELL, S_TMPSHIFT, S_COUNT_
 };
 
 void carve(mlff_plan* pl) {
@@ -148,6 +149,13 @@ void carve(mlff_plan* pl) {
   sz[S_GV1] = 4 * N * 3 * H;
   sz[S_EPS] = 4 * N;
   sz[S_VIR] = 4 * 9 * N;
+  const int64_t CC = c.cell_capacity > 0 ? c.cell_capacity : 0;
+  sz[S_GRID] = CC ? 16 * (int64_t)c.n_structures : 0;
+  sz[S_CELLCNT] = 4 * CC;
+  sz[S_CELLSTART] = CC ? 4 * (CC + 1) : 0;
+  sz[S_CELLATOMS] = CC ? 4 * N : 0;
+  sz[S_ATOMCELL] = CC ? 4 * N : 0;
+  sz[S_TMPSHIFT] = CC ? 12 * E : 0;
   int64_t off = 0;
   for (int k = 0; k < S_COUNT_; ++k) {
     pl->seg[k].off = off;
@@ -317,6 +325,14 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   a.pair_lo = segp<int32_t>(pl, ws, S_PLO); a.pair_hi = segp<int32_t>(pl, ws, S_PHI);
   a.pair_shift = segp<int32_t>(pl, ws, S_PSHIFT); a.geom = segp<float4>(pl, ws, S_GEOM);
   a.pcap = b.pcap; a.counts = b.counts;
+  a.grid = nullptr;   // brute force until mlff_plan_set_cell_grid
+  a.n_cells = 0;
+  a.cell_count = segp<int32_t>(pl, ws, S_CELLCNT);
+  a.cell_start = segp<int32_t>(pl, ws, S_CELLSTART);
+  a.cell_atoms = segp<int32_t>(pl, ws, S_CELLATOMS);
+  a.atom_cell = segp<int32_t>(pl, ws, S_ATOMCELL);
+  a.edge_tmp_shift = segp<int32_t>(pl, ws, S_TMPSHIFT);
+  pl->grid_dev = segp<int4>(pl, ws, S_GRID);
   hipError_t e = hipMemset(b.counts, 0, 4 * MLFF_COUNT_SLOTS);
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
   pl->bound = true;
@@ -363,6 +379,42 @@ int mlff_launch_stage(mlff_plan* pl, void* stream, int stage, int layer) {
     e = launch_stage(pl->buf, s, stage, layer);
   }
   if (e != hipSuccess) return hip_fail(e, "mlff_launch_stage");
+  return MLFF_OK;
+}
+
+int mlff_plan_set_cell_grid(mlff_plan* pl, const int32_t* grid_host, int n_structures) {
+  if (!pl || !pl->bound) { set_error("mlff_plan_set_cell_grid: plan not bound"); return MLFF_ERR_STATE; }
+  if (!grid_host || n_structures != pl->cfg.n_structures) {
+    set_error("mlff_plan_set_cell_grid: need one grid entry per structure");
+    return MLFF_ERR_INVALID;
+  }
+  if (pl->cfg.cell_capacity <= 0 || !pl->cfg.any_periodic) {
+    set_error("mlff_plan_set_cell_grid: plan has no cell capacity or no periodic structure");
+    return MLFF_ERR_STATE;
+  }
+  int used = 0;
+  for (int b = 0; b < n_structures; ++b) {
+    const int32_t* g = grid_host + 4 * b;
+    if (g[0] == 0) continue;
+    if (g[0] < 3 || g[1] < 3 || g[2] < 3 || g[3] < 0) {
+      set_error("mlff_plan_set_cell_grid: structure %d needs >= 3 cells per axis", b);
+      return MLFF_ERR_INVALID;
+    }
+    const int64_t end = (int64_t)g[3] + (int64_t)g[0] * g[1] * g[2];
+    if (end > pl->cfg.cell_capacity) {
+      set_error("mlff_plan_set_cell_grid: %lld cells exceed the capacity %d", (long long)end,
+                pl->cfg.cell_capacity);
+      return MLFF_ERR_INVALID;
+    }
+    used = (int)(end > used ? end : used);
+  }
+  hipError_t e = hipMemcpy(pl->grid_dev, grid_host, 16 * (size_t)n_structures,
+                           hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "mlff_plan_set_cell_grid");
+  pl->nl.grid = used > 0 ? pl->grid_dev : nullptr;
+  pl->nl.n_cells = used;
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
   return MLFF_OK;
 }
 

@@ -39,6 +39,10 @@ struct NLArgs {
   int32_t* col; int32_t* edge_shift; int32_t* edge_pair; int32_t ecap;
   int32_t* pair_lo; int32_t* pair_hi; int32_t* pair_shift; float4* geom; int32_t pcap;
   int32_t* counts;
+  // cell list (periodic structures): grid[b] = {nc0, nc1, nc2, base}, nc0 == 0 -> brute force
+  const int4* grid; int32_t n_cells;
+  int32_t* cell_count; int32_t* cell_start; int32_t* cell_atoms; int32_t* atom_cell;
+  int32_t* edge_tmp_shift;   // [ecap*3] image shifts of cell-list rows before sorting
 };
 hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs);
 

@@ -109,22 +109,142 @@ __device__ __forceinline__ bool within(float dx, float dy, float dz, float rc, b
   return d <= rc;
 }
 
+// ---------------------------------------------------------------------------
+// cell list (periodic structures with at least 3 cells of width >= rc per axis)
+// ---------------------------------------------------------------------------
+
+struct CellList {
+  const int4* grid;          // per structure {nc0, nc1, nc2, first cell}; nc0 == 0: brute force
+  const int32_t* cell_start; // [C+1]
+  const int32_t* cell_atoms; // atoms grouped by cell, ascending within a cell
+  const int32_t* atom_cell;  // [N], -1 for brute-force atoms
+};
+
+// Visit every atom of the 27 cells around atom i's cell (distinct because
+// every axis has at least 3 cells).
+template <class Fn>
+__device__ __forceinline__ void for_each_cell_candidate(const CellList& cl, int i, int b, Fn fn) {
+  const int4 g = cl.grid[b];
+  const int lc = cl.atom_cell[i] - g.w;
+  const int c2 = lc % g.z, c1 = (lc / g.z) % g.y, c0 = lc / (g.y * g.z);
+  for (int da = -1; da <= 1; ++da) {
+    const int n0 = (c0 + da + g.x) % g.x;
+    for (int db = -1; db <= 1; ++db) {
+      const int n1 = (c1 + db + g.y) % g.y;
+      for (int dc = -1; dc <= 1; ++dc) {
+        const int n2 = (c2 + dc + g.z) % g.z;
+        const int cell = g.w + (n0 * g.y + n1) * g.z + n2;
+        for (int k = cl.cell_start[cell]; k < cl.cell_start[cell + 1]; ++k) fn(cl.cell_atoms[k]);
+      }
+    }
+  }
+}
+
+__global__ void cl_zero_kernel(int n, int32_t* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// cell of every cell-list atom from its wrapped fractional coordinates
+__global__ void cl_bin_kernel(const float* __restrict__ pos, int N,
+                              const int32_t* __restrict__ atom_mol,
+                              const float* __restrict__ cells, const uint8_t* __restrict__ pbc,
+                              const int4* __restrict__ grid, int32_t* __restrict__ atom_cell,
+                              int32_t* __restrict__ cell_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int b = atom_mol[i];
+  const int4 g = grid[b];
+  if (g.x == 0) { atom_cell[i] = -1; return; }
+  CellInfo ci;
+  bool periodic;
+  load_cell(cells, pbc, b, ci, periodic);
+  const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+  float f[3];
+  f[0] = __builtin_fmaf(z, ci.inv[6], __builtin_fmaf(y, ci.inv[3], x * ci.inv[0]));
+  f[1] = __builtin_fmaf(z, ci.inv[7], __builtin_fmaf(y, ci.inv[4], x * ci.inv[1]));
+  f[2] = __builtin_fmaf(z, ci.inv[8], __builtin_fmaf(y, ci.inv[5], x * ci.inv[2]));
+  const int nc[3] = {g.x, g.y, g.z};
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float w = f[a] - floorf(f[a]);   // [0, 1]
+    int k = (int)(w * (float)nc[a]);
+    c[a] = k < 0 ? 0 : (k >= nc[a] ? nc[a] - 1 : k);
+  }
+  const int cell = g.w + (c[0] * g.y + c[1]) * g.z + c[2];
+  atom_cell[i] = cell;
+  atomicAdd(&cell_count[cell], 1);
+}
+
+// exclusive scan of the cell counts (one workgroup), then reset the counts
+// so they serve as insertion cursors
+__global__ __launch_bounds__(1024) void cl_scan_kernel(int32_t* __restrict__ cell_count, int C,
+                                                       int32_t* __restrict__ cell_start) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int per = (C + 1023) / 1024;
+  const int lo = min(C, t * per), hi = min(C, lo + per);
+  int a = 0;
+  for (int k = lo; k < hi; ++k) a += cell_count[k];
+  s[t] = a;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = (t >= off) ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int r = s[t] - a;
+  for (int k = lo; k < hi; ++k) {
+    cell_start[k] = r;
+    r += cell_count[k];
+    cell_count[k] = 0;
+  }
+  if (t == 1023) cell_start[C] = s[1023];
+}
+
+__global__ void cl_place_kernel(int N, const int32_t* __restrict__ atom_cell,
+                                const int32_t* __restrict__ cell_start,
+                                int32_t* __restrict__ cursor, int32_t* __restrict__ cell_atoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int cell = atom_cell[i];
+  if (cell < 0) return;
+  const int slot = atomicAdd(&cursor[cell], 1);
+  cell_atoms[cell_start[cell] + slot] = i;
+}
+
+// ascending atom order inside every cell (placement order is not deterministic)
+__global__ void cl_sort_cells_kernel(int C, const int32_t* __restrict__ cell_start,
+                                     int32_t* __restrict__ cell_atoms) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int lo = cell_start[c], hi = cell_start[c + 1];
+  for (int k = lo + 1; k < hi; ++k) {
+    const int v = cell_atoms[k];
+    int m = k - 1;
+    while (m >= lo && cell_atoms[m] > v) { cell_atoms[m + 1] = cell_atoms[m]; --m; }
+    cell_atoms[m + 1] = v;
+  }
+}
+
 __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
                                 const int32_t* __restrict__ mol_ptr,
                                 const int32_t* __restrict__ atom_mol,
                                 const float* __restrict__ cells, const uint8_t* __restrict__ pbc,
-                                float rc, int32_t* __restrict__ deg, int32_t* __restrict__ udeg) {
+                                float rc, int32_t* __restrict__ deg, int32_t* __restrict__ udeg,
+                                CellList cl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int b = atom_mol[i];
-  const int j0 = mol_ptr[b], j1 = mol_ptr[b + 1];
   CellInfo ci;
   bool periodic;
   load_cell(cells, pbc, b, ci, periodic);
   const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   int n = 0, nu = 0;
-  for (int j = j0; j < j1; ++j) {
-    if (j == i) continue;
+  auto test = [&](int j) {
+    if (j == i) return;
     float vx, vy, vz, d;
     int s0, s1, s2;
     if (within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic, ci, vx,
@@ -132,6 +252,12 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
       ++n;
       nu += (j > i);
     }
+  };
+  if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
+    for_each_cell_candidate(cl, i, b, test);
+  } else {
+    const int j0 = mol_ptr[b], j1 = mol_ptr[b + 1];
+    for (int j = j0; j < j1; ++j) test(j);
   }
   deg[i] = n;
   udeg[i] = nu;
@@ -188,7 +314,8 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
                                int32_t* __restrict__ edge_shift, int32_t* __restrict__ edge_pair,
                                int32_t ecap, int32_t* __restrict__ pair_lo,
                                int32_t* __restrict__ pair_hi, int32_t* __restrict__ pair_shift,
-                               float4* __restrict__ geom, int32_t pcap) {
+                               float4* __restrict__ geom, int32_t pcap, CellList cl,
+                               int32_t* __restrict__ tmp_shift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int b = atom_mol[i];
@@ -199,6 +326,63 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   int e = row_ptr[i];
   int p = pair_ptr[i];
+  if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
+    // cell-list row: collect in cell order, sort by neighbour index, then emit
+    const int e0 = e;
+    for_each_cell_candidate(cl, i, b, [&](int j) {
+      if (j == i) return;
+      float vx, vy, vz, d;
+      int s0, s1, s2;
+      if (!within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic, ci,
+                  vx, vy, vz, d, s0, s1, s2))
+        return;
+      if (e < ecap) {
+        col[e] = j;
+        tmp_shift[3 * e] = s0; tmp_shift[3 * e + 1] = s1; tmp_shift[3 * e + 2] = s2;
+      }
+      ++e;
+    });
+    const int e1 = min(e, ecap);
+    for (int k = e0 + 1; k < e1; ++k) {
+      const int v = col[k];
+      const int t0 = tmp_shift[3 * k], t1 = tmp_shift[3 * k + 1], t2 = tmp_shift[3 * k + 2];
+      int m = k - 1;
+      while (m >= e0 && col[m] > v) {
+        col[m + 1] = col[m];
+        tmp_shift[3 * (m + 1)] = tmp_shift[3 * m];
+        tmp_shift[3 * (m + 1) + 1] = tmp_shift[3 * m + 1];
+        tmp_shift[3 * (m + 1) + 2] = tmp_shift[3 * m + 2];
+        --m;
+      }
+      col[m + 1] = v;
+      tmp_shift[3 * (m + 1)] = t0; tmp_shift[3 * (m + 1) + 1] = t1; tmp_shift[3 * (m + 1) + 2] = t2;
+    }
+    for (int k = e0; k < e1; ++k) {
+      const int j = col[k];
+      const int s0 = tmp_shift[3 * k], s1 = tmp_shift[3 * k + 1], s2 = tmp_shift[3 * k + 2];
+      if (edge_shift) {
+        edge_shift[3 * k] = s0; edge_shift[3 * k + 1] = s1; edge_shift[3 * k + 2] = s2;
+      }
+      if (j > i) {
+        if (pair_lo != nullptr && p < pcap) {
+          // the pair vector rounds exactly as inside within()
+          float sx, sy, sz;
+          shift_vec(ci, s0, s1, s2, sx, sy, sz);
+          const float vx = (pos[3 * j] - xi) + sx, vy = (pos[3 * j + 1] - yi) + sy,
+                      vz = (pos[3 * j + 2] - zi) + sz;
+          pair_lo[p] = i;
+          pair_hi[p] = j;
+          if (pair_shift) {
+            pair_shift[3 * p] = s0; pair_shift[3 * p + 1] = s1; pair_shift[3 * p + 2] = s2;
+          }
+          geom[p] = make_float4(vx, vy, vz, norm3_exact(vx, vy, vz));
+        }
+        if (edge_pair != nullptr) edge_pair[k] = (p < pcap) ? p : -1;
+        ++p;
+      }
+    }
+    return;
+  }
   for (int j = j0; j < j1; ++j) {
     if (j == i) continue;
     float vx, vy, vz, d;
@@ -273,15 +457,26 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
   const int tb = 128;
   const int g = (a.N + tb - 1) / tb;
   if (a.N == 0) return hipSuccess;
+  CellList cl{nullptr, nullptr, nullptr, nullptr};
+  if (a.grid != nullptr && a.n_cells > 0) {
+    const int C = a.n_cells;
+    cl_zero_kernel<<<(C + 255) / 256, 256, 0, s>>>(C, a.cell_count);
+    cl_bin_kernel<<<g, tb, 0, s>>>(a.pos, a.N, a.atom_mol, a.cells, a.pbc, a.grid, a.atom_cell,
+                                   a.cell_count);
+    cl_scan_kernel<<<1, 1024, 0, s>>>(a.cell_count, C, a.cell_start);
+    cl_place_kernel<<<g, tb, 0, s>>>(a.N, a.atom_cell, a.cell_start, a.cell_count, a.cell_atoms);
+    cl_sort_cells_kernel<<<(C + 127) / 128, 128, 0, s>>>(C, a.cell_start, a.cell_atoms);
+    cl = CellList{a.grid, a.cell_start, a.cell_atoms, a.atom_cell};
+  }
   nl_count_kernel<<<g, tb, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
-                                   a.deg, a.udeg);
+                                   a.deg, a.udeg, cl);
   nl_scan_kernel<<<1, 1024, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr, a.counts, a.ecap,
                                     build_pairs ? a.pcap : 0x7fffffff);
   nl_fill_kernel<<<g, tb, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                   a.row_ptr, a.pair_ptr, a.col, a.edge_shift,
                                   build_pairs ? a.edge_pair : nullptr, a.ecap,
                                   build_pairs ? a.pair_lo : nullptr, a.pair_hi, a.pair_shift,
-                                  a.geom, a.pcap);
+                                  a.geom, a.pcap, cl, a.edge_tmp_shift);
   if (build_pairs) {
     const int lg = min(4096, max(1, (a.ecap + 255) / 256));
     nl_link_kernel<<<lg, 256, 0, s>>>(a.N, a.row_ptr, a.pair_ptr, a.col, a.edge_pair, a.ecap,

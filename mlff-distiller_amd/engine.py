@@ -83,6 +83,8 @@ class BatchLayout:
     cells: Optional[torch.Tensor]  # [B,3,3] float32 device or None
     pbc: Optional[torch.Tensor]    # [B,3] uint8 device or None
     periodic: bool
+    cells_host: Optional[np.ndarray] = None
+    pbc_host: Optional[np.ndarray] = None
 
     @property
     def n_structures(self) -> int:
@@ -96,6 +98,7 @@ def make_layout(sizes: Sequence[int], device, cells=None, pbc=None) -> BatchLayo
     atom_mol = np.repeat(np.arange(len(sizes), dtype=np.int32), sizes)
     periodic = False
     ct = pt = None
+    cells_np = pbc_np = None
     if cells is not None and pbc is not None:
         pbc_np = np.asarray(pbc, dtype=bool).reshape(len(sizes), 3)
         if pbc_np.any():
@@ -104,7 +107,41 @@ def make_layout(sizes: Sequence[int], device, cells=None, pbc=None) -> BatchLayo
             ct = torch.from_numpy(np.ascontiguousarray(cells_np)).to(device)
             pt = torch.from_numpy(pbc_np.astype(np.uint8)).to(device)
     return BatchLayout(int(mol_ptr[-1]), sizes, torch.from_numpy(mol_ptr).to(device),
-                       torch.from_numpy(atom_mol).to(device), ct, pt, periodic)
+                       torch.from_numpy(atom_mol).to(device), ct, pt, periodic,
+                       cells_np if periodic else None, pbc_np if periodic else None)
+
+
+# structures smaller than this keep the brute-force scan (cheaper than 27 cells)
+CELL_LIST_MIN_ATOMS = 256
+
+
+def cell_grid(layout: BatchLayout, cutoff: float, min_atoms: Optional[int] = None,
+              max_cells: int = 1 << 22):
+    """Cell-list grid per structure: {nc0, nc1, nc2, first cell} for fully
+    periodic structures of at least ``min_atoms`` atoms whose cell holds
+    three cells of perpendicular width >= cutoff along every axis; zeros
+    (brute force) elsewhere.  The widths carry a 1e-5 relative margin so a
+    position rounded onto a cell boundary still lands within one cell."""
+    B = layout.n_structures
+    if min_atoms is None:
+        min_atoms = CELL_LIST_MIN_ATOMS
+    grid = np.zeros((B, 4), dtype=np.int32)
+    if not layout.periodic:
+        return grid, 0
+    base = 0
+    for b in range(B):
+        if layout.sizes[b] < min_atoms or not np.all(layout.pbc_host[b]):
+            continue
+        c = layout.cells_host[b].astype(np.float64)
+        vol = abs(np.linalg.det(c))
+        widths = np.array([vol / np.linalg.norm(np.cross(c[(a + 1) % 3], c[(a + 2) % 3]))
+                           for a in range(3)])
+        nc = np.floor(widths / (cutoff * (1.0 + 1e-5))).astype(np.int64)
+        if np.any(nc < 3) or base + int(np.prod(nc)) > max_cells:
+            continue
+        grid[b] = (nc[0], nc[1], nc[2], base)
+        base += int(np.prod(nc))
+    return grid, base
 
 
 def check_cells(cells: np.ndarray, pbc: np.ndarray, cutoff: float):
@@ -152,8 +189,11 @@ class Plan:
         self.H, self.L, self.R, self.max_z, self.cutoff = H, L, R, max_z, float(cutoff)
         self.ecap = int(edge_capacity)
         self.pcap = int(pair_capacity if pair_capacity is not None else (self.ecap + 1) // 2)
+        grid, n_cells = cell_grid(layout, cutoff)
+        self.cell_grid = grid
+        self.cell_capacity = n_cells
         cfg = C.Config(H, L, R, max_z, float(cutoff), layout.n_atoms, layout.n_structures,
-                       self.ecap, self.pcap, int(layout.periodic), persistent_blocks)
+                       self.ecap, self.pcap, int(layout.periodic), persistent_blocks, n_cells)
         handle = C.vp()
         C.check("mlff_plan_create", self.lib.mlff_plan_create(ctypes_byref(handle), ctypes_byref(cfg)))
         self.handle = handle
@@ -170,6 +210,13 @@ class Plan:
         # view of the counts slots inside the workspace
         off = counts_addr - self.workspace.data_ptr()
         self.counts = self.workspace[off:off + 4 * C.MLFF_COUNT_SLOTS].view(torch.int32)
+        self._apply_cell_grid()
+
+    def _apply_cell_grid(self):
+        if int(self.cell_grid[:, 0].max(initial=0)) > 0:
+            g = np.ascontiguousarray(self.cell_grid, dtype=np.int32)
+            C.check("mlff_plan_set_cell_grid", self.lib.mlff_plan_set_cell_grid(
+                self.handle, g.ctypes.data, self.layout.n_structures))
 
     def set_weights(self, weights: torch.Tensor):
         """Rebind with a new weight blob (same layout)."""
@@ -179,6 +226,7 @@ class Plan:
             self.handle, C.ptr(self.workspace), self.workspace.numel(), C.ptr(weights),
             C.ptr(self.numbers), C.ptr(lay.mol_ptr), C.ptr(lay.atom_mol), C.ptr(lay.cells),
             C.ptr(lay.pbc)))
+        self._apply_cell_grid()
 
     def energy_forces(self, positions: torch.Tensor, energy: torch.Tensor,
                       forces: Optional[torch.Tensor], virial: Optional[torch.Tensor] = None,

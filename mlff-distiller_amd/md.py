@@ -92,6 +92,19 @@ class BatchedNVE:
             0, self.layout.atom_mol.long(), ke0)
         self.ledger[0, :, 0] = self.energy
         self.ledger[0, :, 1] = ke0
+        self.tuning = None
+
+    def autotune(self, **kw):
+        """Choose the pair kernels' persistent grid on this workload
+        (autotune.tune); later plans (after a capacity regrowth) keep it."""
+        from . import autotune
+        forces_saved = self.forces.clone()
+        energy_saved = self.energy.clone()
+        self.tuning = autotune.tune(self.plan, self.pos32, **kw)
+        self._persistent_blocks = int(self.tuning["blocks"])
+        self.forces.copy_(forces_saved)
+        self.energy.copy_(energy_saved)
+        return self.tuning
 
     # ------------------------------------------------------------------
     def _initial_edges(self) -> int:

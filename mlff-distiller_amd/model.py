@@ -312,10 +312,24 @@ class StudentForceField(nn.Module):
         if entry is not None:
             self._plans.move_to_end(key)
             if periodic:
-                entry["layout"].cells.copy_(torch.as_tensor(np.asarray(cells, dtype=np.float32)
-                                                            .reshape(-1, 3, 3)))
-                entry["layout"].pbc.copy_(torch.as_tensor(np.asarray(pbc, dtype=np.uint8)
-                                                          .reshape(-1, 3)))
+                lay = entry["layout"]
+                c32 = np.asarray(cells, dtype=np.float32).reshape(-1, 3, 3)
+                pb = np.asarray(pbc, dtype=bool).reshape(-1, 3)
+                if not (np.array_equal(c32, lay.cells_host) and np.array_equal(pb, lay.pbc_host)):
+                    lay.cells.copy_(torch.as_tensor(c32))
+                    lay.pbc.copy_(torch.as_tensor(pb.astype(np.uint8)))
+                    lay.cells_host, lay.pbc_host = c32, pb
+                    plan = entry["plan"]
+                    grid, n_cells = engine.cell_grid(lay, self.cutoff)
+                    cap = int(plan.cell_capacity)
+                    if n_cells <= cap:
+                        plan.cell_grid = grid
+                        plan._apply_cell_grid()
+                    else:
+                        entry["plan"] = engine.Plan(
+                            self.hidden_dim, self.num_interactions, self.num_rbf, self.max_z,
+                            self.cutoff, lay, entry["numbers"], self.packed_weights(device),
+                            plan.ecap)
             return entry
         layout = engine.make_layout(sizes, device, cells if periodic else None,
                                     pbc if periodic else None)

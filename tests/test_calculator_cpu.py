@@ -53,6 +53,22 @@ def test_unsupported_hidden_fails_loudly_on_use():
         m.evaluate(torch.zeros(2, dtype=torch.long), torch.zeros(2, 3), np.array([2]))
 
 
+def test_cell_grid_choice():
+    engine = pkg_mod("engine")
+    lay = engine.make_layout([3000, 60, 400], "cpu",
+                             cells=np.stack([np.eye(3) * 31.0, np.eye(3) * 20.0,
+                                             np.diag([16.0, 30.0, 12.0])]),
+                             pbc=np.array([[1, 1, 1], [1, 1, 1], [1, 1, 0]], bool))
+    grid, n = engine.cell_grid(lay, 5.0)
+    assert tuple(grid[0]) == (6, 6, 6, 0)     # floor(31 / 5.00005) = 6
+    assert grid[1, 0] == 0                    # below the atom threshold
+    assert grid[2, 0] == 0                    # not periodic along z
+    assert n == 216
+    lay2 = engine.make_layout([500], "cpu", cells=np.eye(3)[None] * 14.0,
+                              pbc=np.ones((1, 3), bool))
+    assert engine.cell_grid(lay2, 5.0)[0][0, 0] == 0   # only 2 cells per axis
+
+
 def test_cells_narrower_than_twice_cutoff_are_rejected():
     engine = pkg_mod("engine")
     with pytest.raises(ValueError, match="half the periodic cell"):

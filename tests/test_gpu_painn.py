@@ -88,6 +88,36 @@ def test_neighbor_list_periodic_water(gpu):
     assert 40 < len(cl) / len(pos32) < 70
 
 
+@pytest.mark.parametrize("skew", [0.0, 2.5])
+def test_cell_list_equals_brute_force(gpu, skew):
+    """The cell-list scan (periodic boxes >= 256 atoms) reproduces the
+    brute-force edge set exactly: energies, forces and virials are
+    bit-identical."""
+    note(f"cell list vs brute force, skew {skew}")
+    synth = pkg_mod("synth")
+    engine = pkg_mod("engine")
+    box = synth.water_box(10, seed=3)
+    cell = box.cell.copy()
+    cell[1, 0] += skew
+    cell[2, 1] -= 0.5 * skew
+    Z = torch.tensor(box.numbers, device=gpu)
+    P = torch.tensor(box.positions.astype(np.float32), device=gpu)
+    m1 = _model(64, 2, 20, seed=0)
+    out_cl = m1.energy_forces_virial(Z, P, torch.tensor(cell), torch.tensor(box.pbc))
+    plan = next(iter(m1._plans.values()))["plan"]
+    assert plan.cell_grid[0, 0] >= 3          # the cell list was used
+    old = engine.CELL_LIST_MIN_ATOMS
+    engine.CELL_LIST_MIN_ATOMS = 10 ** 9
+    try:
+        m2 = _model(64, 2, 20, seed=0)
+        out_bf = m2.energy_forces_virial(Z, P, torch.tensor(cell), torch.tensor(box.pbc))
+        assert next(iter(m2._plans.values()))["plan"].cell_grid[0, 0] == 0
+    finally:
+        engine.CELL_LIST_MIN_ATOMS = old
+    for a, b in zip(out_cl, out_bf):
+        assert torch.equal(a, b)
+
+
 def _check_ef(model, Z, P32, sizes, E, F, label, absolute=True):
     """Compare with the float64 oracle.  ``absolute``: the task's bars
     (1e-5 eV/atom, 1e-4 eV/A).  Otherwise (weights scaled up so forces reach
