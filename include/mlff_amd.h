@@ -218,6 +218,31 @@ int mlff_md_run(mlff_plan* plan, void* stream, int n_steps);
  * n_launches times. */
 int mlff_md_graph_run(mlff_plan* plan, void* stream, int steps_per_graph, int n_launches);
 
+/* ---------------------------------------------------------------------------
+ * Multiple time step (no counterpart in the reference, whose MD is plain
+ * Velocity Verlet, nve_harness.py:214).  r-RESPA with the inner plan's model
+ * (a compact student) as the fast force at dt and the outer plan's model as
+ * the slow correction F_outer - F_inner every n_inner steps.  The outer
+ * model evaluates on the inner plan's neighbour list and pair geometry (same
+ * positions, same cutoff; both plans need equal capacities).  Per structure,
+ * flag != 0 makes the outer model the inner force (fallback: plain
+ * Velocity Verlet with the outer model).  dis_max [B] receives the running
+ * max of |F_outer - F_inner_model| at outer steps.  The ledger gets
+ * (E_outer, E_kin) per structure every outer step.
+ * ------------------------------------------------------------------------- */
+int mlff_mts_bind(mlff_plan* outer, mlff_plan* inner, double* x_dev, double* p_dev,
+                  const double* inv_mass_dev, float* pos32_dev, float* f_outer_dev,
+                  float* f_inner_dev, float* f_mix_dev, double* e_outer_dev,
+                  double* e_inner_dev, const uint8_t* flag_dev, float* dis_max_dev,
+                  double* ledger_dev, int32_t ledger_len, double dt, int32_t n_inner);
+/* Forces of both models at the current positions. */
+int mlff_mts_init_forces(mlff_plan* outer, void* stream);
+/* Enqueue n_outer outer steps; fallback != 0 honours the flags (and
+ * evaluates the outer model at every inner step). */
+int mlff_mts_run(mlff_plan* outer, void* stream, int n_outer, int fallback);
+/* Capture outer_per_graph outer steps (no fallback) once, replay n_launches times. */
+int mlff_mts_graph_run(mlff_plan* outer, void* stream, int outer_per_graph, int n_launches);
+
 #ifdef __cplusplus
 }
 #endif

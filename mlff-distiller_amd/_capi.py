@@ -39,7 +39,8 @@ EXPORTS = (
     "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
     "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run", "mlff_launch_stage",
-    "mlff_plan_set_blocks", "mlff_plan_set_cell_grid",
+    "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
+    "mlff_mts_run", "mlff_mts_graph_run",
 )
 
 STAGE_NEIGHBOR, STAGE_FILTER_FWD, STAGE_MESSAGE_FWD, STAGE_UPDATE_FWD = 0, 1, 2, 3
@@ -117,6 +118,14 @@ def _declare(lib):
     lib.mlff_plan_set_blocks.argtypes = [vp, ctypes.c_int]
     lib.mlff_plan_set_cell_grid.restype = ctypes.c_int
     lib.mlff_plan_set_cell_grid.argtypes = [vp, vp, ctypes.c_int]
+    lib.mlff_mts_bind.restype = ctypes.c_int
+    lib.mlff_mts_bind.argtypes = [vp] * 14 + [i32, ctypes.c_double, i32]
+    lib.mlff_mts_init_forces.restype = ctypes.c_int
+    lib.mlff_mts_init_forces.argtypes = [vp, vp]
+    lib.mlff_mts_run.restype = ctypes.c_int
+    lib.mlff_mts_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.mlff_mts_graph_run.restype = ctypes.c_int
+    lib.mlff_mts_graph_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
 
 
 def lib():

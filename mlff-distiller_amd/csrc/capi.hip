@@ -105,6 +105,12 @@ struct mlff_plan {
   double* ledger; int ledger_len; double dt;
   hipGraph_t graph; hipGraphExec_t graph_exec; int graph_steps; hipStream_t graph_stream;
   int4* grid_dev;
+  // multiple time step (this plan is the outer one)
+  bool mts_bound;
+  mlff_plan* mts_inner;
+  float* f_outer; float* f_inner; float* f_mix; double* e_outer; double* e_inner;
+  const uint8_t* flag; float* dis_max; int n_inner;
+  hipGraph_t mts_graph; hipGraphExec_t mts_exec; int mts_graph_steps;
   // workspace carve-up (byte offsets)
   struct { int64_t off, bytes; } seg[32];
   int nseg;
@@ -261,6 +267,8 @@ int mlff_plan_destroy(mlff_plan* pl) {
   if (!pl) return MLFF_OK;
   if (pl->graph_exec) (void)hipGraphExecDestroy(pl->graph_exec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->mts_exec) (void)hipGraphExecDestroy(pl->mts_exec);
+  if (pl->mts_graph) (void)hipGraphDestroy(pl->mts_graph);
   delete pl;
   return MLFF_OK;
 }
@@ -337,6 +345,9 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
   pl->bound = true;
   pl->md_bound = false;
+  pl->mts_bound = false;
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
   if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
   return MLFF_OK;
@@ -503,6 +514,149 @@ int mlff_md_graph_run(mlff_plan* pl, void* stream, int steps_per_graph, int n_la
   }
   for (int k = 0; k < n_launches; ++k) {
     hipError_t e = hipGraphLaunch(pl->graph_exec, s);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  }
+  return MLFF_OK;
+}
+
+// Model of `pl` on the neighbour list and pair geometry of `src` (same
+// positions, same cutoff, equal capacities).
+static int run_ef_on(mlff_plan* pl, hipStream_t s, double* energy, float* forces,
+                     const Buffers& src) {
+  Buffers b = pl->buf;
+  b.counts = src.counts;
+  b.row_ptr = src.row_ptr; b.col = src.col; b.edge_pair = src.edge_pair;
+  b.pair_lo = src.pair_lo; b.pair_hi = src.pair_hi; b.pair_shift = src.pair_shift;
+  b.geom = src.geom;
+  hipError_t e = launch_forward(b, s);
+  if (e != hipSuccess) return hip_fail(e, "outer forward");
+  e = launch_backward(b, s, forces, false);
+  if (e != hipSuccess) return hip_fail(e, "outer adjoint");
+  e = launch_reduce(b, s, energy, nullptr, nullptr);
+  if (e != hipSuccess) return hip_fail(e, "outer reduce");
+  return MLFF_OK;
+}
+
+int mlff_mts_bind(mlff_plan* pl, mlff_plan* in, double* x_dev, double* p_dev,
+                  const double* inv_mass_dev, float* pos32_dev, float* f_outer_dev,
+                  float* f_inner_dev, float* f_mix_dev, double* e_outer_dev,
+                  double* e_inner_dev, const uint8_t* flag_dev, float* dis_max_dev,
+                  double* ledger_dev, int32_t ledger_len, double dt, int32_t n_inner) {
+  if (!pl || !in || !pl->bound || !in->bound) {
+    set_error("mlff_mts_bind: both plans must be bound");
+    return MLFF_ERR_STATE;
+  }
+  const mlff_config &a = pl->cfg, &b = in->cfg;
+  if (a.n_atoms != b.n_atoms || a.n_structures != b.n_structures || a.cutoff != b.cutoff ||
+      a.edge_capacity != b.edge_capacity || a.pair_capacity != b.pair_capacity ||
+      a.any_periodic != b.any_periodic) {
+    set_error("mlff_mts_bind: plans differ in batch, cutoff or capacity");
+    return MLFF_ERR_INVALID;
+  }
+  if (!x_dev || !p_dev || !inv_mass_dev || !pos32_dev || !f_outer_dev || !f_inner_dev ||
+      !f_mix_dev || !e_outer_dev || !e_inner_dev || !flag_dev || !dis_max_dev || n_inner < 1 ||
+      ledger_len < 0 || (ledger_len > 0 && !ledger_dev)) {
+    set_error("mlff_mts_bind: invalid argument");
+    return MLFF_ERR_INVALID;
+  }
+  pl->x = x_dev; pl->p = p_dev; pl->inv_mass = inv_mass_dev; pl->pos32 = pos32_dev;
+  pl->ledger = ledger_dev; pl->ledger_len = ledger_len; pl->dt = dt;
+  pl->mts_inner = in;
+  pl->f_outer = f_outer_dev; pl->f_inner = f_inner_dev; pl->f_mix = f_mix_dev;
+  pl->e_outer = e_outer_dev; pl->e_inner = e_inner_dev; pl->flag = flag_dev;
+  pl->dis_max = dis_max_dev; pl->n_inner = n_inner;
+  pl->mts_bound = true;
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
+  return MLFF_OK;
+}
+
+int mlff_mts_init_forces(mlff_plan* pl, void* stream) {
+  if (!pl || !pl->mts_bound) { set_error("mlff_mts_init_forces: MTS not bound"); return MLFF_ERR_STATE; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  mlff_plan* in = pl->mts_inner;
+  int rc = run_ef(in, s, pl->pos32, pl->e_inner, pl->f_inner, nullptr, nullptr);
+  if (rc != MLFF_OK) return rc;
+  return run_ef_on(pl, s, pl->e_outer, pl->f_outer, in->buf);
+}
+
+static int mts_outer_step(mlff_plan* pl, hipStream_t s, bool fallback) {
+  mlff_plan* in = pl->mts_inner;
+  const int N = pl->buf.N;
+  const double dt = pl->dt, Dt = pl->n_inner * dt;
+  const float* Fin = fallback ? pl->f_mix : pl->f_inner;
+  hipError_t e = launch_mts_kick(N, pl->p, pl->f_outer, Fin, 0.5 * Dt, s);
+  if (e != hipSuccess) return hip_fail(e, "mts outer kick");
+  for (int k = 0; k < pl->n_inner; ++k) {
+    e = launch_vv_first(N, pl->x, pl->p, pl->inv_mass, Fin, pl->pos32, dt, s);
+    if (e != hipSuccess) return hip_fail(e, "mts drift");
+    int rc = run_ef(in, s, pl->pos32, pl->e_inner, pl->f_inner, nullptr, nullptr);
+    if (rc != MLFF_OK) return rc;
+    if (fallback) {
+      rc = run_ef_on(pl, s, pl->e_outer, pl->f_outer, in->buf);
+      if (rc != MLFF_OK) return rc;
+      e = launch_mts_select(N, pl->buf.atom_mol, pl->flag, pl->f_outer, pl->f_inner, pl->f_mix, s);
+      if (e != hipSuccess) return hip_fail(e, "mts select");
+    }
+    e = launch_mts_kick(N, pl->p, Fin, nullptr, 0.5 * dt, s);
+    if (e != hipSuccess) return hip_fail(e, "mts inner kick");
+  }
+  if (!fallback) {
+    int rc = run_ef_on(pl, s, pl->e_outer, pl->f_outer, in->buf);
+    if (rc != MLFF_OK) return rc;
+  }
+  e = launch_mts_kick(N, pl->p, pl->f_outer, Fin, 0.5 * Dt, s);
+  if (e != hipSuccess) return hip_fail(e, "mts outer kick");
+  e = launch_mts_disagree(N, pl->buf.atom_mol, pl->f_outer, pl->f_inner, pl->dis_max, s);
+  if (e != hipSuccess) return hip_fail(e, "mts disagreement");
+  e = launch_mts_ledger(pl->buf.B, pl->buf.mol_ptr, pl->p, pl->inv_mass, pl->e_outer, pl->ledger,
+                        pl->ledger_len, pl->buf.counts, s);
+  if (e != hipSuccess) return hip_fail(e, "mts ledger");
+  return MLFF_OK;
+}
+
+int mlff_mts_run(mlff_plan* pl, void* stream, int n_outer, int fallback) {
+  if (!pl || !pl->mts_bound) { set_error("mlff_mts_run: MTS not bound"); return MLFF_ERR_STATE; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fallback) {
+    hipError_t e = launch_mts_select(pl->buf.N, pl->buf.atom_mol, pl->flag, pl->f_outer,
+                                     pl->f_inner, pl->f_mix, s);
+    if (e != hipSuccess) return hip_fail(e, "mts select");
+  }
+  for (int k = 0; k < n_outer; ++k) {
+    int rc = mts_outer_step(pl, s, fallback != 0);
+    if (rc != MLFF_OK) return rc;
+  }
+  return MLFF_OK;
+}
+
+int mlff_mts_graph_run(mlff_plan* pl, void* stream, int outer_per_graph, int n_launches) {
+  if (!pl || !pl->mts_bound) { set_error("mlff_mts_graph_run: MTS not bound"); return MLFF_ERR_STATE; }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (s == nullptr || outer_per_graph < 1 || n_launches < 0) {
+    set_error("mlff_mts_graph_run: needs a non-default stream and positive counts");
+    return MLFF_ERR_INVALID;
+  }
+  if (!pl->mts_exec || pl->mts_graph_steps != outer_per_graph) {
+    if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+    if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int rc = MLFF_OK;
+    for (int k = 0; k < outer_per_graph && rc == MLFF_OK; ++k) rc = mts_outer_step(pl, s, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc != MLFF_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) { (void)hipGraphDestroy(g); return hip_fail(e, "hipGraphInstantiate"); }
+    pl->mts_graph = g;
+    pl->mts_exec = ge;
+    pl->mts_graph_steps = outer_per_graph;
+  }
+  for (int k = 0; k < n_launches; ++k) {
+    hipError_t e = hipGraphLaunch(pl->mts_exec, s);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   }
   return MLFF_OK;

@@ -88,6 +88,17 @@ hipError_t launch_vv_second(int N, int B, const int32_t* mol_ptr, double* p,
                             const double* epot, double* ledger, int ledger_len,
                             int32_t* counts, hipStream_t s);
 
+hipError_t launch_mts_kick(int N, double* p, const float* Fa, const float* Fb, double coef,
+                           hipStream_t s);
+hipError_t launch_mts_select(int N, const int32_t* atom_mol, const uint8_t* flag,
+                             const float* F_full, const float* F_fast, float* F_in,
+                             hipStream_t s);
+hipError_t launch_mts_disagree(int N, const int32_t* atom_mol, const float* F_full,
+                               const float* F_fast, float* dis_max, hipStream_t s);
+hipError_t launch_mts_ledger(int B, const int32_t* mol_ptr, const double* p,
+                             const double* inv_mass, const double* epot, double* ledger,
+                             int ledger_len, int32_t* counts, hipStream_t s);
+
 void set_error(const char* fmt, ...);
 
 }  // namespace mlff

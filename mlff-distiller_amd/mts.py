@@ -107,7 +107,11 @@ def train_student(student, frames: FrameSet, steps: int, batch_molecules: int = 
 # ---------------------------------------------------------------------------
 
 class MTSNVE:
-    """r-RESPA NVE: compact student inside, full model outside, with fallback."""
+    """r-RESPA NVE: compact student inside, full model outside, with fallback.
+
+    The step loop runs in the C library (mlff_mts_run / mlff_mts_graph_run):
+    fused kick/drift kernels, the student's neighbour list reused by the
+    full model, chunks of outer steps captured in one HIP graph."""
 
     def __init__(self, full_model, fast_model, structures: Sequence["synth.Structure"],
                  inner_fs: float = 0.5, n_inner: int = 4, temperature_K: float = 300.0,
@@ -116,7 +120,7 @@ class MTSNVE:
         dev = torch.device(device)
         self.device = dev
         self.full, self.fast = full_model, fast_model
-        if abs(full_model.cutoff - fast_model.cutoff) > 0:
+        if full_model.cutoff != fast_model.cutoff:
             raise ValueError("inner and outer models must share the cutoff (one neighbour list)")
         self.structures = list(structures)
         numbers, positions, mol_ptr = synth.flatten(self.structures)
@@ -137,121 +141,64 @@ class MTSNVE:
         self.numbers = torch.tensor(numbers, dtype=torch.int64, device=dev)
         self.x = torch.tensor(positions, dtype=torch.float64, device=dev)
         self.p = torch.tensor(momenta, dtype=torch.float64, device=dev)
-        self.inv_m = torch.tensor(1.0 / masses, dtype=torch.float64, device=dev).unsqueeze(1)
+        self.inv_m = torch.tensor(1.0 / masses, dtype=torch.float64, device=dev)
         self.pos32 = self.x.to(torch.float32)
         f32 = dict(dtype=torch.float32, device=dev)
         f64 = dict(dtype=torch.float64, device=dev)
-        self.F_fast = torch.zeros(self.N, 3, **f32)
         self.F_full = torch.zeros(self.N, 3, **f32)
-        self.F_in = torch.zeros(self.N, 3, **f32)
-        self.E_fast = torch.zeros(self.B, **f64)
+        self.F_fast = torch.zeros(self.N, 3, **f32)
+        self.F_mix = torch.zeros(self.N, 3, **f32)
         self.E_full = torch.zeros(self.B, **f64)
-        self.flag = torch.zeros(self.B, dtype=torch.bool, device=dev)
+        self.E_fast = torch.zeros(self.B, **f64)
+        self.flag = torch.zeros(self.B, dtype=torch.uint8, device=dev)
         self.dis_max = torch.zeros(self.B, **f32)
         self.max_outer = max_outer
         self.ledger = torch.zeros(max_outer + 1, self.B, 2, **f64)
-        self.row = torch.ones(1, dtype=torch.int64, device=dev)
         self.headroom = headroom
-        row_ptr, col, _ = engine.neighbor_list(self.pos32, self.layout, full_model.cutoff)
-        self._make_plans(max(int(col.numel()), 16) * headroom)
+        self.lib = C.lib()
         self.stream = torch.cuda.Stream(device=dev)
-        self.graph = None
-        self.graph_steps = 0
         self.outer_done = 0
         self.fallback_steps = 0
         self.replays = 0
-        self._with_stream(self._init_forces)
+        row_ptr, col, _ = engine.neighbor_list(self.pos32, self.layout, full_model.cutoff)
+        self._make_plans(max(int(col.numel()), 16) * headroom)
+        self._on_stream(lambda: C.check("mlff_mts_init_forces", self.lib.mlff_mts_init_forces(
+            self.plan_full.handle, self.stream.cuda_stream)))
+        self._check_overflow()
+        ke = 0.5 * (self.p * self.p).sum(1) * self.inv_m
         self.ledger[0, :, 0] = self.E_full
-        self.ledger[0, :, 1] = self._ke()
+        self.ledger[0, :, 1] = torch.zeros(self.B, **f64).index_add_(0, self.batch, ke)
 
     # ------------------------------------------------------------------
-    def _make_plans(self, edges: float):
+    def _make_plans(self, edges: float, step: int = 1):
         bound = engine.edge_capacity_bound(self.sizes, False)
         cap = int(min(bound, int(edges) + 64))
-        self.plan_full = engine.Plan(self.full.hidden_dim, self.full.num_interactions,
-                                     self.full.num_rbf, self.full.max_z, self.full.cutoff,
-                                     self.layout, self.numbers,
-                                     self.full.packed_weights(self.device), cap)
-        self.plan_fast = engine.Plan(self.fast.hidden_dim, self.fast.num_interactions,
-                                     self.fast.num_rbf, self.fast.max_z, self.fast.cutoff,
-                                     self.layout, self.numbers,
-                                     self.fast.packed_weights(self.device), cap)
-        self.graph = None
+        mk = lambda m: engine.Plan(m.hidden_dim, m.num_interactions, m.num_rbf, m.max_z,
+                                   m.cutoff, self.layout, self.numbers,
+                                   m.packed_weights(self.device), cap)
+        self.plan_full, self.plan_fast = mk(self.full), mk(self.fast)
+        C.check("mlff_mts_bind", self.lib.mlff_mts_bind(
+            self.plan_full.handle, self.plan_fast.handle, C.ptr(self.x), C.ptr(self.p),
+            C.ptr(self.inv_m), C.ptr(self.pos32), C.ptr(self.F_full), C.ptr(self.F_fast),
+            C.ptr(self.F_mix), C.ptr(self.E_full), C.ptr(self.E_fast), C.ptr(self.flag),
+            C.ptr(self.dis_max), C.ptr(self.ledger), self.max_outer + 1, float(self.dt),
+            self.n_inner))
+        self.plan_full.counts[C.MLFF_COUNT_STEP] = step
 
-    def _with_stream(self, fn, *a):
+    def _on_stream(self, fn):
         self.stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(self.stream):
-            out = fn(*a)
+            fn()
         self.stream.synchronize()
-        return out
 
-    def _init_forces(self):
-        self.pos32.copy_(self.x)
-        self.plan_fast.energy_forces(self.pos32, self.E_fast, self.F_fast)
-        self.plan_full.energy_forces(self.pos32, self.E_full, self.F_full)
-        self.F_in.copy_(self.F_fast)
+    def _check_overflow(self):
+        c = self.plan_fast.read_counts()
+        if c[C.MLFF_COUNT_OVERFLOW]:
+            raise OverflowError(int(c[C.MLFF_COUNT_EDGES]))
 
-    def _ke(self):
-        ke = 0.5 * (self.p * self.p * self.inv_m).sum(1)
-        return torch.zeros(self.B, dtype=torch.float64, device=self.device).index_add_(
-            0, self.batch, ke)
-
-    def _outer_step(self, fallback: bool):
-        n, dt = self.n_inner, self.dt
-        Dt = n * dt
-        flag_atoms = self.flag[self.batch].unsqueeze(1) if fallback else None
-        # outer half kick with the slow force F_full - F_in
-        self.p.add_((self.F_full - self.F_in).to(torch.float64), alpha=0.5 * Dt)
-        for _ in range(n):
-            self.p.add_(self.F_in.to(torch.float64), alpha=0.5 * dt)
-            self.x.add_(self.p * self.inv_m, alpha=dt)
-            self.pos32.copy_(self.x)
-            self.plan_fast.energy_forces(self.pos32, self.E_fast, self.F_fast)
-            if fallback:
-                self.plan_full.energy_forces(self.pos32, self.E_full, self.F_full)
-                self.F_in.copy_(torch.where(flag_atoms, self.F_full, self.F_fast))
-            else:
-                self.F_in.copy_(self.F_fast)
-            self.p.add_(self.F_in.to(torch.float64), alpha=0.5 * dt)
-        if not fallback:
-            self.plan_full.energy_forces(self.pos32, self.E_full, self.F_full)
-        self.p.add_((self.F_full - self.F_in).to(torch.float64), alpha=0.5 * Dt)
-        # disagreement of the two models at the new positions
-        dis = (self.F_full - self.F_fast).norm(dim=1)
-        dmax = torch.zeros(self.B, dtype=torch.float32, device=self.device).scatter_reduce(
-            0, self.batch, dis, reduce="amax", include_self=True)
-        torch.maximum(self.dis_max, dmax, out=self.dis_max)
-        rec = torch.stack([self.E_full, self._ke()], dim=1).unsqueeze(0)
-        self.ledger.index_copy_(0, self.row, rec)
-        self.row.add_(1)
-
-    def _chunk(self, k: int, fallback: bool):
-        if fallback:
-            for _ in range(k):
-                self._outer_step(True)
-            return
-        if self.graph is None or self.graph_steps != k:
-            # warm the allocator on the side stream, then capture k outer steps
-            saved = self._save()
-            self._with_stream(lambda: [self._outer_step(False) for _ in range(1)])
-            self._restore(saved)
-            g = torch.cuda.CUDAGraph()
-            torch.cuda.synchronize()
-            with torch.cuda.graph(g, stream=self.stream):
-                for _ in range(k):
-                    self._outer_step(False)
-            self._restore(saved)
-            self.graph, self.graph_steps = g, k
-        self.graph.replay()
-
-    def _save(self):
-        return [t.clone() for t in (self.x, self.p, self.pos32, self.F_fast, self.F_full,
-                                    self.F_in, self.E_fast, self.E_full, self.row)]
-
-    def _restore(self, s):
-        for dst, src in zip((self.x, self.p, self.pos32, self.F_fast, self.F_full, self.F_in,
-                             self.E_fast, self.E_full, self.row), s):
-            dst.copy_(src)
+    def _state(self):
+        return [self.x, self.p, self.pos32, self.F_full, self.F_fast, self.F_mix, self.E_full,
+                self.E_fast]
 
     def run(self, n_outer: int, chunk: int = 10, graph: bool = True):
         """Advance n_outer outer steps (n_outer * n_inner inner steps)."""
@@ -261,38 +208,35 @@ class MTSNVE:
         while done < n_outer:
             k = min(chunk, n_outer - done)
             fallback = bool(self.flag.any())
-            # the inner force of this chunk's split: full model for flagged
-            # structures (their slow force vanishes), the student elsewhere
-            if fallback:
-                self.F_in.copy_(torch.where(self.flag[self.batch].unsqueeze(1), self.F_full,
-                                            self.F_fast))
-            else:
-                self.F_in.copy_(self.F_fast)
-            saved = self._save()
+            saved = [t.clone() for t in self._state()]
+            step = int(self.plan_full.counts[C.MLFF_COUNT_STEP])
             self.dis_max.zero_()
+            h, s = self.plan_full.handle, self.stream.cuda_stream
             if fallback or not graph:
-                self._with_stream(lambda: [self._outer_step(fallback) for _ in range(k)])
+                self._on_stream(lambda: C.check("mlff_mts_run", self.lib.mlff_mts_run(
+                    h, s, k, int(fallback))))
             else:
-                self._chunk(k, False)
-                torch.cuda.synchronize()
-            over = (self.plan_full.read_counts()[C.MLFF_COUNT_OVERFLOW] or
-                    self.plan_fast.read_counts()[C.MLFF_COUNT_OVERFLOW])
-            if over:
-                E = int(max(self.plan_full.read_counts()[C.MLFF_COUNT_EDGES],
-                            self.plan_fast.read_counts()[C.MLFF_COUNT_EDGES]))
-                self._restore(saved)
-                self._make_plans(E * max(self.headroom, 1.1))
+                self._on_stream(lambda: C.check("mlff_mts_graph_run",
+                                                self.lib.mlff_mts_graph_run(h, s, k, 1)))
+            try:
+                self._check_overflow()
+            except OverflowError as exc:
+                for dst, src in zip(self._state(), saved):
+                    dst.copy_(src)
+                self._make_plans(int(exc.args[0]) * max(self.headroom, 1.1), step)
                 continue
             exceed = self.dis_max > self.threshold
             if not fallback and bool(exceed.any()):
-                # replay this chunk with the full model inside for those structures
-                self._restore(saved)
-                self.flag.copy_(exceed)
+                # replay with the full model inside for the offending structures
+                for dst, src in zip(self._state(), saved):
+                    dst.copy_(src)
+                self.plan_full.counts[C.MLFF_COUNT_STEP] = step
+                self.flag.copy_(exceed.to(torch.uint8))
                 self.replays += 1
                 continue
             if fallback:
                 self.fallback_steps += k * int(self.flag.sum())
-                self.flag.copy_(exceed & self.flag)
+                self.flag.copy_((exceed & self.flag.bool()).to(torch.uint8))
             done += k
             self.outer_done += k
 
