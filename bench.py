@@ -441,6 +441,8 @@ def main():
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=20)
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="2: pair filter kernels on a second stream beside the atom-side kernels")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by a plain device copy at the end (byte-counter calibration)")
     args = ap.parse_args()
@@ -469,7 +471,7 @@ def main():
     log(f"{len(mols)} molecules, {n_atoms} atoms")
     total_steps = args.warmup + args.steps
     md = mdmod.BatchedNVE(model, mols, timestep_fs=0.5, temperature_K=300.0, seed=args.seed,
-                          max_steps=total_steps + 1)
+                          max_steps=total_steps + 1, streams=args.streams)
     tuning = md.autotune() if not args.no_autotune else None
     if tuning:
         log(f"autotune: persistent grid {tuning['blocks']} workgroups "
@@ -479,20 +481,16 @@ def main():
     E0, P0 = int(c0[0]), int(c0[1])
     e_tmp = torch.empty(md.B, dtype=torch.float64, device=dev)
     f_tmp = torch.empty(n_atoms, 3, dtype=torch.float32, device=dev)
-    for _ in range(3):
-        md.plan.energy_forces(md.pos32, e_tmp, f_tmp)
-    torch.cuda.synchronize()
-    a_ev = torch.cuda.Event(enable_timing=True)
-    b_ev = torch.cuda.Event(enable_timing=True)
-    reps = 20
-    a_ev.record()
-    for _ in range(reps):
-        md.plan.energy_forces(md.pos32, e_tmp, f_tmp)
-    b_ev.record()
-    b_ev.synchronize()
-    ef_s = a_ev.elapsed_time(b_ev) / 1e3 / reps
+    autotune = importlib.import_module("mlff-distiller_amd.autotune")
+    ef_s = autotune.time_evaluation(md.plan, md.pos32, e_tmp, f_tmp, reps=20)
     log(f"static E+F at t=0: {ef_s * 1e3:.3f} ms ({n_atoms / ef_s:.4g} atom-evals/s), "
-        f"E={E0}, P={P0}")
+        f"E={E0}, P={P0}, {args.streams} stream(s)")
+    ef_other = None
+    if args.streams == 2:
+        md.plan.set_streams(1)
+        ef_other = autotune.time_evaluation(md.plan, md.pos32, e_tmp, f_tmp, reps=20)
+        md.plan.set_streams(2)
+        log(f"  same evaluation on one stream: {ef_other * 1e3:.3f} ms")
     st = stage_times(md, reps=args.stage_reps)
     for k, v in st.items():
         log(f"  {k:12s} {v['avg_s'] * 1e6:9.1f} us/launch x{v['launches_per_step']}")
@@ -594,9 +592,11 @@ def main():
                    "edges_window_end": E_edges,
                    "note": "random-init forces are not restoring: molecules expand during "
                            "the run and the edge count falls over the window"},
-        "static_ef_t0": {"ms": ef_s * 1e3, "atom_evals_per_s": n_atoms / ef_s},
+        "static_ef_t0": {"ms": ef_s * 1e3, "atom_evals_per_s": n_atoms / ef_s,
+                         "streams": args.streams,
+                         "one_stream_ms": ef_other * 1e3 if ef_other else None},
         "autotune": ({"persistent_blocks": tuning["blocks"],
-                      "pair_kernels_us": {str(k): round(v, 1)
+                      "evaluation_us": {str(k): round(v, 1)
                                           for k, v in tuning["timings_us"].items()}}
                      if tuning else None),
         "accuracy": {"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de,

@@ -200,6 +200,13 @@ int mlff_launch_stage(mlff_plan* plan, void* stream, int stage, int layer);
 /* Set the persistent grid (workgroups) used by the pair kernels. */
 int mlff_plan_set_blocks(mlff_plan* plan, int blocks);
 
+/* Number of streams one evaluation uses: 2 (default) runs the pair filter
+ * kernels on a stream of the plan's own, forked from and joined back into the
+ * caller's stream with events, so they overlap the atom-side kernels; 1 keeps
+ * every kernel on the caller's stream.  Results are bit-identical either way.
+ * The filter stream is created on the device current at mlff_plan_bind. */
+int mlff_plan_set_streams(mlff_plan* plan, int n_streams);
+
 /* ---------------------------------------------------------------------------
  * Molecular dynamics.  Velocity Verlet as ASE's VelocityVerlet.step
  * (called from nve_harness.py:214-235): p += dt/2 F; x += dt p/m; F = F(x);

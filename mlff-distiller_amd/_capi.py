@@ -40,7 +40,7 @@ EXPORTS = (
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
     "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run", "mlff_launch_stage",
     "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
-    "mlff_mts_run", "mlff_mts_graph_run",
+    "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams",
 )
 
 STAGE_NEIGHBOR, STAGE_FILTER_FWD, STAGE_MESSAGE_FWD, STAGE_UPDATE_FWD = 0, 1, 2, 3
@@ -116,6 +116,8 @@ def _declare(lib):
     lib.mlff_launch_stage.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.mlff_plan_set_blocks.restype = ctypes.c_int
     lib.mlff_plan_set_blocks.argtypes = [vp, ctypes.c_int]
+    lib.mlff_plan_set_streams.restype = ctypes.c_int
+    lib.mlff_plan_set_streams.argtypes = [vp, ctypes.c_int]
     lib.mlff_plan_set_cell_grid.restype = ctypes.c_int
     lib.mlff_plan_set_cell_grid.argtypes = [vp, vp, ctypes.c_int]
     lib.mlff_mts_bind.restype = ctypes.c_int

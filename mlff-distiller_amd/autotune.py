@@ -1,10 +1,12 @@
 """Launch-shape autotuner for the persistent pair kernels.
 
 The pair kernels (filter forward and backward) run a persistent grid whose
-size trades load balance against L2 locality and occupancy.  The tuner times
-each candidate grid on the plan's own workload with HIP events (one stage
-launch per repetition, on the stream the stage runs on), keeps the fastest,
-and remembers the choice per (device, hidden size, pair-count bucket).
+size trades load balance against L2 locality and occupancy, and, with the
+filter kernels on their own stream, against the room left for the atom-side
+kernels running beside them.  The tuner times each candidate grid on the
+plan's own workload with HIP events around whole evaluations (neighbour list,
+forward, adjoint: the unit the caller repeats), keeps the fastest, and
+remembers the choice per (device, hidden size, streams, pair-count bucket).
 """
 
 from __future__ import annotations
@@ -25,7 +27,8 @@ def _key(plan) -> str:
     name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
     counts = plan.read_counts()
     pairs = max(int(counts[C.MLFF_COUNT_PAIRS]), 1)
-    return f"{name}|H{plan.H}|L{plan.L}|P2^{int(math.log2(pairs))}"
+    streams = getattr(plan, "streams", 2)
+    return f"{name}|H{plan.H}|L{plan.L}|S{streams}|P2^{int(math.log2(pairs))}"
 
 
 def default_candidates(cus: Optional[int] = None) -> Iterable[int]:
@@ -43,6 +46,21 @@ def time_stage(plan, stage: int, layer: int = 0, reps: int = 10) -> float:
     a.record(s)
     for _ in range(reps):
         C.check("stage", lib.mlff_launch_stage(plan.handle, s.cuda_stream, stage, layer))
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
+
+
+def time_evaluation(plan, positions: torch.Tensor, e: torch.Tensor, f: torch.Tensor,
+                    reps: int = 10) -> float:
+    """Seconds per full energy + forces evaluation on the current stream."""
+    s = torch.cuda.current_stream()
+    plan.energy_forces(positions, e, f, stream=s)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        plan.energy_forces(positions, e, f, stream=s)
     b.record(s)
     b.synchronize()
     return a.elapsed_time(b) / 1e3 / reps
@@ -72,9 +90,7 @@ def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = No
     timings = {}
     for blocks in cands:
         C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, blocks))
-        t = (time_stage(plan, C.STAGE_FILTER_FWD, 0, reps) +
-             time_stage(plan, C.STAGE_FILTER_BWD, 0, reps))
-        timings[blocks] = t * 1e6
+        timings[blocks] = time_evaluation(plan, positions, e, f, reps) * 1e6
     best = min(timings, key=timings.get)
     C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, best))
     _CACHE[key] = best

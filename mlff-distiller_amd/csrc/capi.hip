@@ -111,6 +111,11 @@ struct mlff_plan {
   float* f_outer; float* f_inner; float* f_mix; double* e_outer; double* e_inner;
   const uint8_t* flag; float* dis_max; int n_inner;
   hipGraph_t mts_graph; hipGraphExec_t mts_exec; int mts_graph_steps;
+  // filter stream (created at bind on the current device) and its fork/join events
+  int n_streams;
+  int stream_dev;
+  hipStream_t aux;
+  hipEvent_t ev[2 * 8 + 2];
   // workspace carve-up (byte offsets)
   struct { int64_t off, bytes; } seg[32];
   int nseg;
@@ -151,8 +156,8 @@ void carve(mlff_plan* pl) {
   sz[S_G12] = 4 * L * N * 2 * H;
   sz[S_GS0] = 4 * N * H;
   sz[S_GV0] = 4 * N * 3 * H;
-  sz[S_GS1] = 4 * N * H;
-  sz[S_GV1] = 4 * N * 3 * H;
+  sz[S_GS1] = 4 * L * N * H;
+  sz[S_GV1] = 4 * L * N * 3 * H;
   sz[S_EPS] = 4 * N;
   sz[S_VIR] = 4 * 9 * N;
   const int64_t CC = c.cell_capacity > 0 ? c.cell_capacity : 0;
@@ -170,6 +175,36 @@ void carve(mlff_plan* pl) {
   }
   pl->nseg = S_COUNT_;
   pl->ws_bytes = off;
+}
+
+void release_streams(mlff_plan* pl) {
+  for (hipEvent_t& e : pl->ev)
+    if (e) { (void)hipEventDestroy(e); e = nullptr; }
+  if (pl->aux) { (void)hipStreamDestroy(pl->aux); pl->aux = nullptr; }
+  pl->stream_dev = -1;
+}
+
+// The filter stream lives on the device current at bind time (the device of
+// the workspace).  Default priority, non-blocking: only the events order it.
+hipError_t acquire_streams(mlff_plan* pl) {
+  pl->buf.aux = nullptr;
+  pl->buf.ev = pl->ev;
+  if (pl->n_streams < 2) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (pl->aux && pl->stream_dev != dev) release_streams(pl);
+  if (!pl->aux) {
+    e = hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking);
+    if (e != hipSuccess) { pl->aux = nullptr; return e; }
+    for (hipEvent_t& ev : pl->ev) {
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) { ev = nullptr; release_streams(pl); return e; }
+    }
+    pl->stream_dev = dev;
+  }
+  pl->buf.aux = pl->aux;
+  return hipSuccess;
 }
 
 template <typename T>
@@ -259,6 +294,8 @@ int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
     blocks = 2 * cus;
   }
   pl->buf.blocks = (blocks + 7) / 8 * 8;
+  pl->n_streams = 2;
+  pl->stream_dev = -1;
   *out = pl;
   return MLFF_OK;
 }
@@ -269,6 +306,7 @@ int mlff_plan_destroy(mlff_plan* pl) {
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->mts_exec) (void)hipGraphExecDestroy(pl->mts_exec);
   if (pl->mts_graph) (void)hipGraphDestroy(pl->mts_graph);
+  release_streams(pl);
   delete pl;
   return MLFF_OK;
 }
@@ -343,6 +381,8 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   pl->grid_dev = segp<int4>(pl, ws, S_GRID);
   hipError_t e = hipMemset(b.counts, 0, 4 * MLFF_COUNT_SLOTS);
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
+  e = acquire_streams(pl);
+  if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind: filter stream");
   pl->bound = true;
   pl->md_bound = false;
   pl->mts_bound = false;
@@ -426,6 +466,24 @@ int mlff_plan_set_cell_grid(mlff_plan* pl, const int32_t* grid_host, int n_struc
   pl->nl.n_cells = used;
   if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
+int mlff_plan_set_streams(mlff_plan* pl, int n_streams) {
+  if (!pl || n_streams < 1 || n_streams > 2) {
+    set_error("mlff_plan_set_streams: 1 or 2 streams");
+    return MLFF_ERR_INVALID;
+  }
+  pl->n_streams = n_streams;
+  if (pl->bound) {
+    if (n_streams < 2) release_streams(pl);
+    hipError_t e = acquire_streams(pl);
+    if (e != hipSuccess) return hip_fail(e, "mlff_plan_set_streams");
+  }
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
   return MLFF_OK;
 }
 

@@ -68,9 +68,14 @@ struct Buffers {
   float* S; float* V;               // [L+1][N][H], [L+1][N][3][H]
   float* Smid; float* Vmid;         // [L][N][H], [L][N][3][H]
   float* Aupd; float* G12;          // [L][N][H], [L][N][2H]
-  float* gS0; float* gV0; float* gS1; float* gV1;  // gradient ping-pong
+  float* gS0; float* gV0;           // gradient w.r.t. the update outputs of the block below
+  float* gS1; float* gV1;           // [L][N][H], [L][N][3][H]: gradient w.r.t. message outputs
   float* eps;                       // [N]
   float* vir_atom;                  // [N][9]
+  // second stream for the filter kernels (overlaps the memory-bound message
+  // kernels); ev holds 3L+2 events.  aux == nullptr: one stream.
+  hipStream_t aux;
+  hipEvent_t* ev;
 };
 hipError_t launch_forward(const Buffers& b, hipStream_t s);
 hipError_t launch_backward(const Buffers& b, hipStream_t s, float* forces, bool want_virial);

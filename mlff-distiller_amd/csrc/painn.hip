@@ -724,12 +724,39 @@ __global__ void copy_f32_kernel(int n, const float* __restrict__ src, float* __r
 
 inline size_t nh(const Buffers& b) { return (size_t)b.N * b.H; }
 
+// Fork/join between the caller's stream and the filter stream.  Events order
+// the two streams both eagerly and under graph capture (a wait on an event
+// recorded in a capturing stream pulls the other stream into the capture).
+struct Streams {
+  hipStream_t main, filt;
+  hipEvent_t* ev;
+  hipError_t err = hipSuccess;
+  Streams(const Buffers& b, hipStream_t s) : main(s), filt(b.aux ? b.aux : s), ev(b.ev) {}
+  bool two() const { return filt != main; }
+  // `to` waits for everything issued so far on `from`
+  void order(hipStream_t from, hipStream_t to, int k) {
+    if (!two() || err != hipSuccess) return;
+    err = hipEventRecord(ev[k], from);
+    if (err == hipSuccess) err = hipStreamWaitEvent(to, ev[k], 0);
+  }
+};
+
 template <int H>
 hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   const ModelW& mw = *b.mw;
   const int N = b.N;
   const float* centers = b.w + mw.centers;
   const float* widths = b.w + mw.widths;
+  Streams st(b, s);
+  // the filters depend on the pair geometry only: all L of them run on the
+  // filter stream while the caller's stream walks the atom-side chain
+  st.order(s, st.filt, 0);
+  for (int l = 0; l < b.L; ++l) {
+    filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, st.filt>>>(
+        b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc,
+        b.Wf + (size_t)l * b.pcap * 3 * H);
+    if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
+  }
   {
     const int64_t total = (int64_t)N * H;
     embed_kernel<<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(
@@ -741,8 +768,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   for (int l = 0; l < b.L; ++l) {
     const LayerW& lw = mw.layer[l];
     float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
-    filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
-                                                     widths, b.R, mw.RP, b.rc, Wl);
+    if (st.two() && st.err == hipSuccess) st.err = hipStreamWaitEvent(s, b.ev[1 + l], 0);
     float* Sl = b.S + (size_t)l * nh(b);
     float* Vl = b.V + (size_t)l * 3 * nh(b);
     float* Sm = b.Smid + (size_t)l * nh(b);
@@ -755,6 +781,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   }
   readout_kernel<H><<<atom_grid, kBlock, 0, s>>>(N, b.w, mw, b.S + (size_t)b.L * nh(b), b.eps,
                                                  b.gS0);
+  if (st.err != hipSuccess) return st.err;
   return hipGetLastError();
 }
 
@@ -767,30 +794,39 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
   const int rows_per_block = kBlock / (H / 4);
   const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
   const int atom_grid = (N + 31) / 32;
+  Streams st(b, s);
   // gradient w.r.t. the update outputs of the last block: (gS0, 0)
   float* gs2 = b.gS0;
   float* gv2 = nullptr;
   for (int l = b.L - 1; l >= 0; --l) {
     const LayerW& lw = mw.layer[l];
     const float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+    // block l's message gradient has a slice of its own, so the filter
+    // adjoint of block l (filter stream) and the message adjoint of block l
+    // and update adjoint of block l-1 (caller's stream) run side by side
+    float* gS1 = b.gS1 + (size_t)l * nh(b);
+    float* gV1 = b.gV1 + (size_t)l * 3 * nh(b);
     update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
-        b.G12 + (size_t)l * 2 * nh(b), b.gS1, b.gV1);
-    filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(
+        b.G12 + (size_t)l * 2 * nh(b), gS1, gV1);
+    st.order(s, st.filt, b.L + 1 + l);
+    filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, st.filt>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
-        b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
+        b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), gS1, gV1, b.pacc,
         b.gvec, l == b.L - 1, l == 0);
     if (l > 0) {
       message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
-                                                          b.ecap, b.geom, Wl, b.gS1, b.gV1,
+                                                          b.ecap, b.geom, Wl, gS1, gV1,
                                                           b.gS0, b.gV0);
       gs2 = b.gS0;
       gv2 = b.gV0;
     }
   }
+  st.order(st.filt, s, 2 * b.L + 1);
   force_gather_kernel<<<(N + 127) / 128, 128, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair, b.ecap,
                                                       b.gvec, b.geom, forces,
                                                       want_virial ? b.vir_atom : nullptr);
+  if (st.err != hipSuccess) return st.err;
   return hipGetLastError();
 }
 
@@ -826,22 +862,23 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
                                                      b.gS0);
       break;
     case MLFF_STAGE_UPDATE_BWD:
-      // reads gS0/gV0, writes gS1/gV1 (the buffers of the backward sweep)
+      // reads gS0/gV0, writes block l's slice of gS1/gV1
       update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
           N, b.w, lw, b.gS0, (l == b.L - 1) ? nullptr : b.gV0, b.Vmid + (size_t)l * 3 * nh(b),
-          b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b), b.gS1, b.gV1);
+          b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b), b.gS1 + (size_t)l * nh(b),
+          b.gV1 + (size_t)l * 3 * nh(b));
       break;
     case MLFF_STAGE_FILTER_BWD:
       // first=1 overwrites the pair accumulator, so repeated launches agree
       filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(
           b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
-          b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1, b.gV1, b.pacc,
-          b.gvec, 1, 0);
+          b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1 + (size_t)l * nh(b),
+          b.gV1 + (size_t)l * 3 * nh(b), b.pacc, b.gvec, 1, 0);
       break;
     case MLFF_STAGE_MESSAGE_BWD:
-      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
-                                                          b.ecap, b.geom, Wl, b.gS1, b.gV1,
-                                                          b.gS0, b.gV0);
+      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(
+          N, b.row_ptr, b.col, b.edge_pair, b.ecap, b.geom, Wl, b.gS1 + (size_t)l * nh(b),
+          b.gV1 + (size_t)l * 3 * nh(b), b.gS0, b.gV0);
       break;
     case MLFF_STAGE_FORCES:
       force_gather_kernel<<<(N + 127) / 128, 128, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,

@@ -178,12 +178,16 @@ def edge_capacity_bound(sizes: Sequence[int], periodic: bool, max_neighbors: int
 # plans
 # ---------------------------------------------------------------------------
 
+# streams per evaluation for new plans (Plan.set_streams)
+DEFAULT_STREAMS = 2
+
+
 class Plan:
     """One bound plan: workspace + per-batch constants + weights."""
 
     def __init__(self, H, L, R, max_z, cutoff, layout: BatchLayout, numbers: torch.Tensor,
                  weights: torch.Tensor, edge_capacity: int, pair_capacity: Optional[int] = None,
-                 persistent_blocks: int = 0):
+                 persistent_blocks: int = 0, streams: Optional[int] = None):
         self.lib = C.lib()
         self.layout = layout
         self.H, self.L, self.R, self.max_z, self.cutoff = H, L, R, max_z, float(cutoff)
@@ -211,6 +215,9 @@ class Plan:
         off = counts_addr - self.workspace.data_ptr()
         self.counts = self.workspace[off:off + 4 * C.MLFF_COUNT_SLOTS].view(torch.int32)
         self._apply_cell_grid()
+        self.streams = int(streams if streams is not None else DEFAULT_STREAMS)
+        if self.streams != 2:
+            self.set_streams(self.streams)
 
     def _apply_cell_grid(self):
         if int(self.cell_grid[:, 0].max(initial=0)) > 0:
@@ -234,6 +241,13 @@ class Plan:
         C.check("mlff_energy_forces", self.lib.mlff_energy_forces(
             self.handle, C.stream_handle(stream), C.ptr(positions), C.ptr(energy),
             C.ptr(forces), C.ptr(virial), C.ptr(atom_energy)))
+
+    def set_streams(self, n_streams: int):
+        """1: every kernel on the caller's stream; 2 (default): the pair filter
+        kernels on a second stream of the plan's own, overlapping the
+        atom-side kernels.  Bit-identical results either way."""
+        C.check("mlff_plan_set_streams", self.lib.mlff_plan_set_streams(self.handle, int(n_streams)))
+        self.streams = int(n_streams)
 
     def read_counts(self) -> np.ndarray:
         return self.counts.cpu().numpy()

@@ -31,7 +31,7 @@ class BatchedNVE:
     def __init__(self, model, structures: Sequence["synth.Structure"], timestep_fs: float = 0.5,
                  temperature_K: float = 300.0, seed: int = 0, momenta: Optional[np.ndarray] = None,
                  max_steps: int = 100000, headroom: float = 1.3, device="cuda:0",
-                 persistent_blocks: int = 0):
+                 persistent_blocks: int = 0, streams: Optional[int] = None):
         self.model = model
         self.device = torch.device(device)
         self.structures = list(structures)
@@ -70,6 +70,7 @@ class BatchedNVE:
         self.weights = model.packed_weights(dev)
         self._headroom = headroom
         self._persistent_blocks = persistent_blocks
+        self._streams = streams
         # size the neighbour capacity from the starting configuration
         self._make_plan(self._initial_edges() * headroom)
         self.steps_done = 0
@@ -117,7 +118,7 @@ class BatchedNVE:
         self.plan = engine.Plan(self.model.hidden_dim, self.model.num_interactions,
                                 self.model.num_rbf, self.model.max_z, self.model.cutoff,
                                 self.layout, self.numbers, self.weights, cap,
-                                persistent_blocks=self._persistent_blocks)
+                                persistent_blocks=self._persistent_blocks, streams=self._streams)
         self.lib = C.lib()
         C.check("mlff_md_bind", self.lib.mlff_md_bind(
             self.plan.handle, C.ptr(self.x), C.ptr(self.p), C.ptr(self.inv_mass),

@@ -109,3 +109,22 @@ def test_periodic_water_md_runs(gpu):
     s = md.summary()
     assert np.isfinite(s["final_energy"]).all()
     assert abs(s["energy_drift_per_atom"][0]) < 1e-3
+
+
+def test_graph_with_filter_stream_matches_one_stream(gpu):
+    """Captured MD chunks with the filter kernels on the plan's second stream
+    replay to the same bits as the one-stream graph and as eager steps."""
+    note("MD graph: two streams vs one stream")
+    synth = pkg_mod("synth")
+    md_mod = pkg_mod("md")
+    model = _model()
+    mols = synth.molecule_set(16, 10, 60, seed=8)
+    two = md_mod.BatchedNVE(model, mols, seed=5, max_steps=40, streams=2)
+    one = md_mod.BatchedNVE(model, mols, seed=5, max_steps=40, streams=1)
+    eager = md_mod.BatchedNVE(model, mols, seed=5, max_steps=40, streams=2)
+    two.run(30, graph=True, steps_per_graph=10)
+    one.run(30, graph=True, steps_per_graph=10)
+    eager.run(30, graph=False)
+    for other in (one, eager):
+        assert torch.equal(two.x, other.x) and torch.equal(two.p, other.p)
+        assert np.array_equal(two.energies()[0], other.energies()[0])

@@ -245,3 +245,32 @@ def test_periodic_water_energy_forces_virial(gpu):
     assert abs(e64.item() - E.item()) / n < E_TOL
     assert np.abs(f.cpu().numpy() - F.numpy()).max() < F_TOL
     assert np.abs(vir.cpu().numpy()[0] - W.numpy()[0]).max() < 1e-3 * max(1.0, np.abs(W.numpy()).max())
+
+
+def test_filter_stream_overlap_is_bit_identical(gpu):
+    """The filter kernels on a second stream (fork/join by events) give the
+    same bits as every kernel on one stream: same kernels, same inputs."""
+    note("two streams vs one stream")
+    engine = pkg_mod("engine")
+    synth = pkg_mod("synth")
+    mols = _mols(8, 10, 100, seed=43)
+    Z, P, mol_ptr = synth.flatten(mols)
+    Zt = torch.tensor(Z, device=gpu)
+    Pt = torch.tensor(P.astype(np.float32), device=gpu)
+    box = synth.water_box(5, seed=3)
+    bz = torch.tensor(box.numbers, device=gpu)
+    bp = torch.tensor(box.positions.astype(np.float32), device=gpu)
+    outs = {}
+    old = engine.DEFAULT_STREAMS
+    try:
+        for n in (2, 1):
+            engine.DEFAULT_STREAMS = n
+            m = _model(128, 3, 20, seed=0)
+            mol = m.evaluate(Zt, Pt, np.diff(mol_ptr))
+            water = m.energy_forces_virial(bz, bp, torch.tensor(box.cell), torch.tensor(box.pbc))
+            assert all(v["plan"].streams == n for v in m._plans.values())
+            outs[n] = [mol[0], mol[1]] + list(water)
+    finally:
+        engine.DEFAULT_STREAMS = old
+    for a, b in zip(outs[2], outs[1]):
+        assert torch.equal(a, b)
