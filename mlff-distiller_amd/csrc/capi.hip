@@ -36,7 +36,8 @@ int build_layout(int H, int L, int R, int max_z, ModelW* mw, mlff_slot* slots, i
     return MLFF_ERR_UNSUPPORTED;
   }
   if (L < 1 || L > 8) { set_error("num_interactions %d outside 1..8", L); return MLFF_ERR_UNSUPPORTED; }
-  if (R < 1 || R > 64) { set_error("num_rbf %d outside 1..64", R); return MLFF_ERR_UNSUPPORTED; }
+  // the filter adjoint keeps d e_k/d d of a tile in LDS for at most 32 basis functions
+  if (R < 1 || R > 32) { set_error("num_rbf %d outside 1..32", R); return MLFF_ERR_UNSUPPORTED; }
   if (max_z < 1) { set_error("max_z must be >= 1"); return MLFF_ERR_INVALID; }
   ModelW m;
   memset(&m, 0, sizeof(m));

@@ -34,26 +34,34 @@ __device__ __forceinline__ float silu_grad(float x) {
   return s * (1.0f + x * (1.0f - s));
 }
 
-// Gaussian RBF x cosine cutoff of one basis function (student_model.py:249-253,
-// 285-290, 720-722) and its derivative with respect to d.
-__device__ __forceinline__ float edge_feature(float d, float mu, float width, float rc) {
-  float diff = d - mu;
-  float gamma = 1.0f / (width * width);
-  float phi = expf(-gamma * (diff * diff));
-  float fc = 0.5f * (cosf(kPi * d / rc) + 1.0f);
-  fc = (d < rc) ? fc : 0.0f;
-  return phi * fc;
-}
-__device__ __forceinline__ float edge_feature_grad(float d, float mu, float width, float rc) {
-  float diff = d - mu;
-  float gamma = 1.0f / (width * width);
-  float phi = expf(-gamma * (diff * diff));
-  float arg = kPi * d / rc;
-  float fc = 0.5f * (cosf(arg) + 1.0f);
-  float dfc = -0.5f * (kPi / rc) * sinf(arg);
+// Gaussian RBF x cosine cutoff (student_model.py:249-253, 285-290, 720-722),
+// split so the kernels evaluate the cutoff once per pair and the Gaussian once
+// per (pair, basis function):
+//   e_k(d) = exp(-gamma_k (d - mu_k)^2) * fc(d),  gamma_k = 1 / width_k^2,
+//   fc(d) = (cos(pi d / rc) + 1) / 2 for d < rc, else 0,
+//   de_k/dd = -2 gamma_k (d - mu_k) phi_k fc + phi_k dfc/dd.
+__device__ __forceinline__ float rbf_gamma(float width) { return 1.0f / (width * width); }
+
+__device__ __forceinline__ void cosine_cutoff(float d, float rc, float& fc, float& dfc) {
+  const float arg = kPi * d / rc;
+  float s, c;
+  sincosf(arg, &s, &c);
+  fc = 0.5f * (c + 1.0f);
+  dfc = -0.5f * (kPi / rc) * s;
   if (!(d < rc)) { fc = 0.0f; dfc = 0.0f; }
-  float dphi = -2.0f * gamma * diff * phi;
-  return dphi * fc + phi * dfc;
+}
+
+__device__ __forceinline__ float edge_feature(float d, float mu, float gamma, float fc) {
+  const float diff = d - mu;
+  return expf(-gamma * (diff * diff)) * fc;
+}
+
+__device__ __forceinline__ void edge_feature_and_grad(float d, float mu, float gamma, float fc,
+                                                      float dfc, float& e, float& de) {
+  const float diff = d - mu;
+  const float phi = expf(-gamma * (diff * diff));
+  e = phi * fc;
+  de = (-2.0f * gamma * diff * phi) * fc + phi * dfc;
 }
 
 // ---------------------------------------------------------------------------

@@ -32,6 +32,15 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
 constexpr int kPairWaves = kPairBlock / 64;
+constexpr int kMaxRbf = 32;
+#ifndef MLFF_MSG_UNROLL_FWD
+#define MLFF_MSG_UNROLL_FWD 3
+#endif
+#ifndef MLFF_MSG_UNROLL_BWD
+#define MLFF_MSG_UNROLL_BWD 4
+#endif
+constexpr int kMsgUnrollFwd = MLFF_MSG_UNROLL_FWD;   // edges in flight per row (message forward)
+constexpr int kMsgUnrollBwd = MLFF_MSG_UNROLL_BWD;   // (message adjoint)           // radial basis size limit (padded to 16: 16 or 32)
 
 __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
                              const float* __restrict__ emb, float* __restrict__ S0,
@@ -55,13 +64,20 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
     const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout) {
   constexpr int TM = 64, RT = 4;
-  constexpr int LDE = 64 + 4;
+  constexpr int LDE = kMaxRbf + 4;
   constexpr int LDH = H + 4;
   constexpr int CT1 = H / 16, NT1 = (CT1 + kPairWaves - 1) / kPairWaves;
   constexpr int CT2 = 3 * H / 16, NT2 = (CT2 + kPairWaves - 1) / kPairWaves;
   static_assert(CT1 % NT1 == 0 && CT2 % NT2 == 0, "tile split");
   __shared__ __align__(16) float e_lds[TM * LDE];
   __shared__ __align__(16) float h_lds[TM * LDH];
+  __shared__ float mu_lds[kMaxRbf], gam_lds[kMaxRbf];
+  __shared__ float d_lds[TM], fc_lds[TM];
+  if (threadIdx.x < kMaxRbf) {
+    const int k = threadIdx.x;
+    mu_lds[k] = k < R ? centers[k] : 0.f;
+    gam_lds[k] = k < R ? rbf_gamma(widths[k]) : 0.f;
+  }
   const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
   const int ntiles = (P + TM - 1) / TM;
   int t0, t1;
@@ -74,11 +90,22 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
   const float* f2b = w + lw.f2b;
   for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
+    if (threadIdx.x < TM) {   // the cutoff once per pair
+      const int p = pbase + threadIdx.x;
+      float d = 0.f, fc = 0.f, dfc;
+      if (p < P) {
+        d = geom[p].w;
+        cosine_cutoff(d, rc, fc, dfc);
+      }
+      d_lds[threadIdx.x] = d;
+      fc_lds[threadIdx.x] = fc;
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP;
       const int p = pbase + m;
       float val = 0.f;
-      if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
+      if (p < P && k < R) val = edge_feature(d_lds[m], mu_lds[k], gam_lds[k], fc_lds[m]);
       e_lds[m * LDE + k] = val;
     }
     __syncthreads();
@@ -147,34 +174,60 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
   const int c = 4 * sub;
   f4 as = {0.f, 0.f, 0.f, 0.f}, a0 = as, a1 = as, a2 = as;
   const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
-  for (int e = e0; e < e1; ++e) {
+  // operands of one edge; kUnroll edges are loaded before any is accumulated
+  // (more loads in flight per row), and they are accumulated in edge order,
+  // so the sums round exactly as a one-edge-at-a-time loop
+  struct Edge {
+    f4 ws, wv1, wv2, sj, v0, v1, v2;
+    float ux, uy, uz;
+    bool on;
+  };
+  auto load = [&](int e) {
+    Edge q;
     const int j = col[e];
     const int p = edge_pair[e];
-    if (p < 0) continue;
-    const float* wp = Wf + (size_t)p * 3 * H + c;
+    q.on = p >= 0;
+    const int pc = q.on ? p : 0;
+    const float* wp = Wf + (size_t)pc * 3 * H + c;
     const float* vj = v_in + (size_t)j * 3 * H + c;
-    const f4 ws = *reinterpret_cast<const f4*>(wp);
-    const f4 wv1 = *reinterpret_cast<const f4*>(wp + H);
-    const f4 sj = *reinterpret_cast<const f4*>(s_in + (size_t)j * H + c);
-    const f4 v0 = *reinterpret_cast<const f4*>(vj);
-    const f4 v1 = *reinterpret_cast<const f4*>(vj + H);
-    const f4 v2 = *reinterpret_cast<const f4*>(vj + 2 * H);
-    as += sj * ws;
-    a0 += v0 * wv1;
-    a1 += v1 * wv1;
-    a2 += v2 * wv1;
+    q.ws = *reinterpret_cast<const f4*>(wp);
+    q.wv1 = *reinterpret_cast<const f4*>(wp + H);
+    q.sj = *reinterpret_cast<const f4*>(s_in + (size_t)j * H + c);
+    q.v0 = *reinterpret_cast<const f4*>(vj);
+    q.v1 = *reinterpret_cast<const f4*>(vj + H);
+    q.v2 = *reinterpret_cast<const f4*>(vj + 2 * H);
     if (FWD) {
-      const f4 wv2 = *reinterpret_cast<const f4*>(wp + 2 * H);
-      const float4 gm = geom[p];
+      q.wv2 = *reinterpret_cast<const f4*>(wp + 2 * H);
+      const float4 gm = geom[pc];
       const float den = gm.w + kEdgeEps;
       // u_ji = (p_j - p_i)/(d + eps): +pair vector when j is the pair's hi atom
-      float ux = gm.x / den, uy = gm.y / den, uz = gm.z / den;
-      if (j < i) { ux = -ux; uy = -uy; uz = -uz; }
-      a0 += ux * wv2;
-      a1 += uy * wv2;
-      a2 += uz * wv2;
+      q.ux = gm.x / den; q.uy = gm.y / den; q.uz = gm.z / den;
+      if (j < i) { q.ux = -q.ux; q.uy = -q.uy; q.uz = -q.uz; }
     }
+    return q;
+  };
+  auto accumulate = [&](const Edge& q) {
+    if (!q.on) return;
+    as += q.sj * q.ws;
+    a0 += q.v0 * q.wv1;
+    a1 += q.v1 * q.wv1;
+    a2 += q.v2 * q.wv1;
+    if (FWD) {
+      a0 += q.ux * q.wv2;
+      a1 += q.uy * q.wv2;
+      a2 += q.uz * q.wv2;
+    }
+  };
+  constexpr int kUnroll = FWD ? kMsgUnrollFwd : kMsgUnrollBwd;
+  int e = e0;
+  for (; e + kUnroll - 1 < e1; e += kUnroll) {
+    Edge q[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) q[u] = load(e + u);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) accumulate(q[u]);
   }
+  for (; e < e1; ++e) accumulate(load(e));
   const float* vi = v_in + (size_t)i * 3 * H + c;
   float* vo = v_out + (size_t)i * 3 * H + c;
   *reinterpret_cast<f4*>(s_out + (size_t)i * H + c) =
@@ -493,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 // last block processed turns it into dE/dvec.
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
+__global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     const int32_t* __restrict__ counts, int32_t pcap, const int32_t* __restrict__ pair_lo,
     const int32_t* __restrict__ pair_hi, const float4* __restrict__ geom,
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
@@ -502,13 +555,22 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
     const float* __restrict__ gs1, const float* __restrict__ gv1, float4* __restrict__ pacc,
     float4* __restrict__ gvec, int first, int last) {
   constexpr int TM = 32, RT = 2;
-  constexpr int LDE = 64 + 4, LDG = 3 * H + 4, LDA = H + 4;
+  constexpr int LDE = kMaxRbf + 4, LDG = 3 * H + 4, LDA = H + 4;
   constexpr int C4 = H / 4;                    // lanes per pair in the gather (4 channels each)
   constexpr int NPART = 1;                     // one dE/du partial per pair
   __shared__ __align__(16) float e_lds[TM * LDE];
   __shared__ __align__(16) float gw_lds[TM * LDG];
   __shared__ __align__(16) float ga_lds[TM * LDA];
   __shared__ float gu_lds[TM * NPART * 3];
+  __shared__ float ed_lds[TM * kMaxRbf];       // d e_k / d d of each pair
+  __shared__ float gd_lds[2 * TM];             // dL/dd partials of the two basis halves
+  __shared__ float mu_lds[kMaxRbf], gam_lds[kMaxRbf];
+  __shared__ float d_lds[TM], fc_lds[TM], dfc_lds[TM];
+  if (threadIdx.x < kMaxRbf) {
+    const int k = threadIdx.x;
+    mu_lds[k] = k < R ? centers[k] : 0.f;
+    gam_lds[k] = k < R ? rbf_gamma(widths[k]) : 0.f;
+  }
   const int P = min(counts[MLFF_COUNT_PAIRS], pcap);
   const int ntiles = (P + TM - 1) / TM;
   int t0, t1;
@@ -517,13 +579,29 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
+    if (threadIdx.x < TM) {   // the cutoff once per pair
+      const int p = pbase + threadIdx.x;
+      float d = 0.f, fc = 0.f, dfc = 0.f;
+      if (p < P) {
+        d = geom[p].w;
+        cosine_cutoff(d, rc, fc, dfc);
+      }
+      d_lds[threadIdx.x] = d;
+      fc_lds[threadIdx.x] = fc;
+      dfc_lds[threadIdx.x] = dfc;
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < TM * RP; idx += kPairBlock) {
       const int m = idx / RP, k = idx - m * RP, p = pbase + m;
-      float val = 0.f;
-      if (p < P && k < R) val = edge_feature(geom[p].w, centers[k], widths[k], rc);
+      float val = 0.f, der = 0.f;
+      if (p < P && k < R)
+        edge_feature_and_grad(d_lds[m], mu_lds[k], gam_lds[k], fc_lds[m], dfc_lds[m], val, der);
       e_lds[m * LDE + k] = val;
+      ed_lds[m * kMaxRbf + k] = der;
     }
-    // gather both endpoints, four channels per lane, every load issued up front
+    // gather both endpoints, four channels per lane, every load of one item
+    // issued up front (items one at a time: two blocks per CU fit the registers)
+#pragma unroll 1
     for (int idx = threadIdx.x; idx < TM * C4; idx += kPairBlock) {
       const int m = idx / C4, c = 4 * (idx - m * C4), p = pbase + m;
       f4 gws = {0.f, 0.f, 0.f, 0.f}, gwv1 = gws, gwv2 = gws;
@@ -602,7 +680,9 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
       }
     }
     __syncthreads();
-    float* ge_lds = e_lds;   // RBF inputs are dead after the barrier above
+    // dL/de = ga W1 (basis columns k = 16 wave + r), then dL/dd = sum_k
+    // dL/de_k de_k/dd: products in registers, sum over the 16 columns of the
+    // wave by shuffles, the two halves added in a fixed order below
     if (wave < RP / 16) {
       int cb[1] = {wave * 16};
       f4 acc[RT][1];
@@ -611,7 +691,12 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) ge_lds[(rt * 16 + 4 * g + k) * LDE + cb[0] + r] = acc[rt][0][k];
+        for (int k = 0; k < 4; ++k) {
+          const int m = rt * 16 + 4 * g + k;
+          float t = acc[rt][0][k] * ed_lds[m * kMaxRbf + cb[0] + r];
+          t = group_sum<16>(t);
+          if (r == 0) gd_lds[wave * TM + m] = t;
+        }
     }
     __syncthreads();
     if (threadIdx.x < TM) {
@@ -619,9 +704,7 @@ __global__ __launch_bounds__(kPairBlock) void filter_bwd_kernel(
       if (p < P) {
         const float4 gm = geom[p];
         const float d = gm.w;
-        float gd = 0.f;
-        for (int k = 0; k < R; ++k)
-          gd += ge_lds[m * LDE + k] * edge_feature_grad(d, centers[k], widths[k], rc);
+        const float gd = (RP > 16) ? gd_lds[m] + gd_lds[TM + m] : gd_lds[m];
         float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
         for (int part = 0; part < NPART; ++part) {
