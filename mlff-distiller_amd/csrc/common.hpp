@@ -26,11 +26,13 @@ __device__ __forceinline__ float norm3_exact(float x, float y, float z) {
   return __builtin_sqrtf(acc);
 }
 
-__device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
+// SiLU with the library exponential and a correctly rounded division (the
+// faster forms tried, hardware exp2 and reciprocal, drift by a few ulp in one
+// direction, which adds up over channels and pairs when activations are large)
 __device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
 // d/dx [x sigmoid(x)] = s (1 + x (1 - s))
 __device__ __forceinline__ float silu_grad(float x) {
-  float s = 1.0f / (1.0f + expf(-x));
+  const float s = 1.0f / (1.0f + expf(-x));
   return s * (1.0f + x * (1.0f - s));
 }
 
@@ -127,9 +129,17 @@ __device__ __forceinline__ int xcd_block() {
   return (b % 8) * (G / 8) + b / 8;
 }
 
+// sched_barrier mask: VALU (0x2), SALU (0x4), all LDS (0x80, 0x100, 0x200)
+// and transcendentals (0x400) may cross; global memory and MFMA may not, so
+// each slab's MFMAs stay between its own barrier and the next one
+constexpr int kSchedSlab = 0x0002 | 0x0004 | 0x0080 | 0x0100 | 0x0200 | 0x0400;
+
 // Same product with K fixed at compile time: the slab loop unrolls fully and
 // the weight fetches run PF slabs ahead of the MFMAs that consume them.
-template <int RT, int NT, int K, int PF = 2>
+// PIN keeps each slab's prefetch above its MFMAs with a scheduling barrier
+// (the compiler otherwise may sink the loads next to their use to save
+// registers); which is faster depends on the kernel's occupancy.
+template <int RT, int NT, int K, int PF = 2, bool PIN = false>
 __device__ __forceinline__ void mma_tile_k(const float* __restrict__ A, int lda,
                                            const float* __restrict__ B, int ldb,
                                            const int (&cb)[NT], f4 (&acc)[RT][NT]) {
@@ -139,22 +149,25 @@ __device__ __forceinline__ void mma_tile_k(const float* __restrict__ A, int lda,
   const int r = lane & 15;
   const int g = lane >> 4;
   const float* arow = A + r * lda + 4 * g;
-  const float* brow[NT];
+  // uniform weight base + 32-bit per-lane offsets (one register per column
+  // tile instead of a 64-bit pointer)
+  int boff[NT];
 #pragma unroll
-  for (int ct = 0; ct < NT; ++ct) brow[ct] = B + (size_t)(cb[ct] + r) * ldb + 4 * g;
+  for (int ct = 0; ct < NT; ++ct) boff[ct] = (cb[ct] + r) * ldb + 4 * g;
   f4 b[PF + 1][NT];
 #pragma unroll
   for (int q = 0; q < PF; ++q)
 #pragma unroll
     for (int ct = 0; ct < NT; ++ct)
-      if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(brow[ct] + 16 * q);
+      if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * q);
 #pragma unroll
   for (int sl = 0; sl < S; ++sl) {
     if (sl + PF < S) {
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct)
-        b[(sl + PF) % (PF + 1)][ct] = *reinterpret_cast<const f4*>(brow[ct] + 16 * (sl + PF));
+        b[(sl + PF) % (PF + 1)][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * (sl + PF));
     }
+    if (PIN) __builtin_amdgcn_sched_barrier(kSchedSlab);
     f4 a[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)

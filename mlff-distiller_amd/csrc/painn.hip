@@ -137,18 +137,30 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
       for (int ct = 0; ct < NT2; ++ct) cb[ct] = (wave * NT2 + ct) * 16;
       zero_acc(acc);
       mma_tile_k<RT, NT2, H>(h_lds, LDH, f2w, H, cb, acc);
+      float bias[NT2];
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
+      for (int ct = 0; ct < NT2; ++ct) bias[ct] = f2b[cb[ct] + r];
+      // lane (r, g) holds rows rt*16 + 4g + i, columns cb[ct] + r
+      float* out = Wout + (size_t)(pbase + 4 * g) * 3 * H + r;
+      if (pbase + TM <= P) {   // full tile: no per-element bounds checks
 #pragma unroll
-        for (int ct = 0; ct < NT2; ++ct) {
-          const int c = cb[ct] + r;
-          const float bias = f2b[c];
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int ct = 0; ct < NT2; ++ct)
+              out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]] = acc[rt][ct][i] + bias[ct];
+      } else {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int p = pbase + rt * 16 + 4 * g + i;
-            if (p < P) Wout[(size_t)p * 3 * H + c] = acc[rt][ct][i] + bias;
+            if (pbase + rt * 16 + 4 * g + i >= P) continue;
+#pragma unroll
+            for (int ct = 0; ct < NT2; ++ct)
+              out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]] = acc[rt][ct][i] + bias[ct];
           }
-        }
+      }
     }
     __syncthreads();
   }
@@ -478,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
       zero_acc(acc);
-      mma_tile_k<RT, NT, 3 * H>(y_lds, LDY, w + lw.u2t, 3 * H, cb, acc);
+      mma_tile_k<RT, NT, 3 * H, 2, true>(y_lds, LDY, w + lw.u2t, 3 * H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -501,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
       zero_acc(acc);
-      mma_tile_k<RT, NT, H>(a_lds, LDA, w + lw.u1t, H, cb, acc);
+      mma_tile_k<RT, NT, H, 2, true>(a_lds, LDA, w + lw.u1t, H, cb, acc);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -664,7 +676,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
         zero_acc(pre);
         zero_acc(acc);
         mma_tile<RT, NT>(e_lds, LDE, w + lw.f1w, RP, RP, cb, pre);
-        mma_tile_k<RT, NT, 3 * H>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
+        mma_tile_k<RT, NT, 3 * H, 2, true>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
         const float* b1 = w + lw.f1b;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
