@@ -441,6 +441,8 @@ def main():
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=20)
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="persistent grid of the pair kernels (0: autotune)")
     ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
                     help="2: pair filter kernels on a second stream beside the atom-side kernels")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -471,8 +473,9 @@ def main():
     log(f"{len(mols)} molecules, {n_atoms} atoms")
     total_steps = args.warmup + args.steps
     md = mdmod.BatchedNVE(model, mols, timestep_fs=0.5, temperature_K=300.0, seed=args.seed,
-                          max_steps=total_steps + 1, streams=args.streams)
-    tuning = md.autotune() if not args.no_autotune else None
+                          max_steps=total_steps + 1, streams=args.streams,
+                          persistent_blocks=args.blocks)
+    tuning = md.autotune() if not (args.no_autotune or args.blocks) else None
     if tuning:
         log(f"autotune: persistent grid {tuning['blocks']} workgroups "
             f"{ {k: round(v, 1) for k, v in tuning['timings_us'].items()} }")
