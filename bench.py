@@ -376,7 +376,7 @@ def config5(args, pkg, synth, mdmod, dev):
     full = pkg.StudentForceField().to(dev).eval()
     t0 = time.perf_counter()
     frames = mts.md_frames(full, synth.molecule_set(256, 10, 100, seed=2), n_frames=args.frames,
-                           stride=5, seed=2)
+                           stride=args.frame_stride, seed=2)
     t_frames = time.perf_counter() - t0
     torch.manual_seed(0)
     student = pkg.StudentForceField(hidden_dim=32, num_interactions=2, num_rbf=16).to(dev)
@@ -387,27 +387,30 @@ def config5(args, pkg, synth, mdmod, dev):
     student.eval()
     mols = synth.molecule_set(512, 10, 100, seed=5)
     n_atoms = sum(len(m) for m in mols)
-    outer = args.mts_outer
-    ref = mdmod.BatchedNVE(full, mols, seed=5, max_steps=4 * outer + 1)
+    n_in = args.mts_inner
+    outer = args.mts_outer * 4 // n_in          # the same simulated time for every n_inner
+    steps = n_in * outer
+    ref = mdmod.BatchedNVE(full, mols, seed=5, max_steps=steps + 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ref.run(4 * outer, graph=True, steps_per_graph=20)
+    ref.run(steps, graph=True, steps_per_graph=20)
     torch.cuda.synchronize()
     t_full = time.perf_counter() - t0
     m = mts.MTSNVE(full, student, mols, seed=5, threshold=args.mts_threshold,
-                   max_outer=outer + 1)
+                   n_inner=n_in, max_outer=outer + 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     m.run(outer, chunk=10)
     torch.cuda.synchronize()
     t_mts = time.perf_counter() - t0
     sf, sm = ref.summary(), m.summary()
-    full_rate = n_atoms * 4 * outer / t_full
-    mts_rate = n_atoms * 4 * outer / t_mts
+    full_rate = n_atoms * steps / t_full
+    mts_rate = n_atoms * steps / t_mts
     return _common("MTS atom-steps/s (0.5 fs equivalent) vs full-model MD", mts_rate,
-                   "atom-steps/s", 4 * outer, 0, t_mts, "f32",
-                   f"512 molecules N~U{{10..100}} (seed 5), {outer * 2e-3:.2f} ps NVE; inner "
-                   "compact student H32/L2/R16 at 0.5 fs, outer full H128/L3/R20 at 2 fs",
+                   "atom-steps/s", steps, 0, t_mts, "f32",
+                   f"512 molecules N~U{{10..100}} (seed 5), {steps * 0.5e-3:.2f} ps NVE; inner "
+                   f"compact student H32/L2/R16 at 0.5 fs, outer full H128/L3/R20 at "
+                   f"{0.5 * n_in:g} fs",
                    n_atoms=n_atoms,
                    full_md={"value": full_rate, "unit": "atom-steps/s",
                             "max_abs_drift_eV_per_atom": float(np.abs(sf["energy_drift_per_atom"]).max()),
@@ -417,7 +420,8 @@ def config5(args, pkg, synth, mdmod, dev):
                         "median_abs_drift_eV_per_atom": float(np.median(np.abs(sm["energy_drift_per_atom"]))),
                         "fallback_structure_steps": sm["fallback_structure_steps"],
                         "replayed_chunks": sm["replays"], "threshold_eV_per_A": args.mts_threshold},
-                   student={"frames": frames.n_frames, "frame_seconds": t_frames,
+                   student={"frames": frames.n_frames, "frame_stride_steps": args.frame_stride,
+                            "frame_seconds": t_frames,
                             "train_steps": args.train_steps, "train_seconds": t_train,
                             "loss_first": hist[0], "loss_last_mean10": float(np.mean(hist[-10:]))})
 
@@ -426,9 +430,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=80)
-    ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--frame-stride", type=int, default=100,
+                    help="MD steps between training frames (config 5)")
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--mts-outer", type=int, default=5000)
     ap.add_argument("--mts-threshold", type=float, default=5.0)
+    ap.add_argument("--mts-inner", type=int, default=4,
+                    help="inner 0.5 fs steps per outer step (config 5)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)

@@ -254,12 +254,12 @@ class Plan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
+        self.handle = None
         if h is not None and h.value:
             try:
                 self.lib.mlff_plan_destroy(h)
             except Exception:
                 pass
-            self.handle = C.vp()
 
 
 def ctypes_byref(x):

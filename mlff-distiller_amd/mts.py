@@ -80,21 +80,27 @@ def md_frames(full_model, structures, n_frames: int, stride: int, seed: int = 0,
                     seed=seed, max_steps=n_frames * stride + 1)
     fs = FrameSet(md.numbers.cpu().numpy(), md.sizes, md.device)
     fs.add(md.pos32, md.forces, md.energy)
+    per_graph = max(d for d in range(1, 26) if stride % d == 0)
     for _ in range(n_frames - 1):
-        md.run(stride, graph=stride >= 5, steps_per_graph=stride if stride >= 5 else 1)
+        md.run(stride, graph=per_graph >= 5, steps_per_graph=per_graph)
         fs.add(md.pos32, md.forces, md.energy)
     return fs
 
 
 def train_student(student, frames: FrameSet, steps: int, batch_molecules: int = 64,
                   lr: float = 1e-3, w_energy: float = 1.0, w_force: float = 100.0,
-                  seed: int = 0, log_every: int = 0):
-    """Fit the compact student on the frames; returns the loss history."""
+                  seed: int = 0, log_every: int = 0, lr_final: float = 0.02):
+    """Fit the compact student on the frames; returns the loss history.
+    The learning rate follows a cosine from ``lr`` down to ``lr_final * lr``."""
+    import math
     from .train import ForceMatchingTrainer
     tr = ForceMatchingTrainer(student, lr=lr, w_energy=w_energy, w_force=w_force)
     rng = np.random.Generator(np.random.PCG64(seed))
     hist = []
     for k in range(steps):
+        c = 0.5 * (1.0 + math.cos(math.pi * k / max(steps - 1, 1)))
+        for group in tr.opt.param_groups:
+            group["lr"] = lr * (lr_final + (1.0 - lr_final) * c)
         Z, x, sizes, E, F = frames.batch(rng, batch_molecules)
         hist.append(tr.step(Z, x, sizes, E, F))
         if log_every and (k % log_every == 0 or k == steps - 1):
