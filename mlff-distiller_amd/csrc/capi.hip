@@ -118,7 +118,7 @@ struct mlff_plan {
   hipStream_t aux;
   hipEvent_t ev[2 * 8 + 2];
   // workspace carve-up (byte offsets)
-  struct { int64_t off, bytes; } seg[32];
+  struct { int64_t off, bytes; } seg[48];
   int nseg;
 };
 
@@ -127,13 +127,15 @@ namespace {
 enum Seg {
   S_COUNTS, S_DEG, S_UDEG, S_ROWPTR, S_PAIRPTR, S_COL, S_EPAIR, S_PLO, S_PHI, S_PSHIFT, S_GEOM,
   S_PACC, S_GVEC, S_WF, S_S, S_V, S_SMID, S_VMID, S_AUPD, S_G12, S_GS0, S_GV0, S_GS1, S_GV1,
-  S_EPS, S_VIR, S_GRID, S_CELLCNT, S_CELLSTART, S_CELLATOMS, S_ATOMCELL, S_TMPSHIFT, S_COUNT_
+  S_EPS, S_VIR, S_GRID, S_CELLCNT, S_CELLSTART, S_CELLATOMS, S_ATOMCELL, S_TMPSHIFT, S_WBF,
+  S_COUNT_
 };
 
 void carve(mlff_plan* pl) {
   const mlff_config& c = pl->cfg;
   const int64_t N = c.n_atoms, H = c.hidden_dim, L = c.num_interactions;
   const int64_t E = c.edge_capacity, P = c.pair_capacity;
+  static_assert(S_COUNT_ <= (int)(sizeof(pl->seg) / sizeof(pl->seg[0])), "segment table too small");
   int64_t sz[S_COUNT_] = {};
   sz[S_COUNTS] = 4 * MLFF_COUNT_SLOTS;
   sz[S_DEG] = 4 * N;
@@ -168,6 +170,7 @@ void carve(mlff_plan* pl) {
   sz[S_CELLATOMS] = CC ? 4 * N : 0;
   sz[S_ATOMCELL] = CC ? 4 * N : 0;
   sz[S_TMPSHIFT] = CC ? 12 * E : 0;
+  sz[S_WBF] = 2 * wbf_elems((int)H, (int)L);
   int64_t off = 0;
   for (int k = 0; k < S_COUNT_; ++k) {
     pl->seg[k].off = off;
@@ -360,6 +363,7 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   b.gV1 = segp<float>(pl, ws, S_GV1);
   b.eps = segp<float>(pl, ws, S_EPS);
   b.vir_atom = segp<float>(pl, ws, S_VIR);
+  b.wbf = segp<uint16_t>(pl, ws, S_WBF);
   NLArgs& a = pl->nl;
   memset(&a, 0, sizeof(a));
   a.N = b.N; a.B = b.B; a.mol_ptr = mol_ptr_dev; a.atom_mol = atom_mol_dev;
@@ -384,6 +388,8 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
   e = acquire_streams(pl);
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind: filter stream");
+  e = split_filter_weights(b);
+  if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind: bf16 weight planes");
   pl->bound = true;
   pl->md_bound = false;
   pl->mts_bound = false;

@@ -184,6 +184,82 @@ __device__ __forceinline__ void mma_tile_k(const float* __restrict__ A, int lda,
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 products on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Each float32 operand x is split into three bfloat16 parts, x = h + m + l
+// (each part the round-to-nearest-even bfloat16 of what the previous parts
+// leave), which carry 24 significant bits between them.  A product a b is
+// then the sum of the six part products whose magnitude is at least 2^-16 of
+// |h_a h_b| (l_a h_b, h_a l_b, m_a m_b, m_a h_b, h_a m_b, h_a h_b, small terms
+// first); each part product is exact in float32 and the MFMA accumulates in
+// float32, so the result carries float32 accuracy at 6/16 of the FP32 MFMA
+// cost.  The three parts sit in three planes of the same shape.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__host__ __device__ inline uint16_t bf16_rne(float x) {
+  uint32_t u = __builtin_bit_cast(uint32_t, x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+__device__ __forceinline__ void split3_bf16(float x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  h = bf16_rne(x);
+  const float r = x - bf16_to_f32(h);
+  m = bf16_rne(r);
+  l = bf16_rne(r - bf16_to_f32(m));
+}
+
+// acc[rt][ct] += A[rt*16 .. +16][0..K) * B[cb[ct] .. +16][0..K)^T with A in
+// LDS and B in global memory, both as three bf16 planes, row-major with row
+// strides lda / ldb (elements, multiples of 8) and plane strides pa / pb.
+// Lane (r, g) supplies row r of each operand and 8 consecutive k of slot
+// group g; the MFMA sums over all 32 k of the slab, so which k the hardware
+// pairs with which slot does not matter as long as A and B agree.
+template <int RT, int NT, int K>
+__device__ __forceinline__ void mma_tile_x6(const uint16_t* __restrict__ A, int lda, int pa,
+                                            const uint16_t* __restrict__ B, int ldb, int pb,
+                                            const int (&cb)[NT], f4 (&acc)[RT][NT]) {
+  static_assert(K % 32 == 0, "K must be a multiple of 32");
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const uint16_t* arow = A + r * lda + 8 * g;
+  int boff[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) boff[ct] = (cb[ct] + r) * ldb + 8 * g;
+#pragma unroll 1
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf8 bh[NT], bm[NT], bl[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+      bh[ct] = *reinterpret_cast<const bf8*>(B + boff[ct] + k0);
+      bm[ct] = *reinterpret_cast<const bf8*>(B + pb + boff[ct] + k0);
+      bl[ct] = *reinterpret_cast<const bf8*>(B + 2 * pb + boff[ct] + k0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const uint16_t* ar = arow + rt * 16 * lda + k0;
+      const bf8 ah = *reinterpret_cast<const bf8*>(ar);
+      const bf8 am = *reinterpret_cast<const bf8*>(ar + pa);
+      const bf8 al = *reinterpret_cast<const bf8*>(ar + 2 * pa);
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        f4 c = acc[rt][ct];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[ct], c, 0, 0, 0);
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ct], c, 0, 0, 0);
+      }
+    }
+  }
+}
+
 template <int RT, int NT>
 __device__ __forceinline__ void zero_acc(f4 (&acc)[RT][NT]) {
 #pragma unroll

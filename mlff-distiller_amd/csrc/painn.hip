@@ -59,18 +59,20 @@ __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
 // filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for 64 pairs per tile
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
+__global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
     const float4* __restrict__ geom, const int32_t* __restrict__ counts, int32_t pcap,
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
-    const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout) {
+    const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout,
+    const uint16_t* __restrict__ w2bf) {
   constexpr int TM = 64, RT = 4;
   constexpr int LDE = kMaxRbf + 4;
-  constexpr int LDH = H + 4;
+  constexpr int LDHB = H + 8;          // bf16 row stride of the split hidden activations
+  constexpr int PHB = TM * LDHB;       // plane stride
   constexpr int CT1 = H / 16, NT1 = (CT1 + kPairWaves - 1) / kPairWaves;
   constexpr int CT2 = 3 * H / 16, NT2 = (CT2 + kPairWaves - 1) / kPairWaves;
   static_assert(CT1 % NT1 == 0 && CT2 % NT2 == 0, "tile split");
   __shared__ __align__(16) float e_lds[TM * LDE];
-  __shared__ __align__(16) float h_lds[TM * LDH];
+  __shared__ __align__(16) uint16_t h_bf[3 * PHB];   // silu(W1 e + b1) as three bf16 planes
   __shared__ float mu_lds[kMaxRbf], gam_lds[kMaxRbf];
   __shared__ float d_lds[TM], fc_lds[TM];
   if (threadIdx.x < kMaxRbf) {
@@ -86,7 +88,6 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const float* f1w = w + lw.f1w;
   const float* f1b = w + lw.f1b;
-  const float* f2w = w + lw.f2w;
   const float* f2b = w + lw.f2b;
   for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
@@ -125,7 +126,11 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int m = rt * 16 + 4 * g + i;
-            h_lds[m * LDH + c] = silu(acc[rt][ct][i] + bias);
+            uint16_t hh, hm, hl;
+            split3_bf16(silu(acc[rt][ct][i] + bias), hh, hm, hl);
+            h_bf[m * LDHB + c] = hh;
+            h_bf[PHB + m * LDHB + c] = hm;
+            h_bf[2 * PHB + m * LDHB + c] = hl;
           }
         }
     }
@@ -136,7 +141,8 @@ __global__ __launch_bounds__(kPairBlock) void filter_fwd_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT2; ++ct) cb[ct] = (wave * NT2 + ct) * 16;
       zero_acc(acc);
-      mma_tile_k<RT, NT2, H>(h_lds, LDH, f2w, H, cb, acc);
+      // W_p = h W2^T + b2 on the bf16 matrix cores at float32 accuracy
+      mma_tile_x6<RT, NT2, H>(h_bf, LDHB, PHB, w2bf, H, (int)wbf_plane(H), cb, acc);
       float bias[NT2];
 #pragma unroll
       for (int ct = 0; ct < NT2; ++ct) bias[ct] = f2b[cb[ct] + r];
@@ -817,6 +823,27 @@ __global__ void copy_f32_kernel(int n, const float* __restrict__ src, float* __r
   if (i < n) dst[i] = src[i];
 }
 
+// bf16 planes of W2 [3H][H] and W2^T [H][3H] of every block (both dense in the blob)
+__global__ void split_filter_weights_kernel(const float* __restrict__ w, ModelW mw,
+                                            uint16_t* __restrict__ wbf) {
+  const int H = mw.H;
+  const int64_t S = wbf_plane(H);
+  const int64_t total = (int64_t)mw.L * 2 * S;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(idx / (2 * S));
+    const int mat = (int)((idx / S) % 2);
+    const int64_t e = idx % S;
+    const float x = w[(mat == 0 ? mw.layer[l].f2w : mw.layer[l].f2t) + e];
+    uint16_t h, m, lo;
+    split3_bf16(x, h, m, lo);
+    uint16_t* dst = wbf + (mat == 0 ? wbf_f2w(H, l) : wbf_f2t(H, l)) + e;
+    dst[0] = h;
+    dst[S] = m;
+    dst[2 * S] = lo;
+  }
+}
+
 inline size_t nh(const Buffers& b) { return (size_t)b.N * b.H; }
 
 // Fork/join between the caller's stream and the filter stream.  Events order
@@ -849,7 +876,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   for (int l = 0; l < b.L; ++l) {
     filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, st.filt>>>(
         b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc,
-        b.Wf + (size_t)l * b.pcap * 3 * H);
+        b.Wf + (size_t)l * b.pcap * 3 * H, b.wbf + wbf_f2w(H, l));
     if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
   }
   {
@@ -939,7 +966,8 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   switch (stage) {
     case MLFF_STAGE_FILTER_FWD:
       filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
-                                                       widths, b.R, mw.RP, b.rc, Wl);
+                                                       widths, b.R, mw.RP, b.rc, Wl,
+                                                       b.wbf + wbf_f2w(H, l));
       break;
     case MLFF_STAGE_MESSAGE_FWD:
       message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(
@@ -987,6 +1015,17 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
 }
 
 }  // namespace
+
+hipError_t split_filter_weights(const Buffers& b) {
+  if (b.wbf == nullptr || b.L < 1) return hipSuccess;
+  // the weights may have just been written on another stream
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  split_filter_weights_kernel<<<256, 256, 0, nullptr>>>(b.w, *b.mw, b.wbf);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipDeviceSynchronize();
+}
 
 hipError_t launch_stage(const Buffers& b, hipStream_t s, int stage, int layer) {
   if (layer < 0 || layer >= b.L) return hipErrorInvalidValue;
