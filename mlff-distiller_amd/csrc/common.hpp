@@ -260,6 +260,64 @@ __device__ __forceinline__ void mma_tile_x6(const uint16_t* __restrict__ A, int 
   }
 }
 
+// mma_tile_k with the k sub-steps of every slab split over two accumulator
+// sets (even and odd), added at the end: twice the independent MFMA chains
+// for tiles with few output blocks per wave.
+template <int RT, int NT, int K, int PF = 2, bool PIN = false>
+__device__ __forceinline__ void mma_tile_k2(const float* __restrict__ A, int lda,
+                                            const float* __restrict__ B, int ldb,
+                                            const int (&cb)[NT], f4 (&acc)[RT][NT]) {
+  static_assert(K % 16 == 0, "K must be a multiple of 16");
+  constexpr int S = K / 16;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const float* arow = A + r * lda + 4 * g;
+  int boff[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) boff[ct] = (cb[ct] + r) * ldb + 4 * g;
+  f4 odd[RT][NT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) odd[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+  f4 b[PF + 1][NT];
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct)
+      if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * q);
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    if (sl + PF < S) {
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct)
+        b[(sl + PF) % (PF + 1)][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * (sl + PF));
+    }
+    if (PIN) __builtin_amdgcn_sched_barrier(kSchedSlab);
+    f4 a[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+      a[rt] = *reinterpret_cast<const f4*>(arow + rt * 16 * lda + 16 * sl);
+    const int cur = sl % (PF + 1);
+#pragma unroll
+    for (int s = 0; s < 4; s += 2)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b[cur][ct][s], acc[rt][ct],
+                                                             0, 0, 0);
+          odd[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s + 1], b[cur][ct][s + 1],
+                                                             odd[rt][ct], 0, 0, 0);
+        }
+  }
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) acc[rt][ct] += odd[rt][ct];
+}
+
 template <int RT, int NT>
 __device__ __forceinline__ void zero_acc(f4 (&acc)[RT][NT]) {
 #pragma unroll

@@ -554,6 +554,27 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
   }
 }
 
+// Partial dL/dd of a tile: (ga[:, k0:k0+KC] W1t[c0:c0+16, k0:k0+KC]^T) times
+// de/dd for the 16 basis columns c0 + r, summed over those columns.
+template <int RT, int KC, int TM>
+__device__ __forceinline__ void gd_partial(const float* ga, int lda, const float* w1t, int ldb,
+                                           int c0, int k0, const float* ed, float* out) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  int cb[1] = {c0};
+  f4 acc[RT][1];
+  zero_acc(acc);
+  mma_tile_k<RT, 1, KC>(ga + k0, lda, w1t + k0, ldb, cb, acc);
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = rt * 16 + 4 * g + i;
+      float t = acc[rt][0][i] * ed[m * kMaxRbf + c0 + r];
+      t = group_sum<16>(t);
+      if (r == 0) out[m] = t;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // filter backward, 32 pairs per tile.
 // For pair p = (a < b): edge b->a carries s_b*Ws, v_b*Wv1 + u (x) Wv2 to a
@@ -581,7 +602,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
   __shared__ __align__(16) float ga_lds[TM * LDA];
   __shared__ float gu_lds[TM * NPART * 3];
   __shared__ float ed_lds[TM * kMaxRbf];       // d e_k / d d of each pair
-  __shared__ float gd_lds[2 * TM];             // dL/dd partials of the two basis halves
+  __shared__ float gd_lds[kPairWaves * TM];    // dL/dd partials, one set per wave
   __shared__ float mu_lds[kMaxRbf], gam_lds[kMaxRbf];
   __shared__ float d_lds[TM], fc_lds[TM], dfc_lds[TM];
   if (threadIdx.x < kMaxRbf) {
@@ -682,7 +703,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
         zero_acc(pre);
         zero_acc(acc);
         mma_tile<RT, NT>(e_lds, LDE, w + lw.f1w, RP, RP, cb, pre);
-        mma_tile_k<RT, NT, 3 * H, 2, true>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
+        mma_tile_k2<RT, NT, 3 * H, 2, true>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
         const float* b1 = w + lw.f1b;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -698,23 +719,20 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
       }
     }
     __syncthreads();
-    // dL/de = ga W1 (basis columns k = 16 wave + r), then dL/dd = sum_k
-    // dL/de_k de_k/dd: products in registers, sum over the 16 columns of the
-    // wave by shuffles, the two halves added in a fixed order below
-    if (wave < RP / 16) {
-      int cb[1] = {wave * 16};
-      f4 acc[RT][1];
-      zero_acc(acc);
-      mma_tile_k<RT, 1, H>(ga_lds, LDA, w + lw.f1t, H, cb, acc);
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int m = rt * 16 + 4 * g + k;
-          float t = acc[rt][0][k] * ed_lds[m * kMaxRbf + cb[0] + r];
-          t = group_sum<16>(t);
-          if (r == 0) gd_lds[wave * TM + m] = t;
-        }
+    // dL/dd = sum_k (ga W1)_k de_k/dd.  All waves share the W1 product: wave
+    // w takes basis columns 16 (w % ctiles) + r and the K (hidden) slice
+    // w / ctiles; each multiplies its partial dL/de by de/dd, sums its 16
+    // columns by shuffles, and the partials of all waves are added in wave
+    // order below (the sum is linear in dL/de).
+    const int ctiles = RP / 16;
+    if (ctiles == 2) {
+      constexpr int NP = (H / 16 < 4) ? H / 16 : 4, KC = H / NP;
+      if (wave < 2 * NP) gd_partial<RT, KC, TM>(ga_lds, LDA, w + lw.f1t, H, (wave % 2) * 16,
+                                                (wave / 2) * KC, ed_lds, gd_lds + wave * TM);
+    } else {
+      constexpr int NP = (H / 16 < 8) ? H / 16 : 8, KC = H / NP;
+      if (wave < NP) gd_partial<RT, KC, TM>(ga_lds, LDA, w + lw.f1t, H, 0, wave * KC, ed_lds,
+                                            gd_lds + wave * TM);
     }
     __syncthreads();
     if (threadIdx.x < TM) {
@@ -722,7 +740,9 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
       if (p < P) {
         const float4 gm = geom[p];
         const float d = gm.w;
-        const float gd = (RP > 16) ? gd_lds[m] + gd_lds[TM + m] : gd_lds[m];
+        const int nparts = (RP > 16) ? 2 * ((H / 16 < 4) ? H / 16 : 4) : ((H / 16 < 8) ? H / 16 : 8);
+        float gd = 0.f;
+        for (int q = 0; q < nparts; ++q) gd += gd_lds[q * TM + m];
         float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
         for (int part = 0; part < NPART; ++part) {
