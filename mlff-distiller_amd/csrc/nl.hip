@@ -263,40 +263,74 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
   udeg[i] = nu;
 }
 
-// Single-workgroup exclusive scan of deg and udeg (N up to a few million).
+// Single-workgroup exclusive scan of deg and udeg (any N), in chunks of
+// 1024 x kScanPer elements: coalesced loads into LDS, a contiguous scan of
+// kScanPer elements per thread, wave prefix sums by shuffles, one LDS pass
+// over the 16 wave totals, coalesced stores, and a carry into the next chunk.
+constexpr int kScanPer = 8;
+constexpr int kScanChunk = 1024 * kScanPer;
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(1024) void nl_scan_kernel(const int32_t* __restrict__ deg,
                                                        const int32_t* __restrict__ udeg, int N,
                                                        int32_t* __restrict__ row_ptr,
                                                        int32_t* __restrict__ pair_ptr,
                                                        int32_t* __restrict__ counts,
                                                        int32_t ecap, int32_t pcap) {
-  __shared__ int s1[1024];
-  __shared__ int s2[1024];
-  const int t = threadIdx.x;
-  const int per = (N + 1023) / 1024;
-  const int lo = min(N, t * per), hi = min(N, lo + per);
-  int a1 = 0, a2 = 0;
-  for (int i = lo; i < hi; ++i) { a1 += deg[i]; a2 += udeg[i]; }
-  s1[t] = a1;
-  s2[t] = a2;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v1 = (t >= off) ? s1[t - off] : 0;
-    int v2 = (t >= off) ? s2[t - off] : 0;
+  __shared__ int b1[kScanChunk];
+  __shared__ int b2[kScanChunk];
+  __shared__ int w1[16], w2[16];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int carry1 = 0, carry2 = 0;
+  for (int base = 0; base < N; base += kScanChunk) {
+    const int n = min(kScanChunk, N - base);
+    for (int i = t; i < kScanChunk; i += 1024) {
+      b1[i] = i < n ? deg[base + i] : 0;
+      b2[i] = i < n ? udeg[base + i] : 0;
+    }
     __syncthreads();
-    s1[t] += v1;
-    s2[t] += v2;
+    int x1[kScanPer], x2[kScanPer];
+    int s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      x1[k] = s1; s1 += b1[t * kScanPer + k];
+      x2[k] = s2; s2 += b2[t * kScanPer + k];
+    }
+    const int i1 = wave_inclusive_scan(s1), i2 = wave_inclusive_scan(s2);
+    if (lane == 63) { w1[wv] = i1; w2[wv] = i2; }
     __syncthreads();
+    int o1 = carry1, o2 = carry2;
+    for (int q = 0; q < wv; ++q) { o1 += w1[q]; o2 += w2[q]; }
+    o1 += i1 - s1;
+    o2 += i2 - s2;
+    int tot1 = carry1, tot2 = carry2;
+    for (int q = 0; q < 16; ++q) { tot1 += w1[q]; tot2 += w2[q]; }
+    __syncthreads();   // every thread has read its chunk elements and the wave totals
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      b1[t * kScanPer + k] = o1 + x1[k];
+      b2[t * kScanPer + k] = o2 + x2[k];
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += 1024) {
+      row_ptr[base + i] = b1[i];
+      pair_ptr[base + i] = b2[i];
+    }
+    carry1 = tot1;
+    carry2 = tot2;
+    __syncthreads();   // the next chunk overwrites b1, b2, w1, w2
   }
-  int r1 = s1[t] - a1, r2 = s2[t] - a2;
-  for (int i = lo; i < hi; ++i) {
-    row_ptr[i] = r1;
-    pair_ptr[i] = r2;
-    r1 += deg[i];
-    r2 += udeg[i];
-  }
-  if (t == 1023) {
-    const int E = s1[1023], P = s2[1023];
+  if (t == 0) {
+    const int E = carry1, P = carry2;
     row_ptr[N] = E;
     pair_ptr[N] = P;
     counts[MLFF_COUNT_EDGES] = E;
