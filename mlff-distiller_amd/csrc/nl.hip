@@ -235,7 +235,10 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
                                 const float* __restrict__ cells, const uint8_t* __restrict__ pbc,
                                 float rc, int32_t* __restrict__ deg, int32_t* __restrict__ udeg,
                                 CellList cl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per atom: the lanes test 64 candidates at a time (brute force
+  // within the structure); a cell-list row is walked by lane 0
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= N) return;
   const int b = atom_mol[i];
   CellInfo ci;
@@ -243,24 +246,34 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
   load_cell(cells, pbc, b, ci, periodic);
   const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   int n = 0, nu = 0;
-  auto test = [&](int j) {
-    if (j == i) return;
+  auto hit = [&](int j) {
     float vx, vy, vz, d;
     int s0, s1, s2;
-    if (within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic, ci, vx,
-               vy, vz, d, s0, s1, s2)) {
-      ++n;
-      nu += (j > i);
-    }
+    return within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic, ci,
+                  vx, vy, vz, d, s0, s1, s2);
   };
   if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
-    for_each_cell_candidate(cl, i, b, test);
+    if (lane == 0) {
+      for_each_cell_candidate(cl, i, b, [&](int j) {
+        if (j != i && hit(j)) {
+          ++n;
+          nu += (j > i);
+        }
+      });
+    }
   } else {
     const int j0 = mol_ptr[b], j1 = mol_ptr[b + 1];
-    for (int j = j0; j < j1; ++j) test(j);
+    for (int jb = j0; jb < j1; jb += 64) {
+      const int j = jb + lane;
+      const bool h = (j < j1) && (j != i) && hit(j);
+      n += __popcll(__ballot(h));
+      nu += __popcll(__ballot(h && j > i));
+    }
   }
-  deg[i] = n;
-  udeg[i] = nu;
+  if (lane == 0) {
+    deg[i] = n;
+    udeg[i] = nu;
+  }
 }
 
 // Single-workgroup exclusive scan of deg and udeg (any N), in chunks of
@@ -350,7 +363,9 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
                                int32_t* __restrict__ pair_hi, int32_t* __restrict__ pair_shift,
                                float4* __restrict__ geom, int32_t pcap, CellList cl,
                                int32_t* __restrict__ tmp_shift) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per atom, as the count pass
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= N) return;
   const int b = atom_mol[i];
   const int j0 = mol_ptr[b], j1 = mol_ptr[b + 1];
@@ -361,6 +376,7 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   int e = row_ptr[i];
   int p = pair_ptr[i];
   if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
+    if (lane != 0) return;
     // cell-list row: collect in cell order, sort by neighbour index, then emit
     const int e0 = e;
     for_each_cell_candidate(cl, i, b, [&](int j) {
@@ -417,33 +433,42 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
     }
     return;
   }
-  for (int j = j0; j < j1; ++j) {
-    if (j == i) continue;
-    float vx, vy, vz, d;
-    int s0, s1, s2;
-    if (!within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic, ci, vx,
-                vy, vz, d, s0, s1, s2))
-      continue;
-    const bool e_ok = e < ecap;
-    if (e_ok) {
-      col[e] = j;
-      if (edge_shift) {
-        edge_shift[3 * e] = s0; edge_shift[3 * e + 1] = s1; edge_shift[3 * e + 2] = s2;
-      }
-    }
-    if (j > i) {
-      if (pair_lo != nullptr && p < pcap) {
-        pair_lo[p] = i;
-        pair_hi[p] = j;
-        if (pair_shift) {
-          pair_shift[3 * p] = s0; pair_shift[3 * p + 1] = s1; pair_shift[3 * p + 2] = s2;
+  // 64 candidates per step; a hit's slot is the number of hits in lower
+  // lanes, so the row stays in ascending neighbour order
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int jb = j0; jb < j1; jb += 64) {
+    const int j = jb + lane;
+    float vx = 0.f, vy = 0.f, vz = 0.f, d = 0.f;
+    int s0 = 0, s1 = 0, s2 = 0;
+    const bool h = (j < j1) && (j != i) &&
+                   within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc, periodic,
+                          ci, vx, vy, vz, d, s0, s1, s2);
+    const uint64_t mh = __ballot(h);
+    const uint64_t mu = __ballot(h && j > i);
+    if (h) {
+      const int ek = e + __popcll(mh & below);
+      const bool e_ok = ek < ecap;
+      if (e_ok) {
+        col[ek] = j;
+        if (edge_shift) {
+          edge_shift[3 * ek] = s0; edge_shift[3 * ek + 1] = s1; edge_shift[3 * ek + 2] = s2;
         }
-        geom[p] = make_float4(vx, vy, vz, d);
       }
-      if (edge_pair != nullptr && e_ok) edge_pair[e] = (p < pcap) ? p : -1;
-      ++p;
+      if (j > i) {
+        const int pk = p + __popcll(mu & below);
+        if (pair_lo != nullptr && pk < pcap) {
+          pair_lo[pk] = i;
+          pair_hi[pk] = j;
+          if (pair_shift) {
+            pair_shift[3 * pk] = s0; pair_shift[3 * pk + 1] = s1; pair_shift[3 * pk + 2] = s2;
+          }
+          geom[pk] = make_float4(vx, vy, vz, d);
+        }
+        if (edge_pair != nullptr && e_ok) edge_pair[ek] = (pk < pcap) ? pk : -1;
+      }
     }
-    ++e;
+    e += __popcll(mh);
+    p += __popcll(mu);
   }
 }
 
@@ -502,11 +527,13 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
     cl_sort_cells_kernel<<<(C + 127) / 128, 128, 0, s>>>(C, a.cell_start, a.cell_atoms);
     cl = CellList{a.grid, a.cell_start, a.cell_atoms, a.atom_cell};
   }
-  nl_count_kernel<<<g, tb, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
+  // count and fill: one wave per atom
+  const int gw = (int)(((int64_t)a.N * 64 + 255) / 256);
+  nl_count_kernel<<<gw, 256, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                    a.deg, a.udeg, cl);
   nl_scan_kernel<<<1, 1024, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr, a.counts, a.ecap,
                                     build_pairs ? a.pcap : 0x7fffffff);
-  nl_fill_kernel<<<g, tb, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
+  nl_fill_kernel<<<gw, 256, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                   a.row_ptr, a.pair_ptr, a.col, a.edge_shift,
                                   build_pairs ? a.edge_pair : nullptr, a.ecap,
                                   build_pairs ? a.pair_lo : nullptr, a.pair_hi, a.pair_shift,
