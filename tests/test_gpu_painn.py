@@ -274,3 +274,26 @@ def test_filter_stream_overlap_is_bit_identical(gpu):
         engine.DEFAULT_STREAMS = old
     for a, b in zip(outs[2], outs[1]):
         assert torch.equal(a, b)
+
+
+def test_weight_update_refreshes_split_planes(gpu):
+    """The filter's second-layer weights also live as bf16 planes split at
+    bind; changing the weights in place must rebind the plan and refresh
+    them (a stale copy would give the old model's energies and forces)."""
+    note("in-place weight update -> new results")
+    model = _model(128, 3, 20, seed=0)
+    mols = _mols(6, 10, 60, seed=47)
+    synth = pkg_mod("synth")
+    Z, P, mol_ptr = synth.flatten(mols)
+    sizes = np.diff(mol_ptr)
+    P32 = P.astype(np.float32)
+    Zt, Pt = torch.tensor(Z, device=gpu), torch.tensor(P32, device=gpu)
+    e0, f0, _ = model.evaluate(Zt, Pt, sizes)
+    gen = torch.Generator().manual_seed(5)
+    with torch.no_grad():
+        for l in range(3):
+            w = model.interactions[l].message.rbf_to_scalar[2].weight
+            w.mul_(1.25).add_(0.02 * torch.randn(w.shape, generator=gen).to(w.device))
+    e1, f1, _ = model.evaluate(Zt, Pt, sizes)
+    assert not torch.equal(f0, f1)
+    _check_ef(model, Z, P32, sizes, e1.cpu().numpy(), f1.cpu().numpy(), "after weight update")
