@@ -104,6 +104,23 @@ def _edge_features_d2(d, centers, widths, rc):
     return d2phi * fc + 2 * dphi * dfc + phi * d2fc
 
 
+def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
+    """a.T @ b for tall a [K, M], b [K, N] (a weight gradient, K = pairs or
+    atoms): the K reduction is split into chunks computed as one batched
+    product and then summed, because a single long reduction leaves most
+    of the GPU idle in the library's kernels."""
+    K = a.shape[0]
+    if K < 4 * chunk:
+        return a.T @ b
+    nc = K // chunk
+    head = nc * chunk
+    at = a[:head].reshape(nc, chunk, a.shape[1]).transpose(1, 2).contiguous()
+    out = torch.bmm(at, b[:head].reshape(nc, chunk, b.shape[1]).contiguous()).sum(0)
+    if head < K:
+        out = out + a[head:].T @ b[head:]
+    return out
+
+
 def _lin(x, p, name):
     return x @ p[name + ".weight"].T + p[name + ".bias"]
 
@@ -117,12 +134,22 @@ def _add_pairs(n, lo, hi, to_lo, to_hi):
 
 
 def _mixT(M, v):
-    """(M^T v)_z = sum_x M[x][z] v_x over the xyz axis of [N,3,H]."""
-    return torch.einsum("xz,nxh->nzh", M, v)
+    """(M^T v)_z = sum_x M[x][z] v_x over the xyz axis of [N,3,H] (three
+    weighted sums: a 3x3 contraction as a batched product would launch one
+    tiny GEMM per atom)."""
+    v0, v1, v2 = v[:, 0], v[:, 1], v[:, 2]
+    return torch.stack([M[0, z] * v0 + M[1, z] * v1 + M[2, z] * v2 for z in range(3)], dim=1)
+
+
+def _xyz_outer(a, b):
+    """sum_{n,h} a[n,x,h] b[n,z,h] -> [3,3] (gradient of a 3x3 mixing matrix)."""
+    return _tmm(a.permute(0, 2, 1).reshape(-1, 3), b.permute(0, 2, 1).reshape(-1, 3))
 
 
 def _mix(M, v):
-    return torch.einsum("xz,nzh->nxh", M, v)
+    """(M v)_x = sum_z M[x][z] v_z over the xyz axis of [N,3,H]."""
+    v0, v1, v2 = v[:, 0], v[:, 1], v[:, 2]
+    return torch.stack([M[x, 0] * v0 + M[x, 1] * v1 + M[x, 2] * v2 for x in range(3)], dim=1)
 
 
 # ---------------------------------------------------------------------------
@@ -259,11 +286,11 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     acc(grads, "energy_head.4.bias", geps.sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
     gr2 = gh4 * _silu1(r2)
-    acc(grads, "energy_head.2.weight", gr2.T @ h3)
+    acc(grads, "energy_head.2.weight", _tmm(gr2, h3))
     acc(grads, "energy_head.2.bias", gr2.sum(0))
     gh3 = gr2 @ p["energy_head.2.weight"]
     gr1 = gh3 * _silu1(r1)
-    acc(grads, "energy_head.0.weight", gr1.T @ sL)
+    acc(grads, "energy_head.0.weight", _tmm(gr1, sL))
     acc(grads, "energy_head.0.bias", gr1.sum(0))
     gs = gr1 @ p["energy_head.0.weight"]
     gv = torch.zeros(n, 3, H, dtype=dt, device=gs.device)
@@ -271,11 +298,11 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         acc(tg, "energy_head.4.weight", (geps.unsqueeze(1) * dh4).sum(0, keepdim=True))
         acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=dt, device=gs.device))
         dgr2 = gh4 * _silu2(r2) * dr2
-        acc(tg, "energy_head.2.weight", dgr2.T @ h3 + gr2.T @ dh3)
+        acc(tg, "energy_head.2.weight", _tmm(dgr2, h3) + _tmm(gr2, dh3))
         acc(tg, "energy_head.2.bias", dgr2.sum(0))
         dgh3 = dgr2 @ p["energy_head.2.weight"]
         dgr1 = dgh3 * _silu1(r1) + gh3 * _silu2(r1) * dr1
-        acc(tg, "energy_head.0.weight", dgr1.T @ sL + gr1.T @ dsL)
+        acc(tg, "energy_head.0.weight", _tmm(dgr1, sL) + _tmm(gr1, dsL))
         acc(tg, "energy_head.0.bias", dgr1.sum(0))
         dgs = dgr1 @ p["energy_head.0.weight"]
         dgv = torch.zeros_like(gv)
@@ -295,16 +322,16 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gg1 = (gv * v_m).sum(1)
         gg2 = (gv * Mv).sum(1)
         gy = torch.cat([gs, gg1, gg2], dim=-1)
-        acc(grads, upre + ".update_mlp.2.weight", gy.T @ lay["h2"])
+        acc(grads, upre + ".update_mlp.2.weight", _tmm(gy, lay["h2"]))
         acc(grads, upre + ".update_mlp.2.bias", gy.sum(0))
         gh2 = gy @ U2
         ga2 = gh2 * _silu1(lay["a2"])
-        acc(grads, upre + ".update_mlp.0.weight", ga2.T @ lay["xin"])
+        acc(grads, upre + ".update_mlp.0.weight", _tmm(ga2, lay["xin"]))
         acc(grads, upre + ".update_mlp.0.bias", ga2.sum(0))
         gx = ga2 @ U1
         gs_x, gn = gx[:, :H], gx[:, H:]
         gvg2 = gv * g2.unsqueeze(1)
-        acc(grads, upre + ".mixing_matrix", torch.einsum("nxh,nzh->xz", gvg2, v_m))
+        acc(grads, upre + ".mixing_matrix", _xyz_outer(gvg2, v_m))
         safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
         unit_n = torch.where((nrm > 0).unsqueeze(1), v_m / safe.unsqueeze(1), torch.zeros_like(v_m))
         gs_m = gs + gs_x
@@ -315,17 +342,17 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             dgg1 = (dgv * v_m + gv * dvm).sum(1)
             dgg2 = (dgv * Mv + gv * dMv).sum(1)
             dgy = torch.cat([dgs, dgg1, dgg2], dim=-1)
-            acc(tg, upre + ".update_mlp.2.weight", dgy.T @ lay["h2"] + gy.T @ lay["dh2"])
+            acc(tg, upre + ".update_mlp.2.weight", _tmm(dgy, lay["h2"]) + _tmm(gy, lay["dh2"]))
             acc(tg, upre + ".update_mlp.2.bias", dgy.sum(0))
             dgh2 = dgy @ U2
             dga2 = dgh2 * _silu1(lay["a2"]) + gh2 * _silu2(lay["a2"]) * lay["da2"]
-            acc(tg, upre + ".update_mlp.0.weight", dga2.T @ lay["xin"] + ga2.T @ dxin)
+            acc(tg, upre + ".update_mlp.0.weight", _tmm(dga2, lay["xin"]) + _tmm(ga2, dxin))
             acc(tg, upre + ".update_mlp.0.bias", dga2.sum(0))
             dgx = dga2 @ U1
             dgs_x, dgn = dgx[:, :H], dgx[:, H:]
             dgvg2 = dgv * g2.unsqueeze(1) + gv * dg2.unsqueeze(1)
             acc(tg, upre + ".mixing_matrix",
-                torch.einsum("nxh,nzh->xz", dgvg2, v_m) + torch.einsum("nxh,nzh->xz", gvg2, dvm))
+                _xyz_outer(dgvg2, v_m) + _xyz_outer(gvg2, dvm))
             dnrm = lay["dnrm"]
             dunit = torch.where((nrm > 0).unsqueeze(1),
                                 dvm / safe.unsqueeze(1) - v_m * (dnrm / safe ** 2).unsqueeze(1),
@@ -345,11 +372,11 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gW = torch.cat([gWs, gWv1, gWv2], dim=-1)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        acc(grads, mpre + ".2.weight", gW.T @ lay["h"])
+        acc(grads, mpre + ".2.weight", _tmm(gW, lay["h"]))
         acc(grads, mpre + ".2.bias", gW.sum(0))
         gh = gW @ W2
         ga1 = gh * _silu1(lay["a1"])
-        acc(grads, mpre + ".0.weight", ga1.T @ c["e"])
+        acc(grads, mpre + ".0.weight", _tmm(ga1, c["e"]))
         acc(grads, mpre + ".0.bias", ga1.sum(0))
         ge = ga1 @ W1
         gd = gd + (ge * c["de"]).sum(-1)
@@ -365,11 +392,11 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             ddgv_ab = dgv_m[ga] - dgv_m[gb]
             dgWv2 = (du.unsqueeze(-1) * dgv_ab).sum(1) + (u.unsqueeze(-1) * ddgv_ab).sum(1)
             dgW = torch.cat([dgWs, dgWv1, dgWv2], dim=-1)
-            acc(tg, mpre + ".2.weight", dgW.T @ lay["h"] + gW.T @ dh)
+            acc(tg, mpre + ".2.weight", _tmm(dgW, lay["h"]) + _tmm(gW, dh))
             acc(tg, mpre + ".2.bias", dgW.sum(0))
             dgh = dgW @ W2
             dga1 = dgh * _silu1(lay["a1"]) + gh * _silu2(lay["a1"]) * da1
-            acc(tg, mpre + ".0.weight", dga1.T @ c["e"] + ga1.T @ de_t)
+            acc(tg, mpre + ".0.weight", _tmm(dga1, c["e"]) + _tmm(ga1, de_t))
             acc(tg, mpre + ".0.bias", dga1.sum(0))
             dgs = dgs_m + _add_pairs(n, ga, gb, dgs_m[gb] * Ws + gs_m[gb] * dWs,
                                      dgs_m[ga] * Ws + gs_m[ga] * dWs)
@@ -414,7 +441,10 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
         n = row_ptr.numel() - 1
         deg = (row_ptr[1:] - row_ptr[:-1]).long()
         row = torch.repeat_interleave(torch.arange(n, device=x.device), deg)
-        shift_vec = torch.einsum("ea,eab->eb", shift.to(x.dtype), cell_t[batch[row]])
+        cells_e = cell_t[batch[row]]
+        sh = shift.to(x.dtype)
+        shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
+                     sh[:, 2:3] * cells_e[:, 2])
     return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec)
 
 

@@ -106,3 +106,17 @@ def test_loss_gradient_central_differences():
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - grads[name][idx].item()) < 1e-6 * max(1.0, abs(fd)), (name, fd,
                                                                             grads[name][idx].item())
+
+
+def test_chunked_weight_gradient_product():
+    """The split-K weight-gradient product equals a.T @ b, including a
+    ragged tail and the short-K path."""
+    train = pkg_mod("train")
+    g = torch.Generator().manual_seed(3)
+    for K in (100, 8192, 10000, 20480 + 7):
+        a = torch.randn(K, 12, dtype=torch.float64, generator=g)
+        b = torch.randn(K, 5, dtype=torch.float64, generator=g)
+        ref = a.T @ b
+        got = train._tmm(a, b)
+        assert got.shape == ref.shape
+        assert torch.allclose(got, ref, rtol=1e-12, atol=1e-10)
