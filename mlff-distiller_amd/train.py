@@ -212,13 +212,21 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float)
 # ---------------------------------------------------------------------------
 
 def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
-            lam: Optional[Tensor] = None, want_forces: bool = True):
+            lam: Optional[Tensor] = None, want_forces: bool = True, want_grads: bool = True,
+            gw: Optional[Tensor] = None):
     """Adjoint of E weighted by gE [B].
 
-    Returns (grads, forces, tgrads): parameter gradients of sum_b gE_b E_b,
-    forces -d(sum gE E)/dx, and -- when ``lam`` [N,3] is given -- the
-    parameter gradients of d/deps [sum_b gE_b dE_b/dx] . lam, i.e. of
-    < lam, grad_x (sum gE E) >.
+    Returns (grads, forces, tgrads): parameter gradients of sum_b gw_b E_b
+    (gw defaults to gE), forces -d(sum gE E)/dx, and -- when ``lam`` [N,3] is
+    given -- the parameter gradients of d/deps [sum_b gE_b dE_b/dx] . lam,
+    i.e. of < lam, grad_x (sum gE E) >.
+
+    The adjoint is linear in the structure weights and structures do not
+    interact, so the parameter gradients for weights gw are the terms of
+    the gE adjoint with each atom or pair row scaled by gw_b / gE_b of its
+    structure; ``gw`` is applied that way (gE must then be all ones), which
+    lets one pass give the unit-weight tangent and the gw-weighted
+    gradients together.  ``want_grads=False`` skips the parameter gradients.
     """
     H, L, n = c["H"], c["L"], c["n"]
     T = lam is not None
@@ -228,6 +236,19 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
 
     def acc(store, name, val):
         store[name] = store[name] + val if name in store else val
+
+    def accg(name, fn):
+        """parameter gradient term (evaluated only when wanted)"""
+        if want_grads:
+            acc(grads, name, fn())
+
+    if gw is not None:
+        w_atom = gw[g.batch]
+        w_pair = gw[g.batch[g.lo]]
+        wa = lambda t: t * w_atom.view(-1, *([1] * (t.dim() - 1)))
+        wp = lambda t: t * w_pair.view(-1, *([1] * (t.dim() - 1)))
+    else:
+        wa = wp = lambda t: t
 
     # ---- tangents of the forward pass along lam --------------------------------
     if T:
@@ -282,16 +303,16 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     geps = gE[g.batch]
     r1, h3, r2, h4, sL = c["r1"], c["h3"], c["r2"], c["h4"], c["sL"]
     Q3 = p["energy_head.4.weight"]
-    acc(grads, "energy_head.4.weight", (geps.unsqueeze(1) * h4).sum(0, keepdim=True))
-    acc(grads, "energy_head.4.bias", geps.sum().reshape(1))
+    accg("energy_head.4.weight", lambda: (wa(geps).unsqueeze(1) * h4).sum(0, keepdim=True))
+    accg("energy_head.4.bias", lambda: wa(geps).sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
     gr2 = gh4 * _silu1(r2)
-    acc(grads, "energy_head.2.weight", _tmm(gr2, h3))
-    acc(grads, "energy_head.2.bias", gr2.sum(0))
+    accg("energy_head.2.weight", lambda: _tmm(wa(gr2), h3))
+    accg("energy_head.2.bias", lambda: wa(gr2).sum(0))
     gh3 = gr2 @ p["energy_head.2.weight"]
     gr1 = gh3 * _silu1(r1)
-    acc(grads, "energy_head.0.weight", _tmm(gr1, sL))
-    acc(grads, "energy_head.0.bias", gr1.sum(0))
+    accg("energy_head.0.weight", lambda: _tmm(wa(gr1), sL))
+    accg("energy_head.0.bias", lambda: wa(gr1).sum(0))
     gs = gr1 @ p["energy_head.0.weight"]
     gv = torch.zeros(n, 3, H, dtype=dt, device=gs.device)
     if T:
@@ -322,16 +343,16 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gg1 = (gv * v_m).sum(1)
         gg2 = (gv * Mv).sum(1)
         gy = torch.cat([gs, gg1, gg2], dim=-1)
-        acc(grads, upre + ".update_mlp.2.weight", _tmm(gy, lay["h2"]))
-        acc(grads, upre + ".update_mlp.2.bias", gy.sum(0))
+        accg(upre + ".update_mlp.2.weight", lambda: _tmm(wa(gy), lay["h2"]))
+        accg(upre + ".update_mlp.2.bias", lambda: wa(gy).sum(0))
         gh2 = gy @ U2
         ga2 = gh2 * _silu1(lay["a2"])
-        acc(grads, upre + ".update_mlp.0.weight", _tmm(ga2, lay["xin"]))
-        acc(grads, upre + ".update_mlp.0.bias", ga2.sum(0))
+        accg(upre + ".update_mlp.0.weight", lambda: _tmm(wa(ga2), lay["xin"]))
+        accg(upre + ".update_mlp.0.bias", lambda: wa(ga2).sum(0))
         gx = ga2 @ U1
         gs_x, gn = gx[:, :H], gx[:, H:]
         gvg2 = gv * g2.unsqueeze(1)
-        acc(grads, upre + ".mixing_matrix", _xyz_outer(gvg2, v_m))
+        accg(upre + ".mixing_matrix", lambda: _xyz_outer(wa(gvg2), v_m))
         safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
         unit_n = torch.where((nrm > 0).unsqueeze(1), v_m / safe.unsqueeze(1), torch.zeros_like(v_m))
         gs_m = gs + gs_x
@@ -372,12 +393,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gW = torch.cat([gWs, gWv1, gWv2], dim=-1)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        acc(grads, mpre + ".2.weight", _tmm(gW, lay["h"]))
-        acc(grads, mpre + ".2.bias", gW.sum(0))
+        accg(mpre + ".2.weight", lambda: _tmm(wp(gW), lay["h"]))
+        accg(mpre + ".2.bias", lambda: wp(gW).sum(0))
         gh = gW @ W2
         ga1 = gh * _silu1(lay["a1"])
-        acc(grads, mpre + ".0.weight", _tmm(ga1, c["e"]))
-        acc(grads, mpre + ".0.bias", ga1.sum(0))
+        accg(mpre + ".0.weight", lambda: _tmm(wp(ga1), c["e"]))
+        accg(mpre + ".0.bias", lambda: wp(ga1).sum(0))
         ge = ga1 @ W1
         gd = gd + (ge * c["de"]).sum(-1)
         gs_new = gs_m + _add_pairs(n, ga, gb, gs_m[gb] * Ws, gs_m[ga] * Ws)
@@ -404,9 +425,10 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
                                      dgv_m[gb] * Wv1.unsqueeze(1) + gv_m[gb] * dWv1.unsqueeze(1),
                                      dgv_m[ga] * Wv1.unsqueeze(1) + gv_m[ga] * dWv1.unsqueeze(1))
         gs, gv = gs_new, gv_new
-    emb = torch.zeros_like(p["embedding.weight"])
-    emb.index_add_(0, c["Z"], gs.to(emb.dtype))
-    acc(grads, "embedding.weight", emb)
+    if want_grads:
+        emb = torch.zeros_like(p["embedding.weight"])
+        emb.index_add_(0, c["Z"], wa(gs).to(emb.dtype))
+        acc(grads, "embedding.weight", emb)
     if T:
         demb = torch.zeros_like(p["embedding.weight"])
         demb.index_add_(0, c["Z"], dgs.to(demb.dtype))
@@ -461,16 +483,16 @@ def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGrap
     B = E.shape[0]
     ones = torch.ones_like(E)
     # forces (and the tangent machinery) come from the unit-weighted adjoint
-    gE_unit, F, _ = reverse(p, cache, g, ones, lam=None, want_forces=True)
+    _, F, _ = reverse(p, cache, g, ones, lam=None, want_forces=True, want_grads=False)
     eE = E - E_label
     eF = F - F_label
     loss = w_energy * (eE ** 2).mean() + w_force * (eF ** 2).mean()
     dLdE = w_energy * 2.0 * eE / B
     dLdF = w_force * 2.0 * eF / eF.numel()
-    # d/dtheta of wE MSE(E): adjoint weighted by dL/dE
-    gE, _, _ = reverse(p, cache, g, dLdE, lam=None, want_forces=False)
-    # d/dtheta of wF MSE(F) = d/dtheta < dL/dF, F > = - d/dtheta < dL/dF, grad_x sum E >
-    _, _, tg = reverse(p, cache, g, ones, lam=dLdF, want_forces=False)
+    # one more pass gives both parts:
+    #   d/dtheta of wE MSE(E): the unit adjoint's terms weighted by dL/dE (gw)
+    #   d/dtheta of wF MSE(F) = d/dtheta < dL/dF, F > = - d/dtheta < dL/dF, grad_x sum E >
+    gE, _, tg = reverse(p, cache, g, ones, lam=dLdF, want_forces=False, gw=dLdE)
     grads = {}
     for name in p:
         a = gE.get(name)

@@ -120,3 +120,21 @@ def test_chunked_weight_gradient_product():
         got = train._tmm(a, b)
         assert got.shape == ref.shape
         assert torch.allclose(got, ref, rtol=1e-12, atol=1e-10)
+
+
+def test_weighted_gradients_equal_weighted_adjoint():
+    """Structures do not interact, so the parameter gradients of sum_b w_b E_b
+    from the unit adjoint with rows weighted by w (gw) equal those of the
+    adjoint seeded with w; want_grads=False returns no gradients."""
+    train = pkg_mod("train")
+    sd, Z, x, batch, rc = _setup(seed=2)
+    g = _graph(x, batch, rc, 3)
+    E, cache = train.forward(sd, Z, x, g, rc)
+    w = torch.tensor([0.7, -1.3, 2.1], dtype=torch.float64)
+    ref, _, _ = train.reverse(sd, cache, g, w, want_forces=False)
+    got, _, _ = train.reverse(sd, cache, g, torch.ones_like(E), want_forces=False, gw=w)
+    assert set(ref) == set(got)
+    for k in ref:
+        assert torch.allclose(got[k], ref[k], rtol=1e-11, atol=1e-12), k
+    none, F, _ = train.reverse(sd, cache, g, torch.ones_like(E), want_grads=False)
+    assert none == {} and F is not None
