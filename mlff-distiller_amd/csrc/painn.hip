@@ -12,7 +12,8 @@
 // output W_p [3H] = (W_s | W_v1 | W_v2) of each block.
 //
 // Kernels and what bounds them:
-//   filter_fwd   pairs, MFMA (R->H, SiLU, H->3H); 2(RH + 3H^2) flop/pair
+//   filter_fwd   pairs, MFMA (R->H, SiLU, H->3H; the H->3H product on the bf16
+//                matrix cores with three-way split operands); 2(RH + 3H^2) flop/pair
 //   message      receiver rows, gather of s_j, v_j, W_p (HBM/L2)
 //   update_fwd   atoms, MFMA (2H->H, SiLU, H->3H) + gated vector epilogue
 //   readout      atoms, MFMA head and its input gradient in one pass
@@ -33,6 +34,8 @@ constexpr int kBlock = 256;
 constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
 constexpr int kPairWaves = kPairBlock / 64;
 constexpr int kMaxRbf = 32;
+constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
+constexpr int kUpdBlock = 64 * kUpdWaves;
 #ifndef MLFF_MSG_UNROLL_FWD
 #define MLFF_MSG_UNROLL_FWD 3
 #endif
@@ -176,7 +179,8 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
 // message (forward) and its node-gradient transpose (backward).
 // FWD:  s'_i = s_i + sum_j s_j * Ws_p ;  v'_i = v_i + sum_j v_j * Wv1_p + u_ji (x) Wv2_p
 // BWD:  gs_i = gs'_i + sum_j gs'_j * Ws_p ; gv_i = gv'_i + sum_j gv'_j * Wv1_p
-// One row per H/2 lanes, two channels per lane; neighbours in ascending j.
+// One row per H/4 lanes, four channels (one float4) per lane; neighbours in
+// ascending j, a few edges' operands loaded before any is added.
 // ---------------------------------------------------------------------------
 template <int H, bool FWD>
 __global__ __launch_bounds__(kBlock) void message_kernel(
@@ -259,19 +263,19 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
 // update forward, 32 atoms per tile
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kBlock) void update_fwd_kernel(
+__global__ __launch_bounds__(kUpdBlock) void update_fwd_kernel(
     int N, const float* __restrict__ w, LayerW lw, const float* __restrict__ s_mid,
     const float* __restrict__ v_mid, float* __restrict__ s_out, float* __restrict__ v_out,
     float* __restrict__ a_upd, float* __restrict__ g12) {
   constexpr int TM = 32, RT = 2;
   constexpr int LDX = 2 * H + 4, LDH = H + 4;
-  constexpr int CT1 = H / 16, NT1 = (CT1 + 3) / 4;
+  constexpr int CT1 = H / 16, NT1 = (CT1 + kUpdWaves - 1) / kUpdWaves;
   constexpr int NTC = NT1, NT2 = 3 * NTC;
   __shared__ __align__(16) float x_lds[TM * LDX];
   __shared__ __align__(16) float h_lds[TM * LDH];
   const int base = blockIdx.x * TM;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
     const int m = idx / H, c = idx - m * H, i = base + m;
     float sv = 0.f, nv = 0.f;
     if (i < N) {
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
 // update backward, 32 atoms per tile.  gv2 == nullptr means a zero gradient.
 // ---------------------------------------------------------------------------
 template <int H>
-__global__ __launch_bounds__(kBlock) void update_bwd_kernel(
+__global__ __launch_bounds__(kUpdBlock) void update_bwd_kernel(
     int N, const float* __restrict__ w, LayerW lw, const float* __restrict__ gs2,
     const float* __restrict__ gv2, const float* __restrict__ v_mid,
     const float* __restrict__ a_upd, const float* __restrict__ g12, float* __restrict__ gs1,
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
   float u[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) u[k] = U[k];
-  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
     const int m = idx / H, c = idx - m * H, i = base + m;
     float y0 = 0.f, y1 = 0.f, y2 = 0.f;
     if (i < N) {
@@ -489,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
   }
   __syncthreads();
   {
-    constexpr int CT = H / 16, NT = (CT + 3) / 4;
+    constexpr int CT = H / 16, NT = (CT + kUpdWaves - 1) / kUpdWaves;
     if (wave < CT / NT) {
       int cb[NT];
       f4 acc[RT][NT];
@@ -512,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
   __syncthreads();
   float* x_lds = y_lds;   // y is dead after the barrier above
   {
-    constexpr int CT = 2 * H / 16, NT = (CT + 3) / 4;
+    constexpr int CT = 2 * H / 16, NT = (CT + kUpdWaves - 1) / kUpdWaves;
     if (wave < CT / NT) {
       int cb[NT];
       f4 acc[RT][NT];
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void update_bwd_kernel(
     }
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
     const int m = idx / H, c = idx - m * H, i = base + m;
     if (i >= N) continue;
     gs1[(size_t)i * H + c] = gs2[(size_t)i * H + c] + x_lds[m * LDX + c];
@@ -917,7 +921,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
     float* Vm = b.Vmid + (size_t)l * 3 * nh(b);
     message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair, b.ecap,
                                                        b.geom, Wl, Sl, Vl, Sm, Vm);
-    update_fwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+    update_fwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
         N, b.w, lw, Sm, Vm, b.S + (size_t)(l + 1) * nh(b), b.V + (size_t)(l + 1) * 3 * nh(b),
         b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b));
   }
@@ -948,7 +952,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
     // and update adjoint of block l-1 (caller's stream) run side by side
     float* gS1 = b.gS1 + (size_t)l * nh(b);
     float* gV1 = b.gV1 + (size_t)l * 3 * nh(b);
-    update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+    update_bwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
         b.G12 + (size_t)l * 2 * nh(b), gS1, gV1);
     st.order(s, st.filt, b.L + 1 + l);
@@ -995,7 +999,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
           b.V + (size_t)l * 3 * nh(b), b.Smid + (size_t)l * nh(b), b.Vmid + (size_t)l * 3 * nh(b));
       break;
     case MLFF_STAGE_UPDATE_FWD:
-      update_fwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+      update_fwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
           N, b.w, lw, b.Smid + (size_t)l * nh(b), b.Vmid + (size_t)l * 3 * nh(b),
           b.S + (size_t)(l + 1) * nh(b), b.V + (size_t)(l + 1) * 3 * nh(b),
           b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b));
@@ -1006,7 +1010,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
       break;
     case MLFF_STAGE_UPDATE_BWD:
       // reads gS0/gV0, writes block l's slice of gS1/gV1
-      update_bwd_kernel<H><<<atom_grid, kBlock, 0, s>>>(
+      update_bwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
           N, b.w, lw, b.gS0, (l == b.L - 1) ? nullptr : b.gV0, b.Vmid + (size_t)l * 3 * nh(b),
           b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b), b.gS1 + (size_t)l * nh(b),
           b.gV1 + (size_t)l * 3 * nh(b));

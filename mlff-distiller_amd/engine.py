@@ -50,13 +50,18 @@ def hyper_from_state(sd: Dict[str, torch.Tensor]) -> Tuple[int, int, int, int]:
 
 
 def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
-    """Pack a StudentForceField state dict into the library's float32 blob."""
+    """Pack a StudentForceField state dict into the library's float32 blob
+    (on the device directly when the parameters already live there, so a
+    training step's weight update costs device copies only)."""
     H, L, R, max_z = hyper_from_state(sd)
     slots, total = C.weights_layout(H, L, R, max_z)
-    host = torch.zeros(total, dtype=torch.float32)
+    device = torch.device(device)
+    on_device = all(v.device == device for v in sd.values())
+    where = device if on_device else torch.device("cpu")
+    host = torch.zeros(total, dtype=torch.float32, device=where)
     for s in slots:
         name = _NAMES[s.tensor].format(l=s.layer)
-        t = sd[name].detach().to("cpu", torch.float32)
+        t = sd[name].detach().to(where, torch.float32)
         if t.dim() == 1:
             t = t.unsqueeze(0)
         if s.kind == 1:
@@ -66,7 +71,7 @@ def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
             raise ValueError(f"slot for {name} too small ({rows}x{cols} into {s.rows}x{s.ld})")
         view = host[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)
         view[:rows, :cols] = t
-    return host.to(device)
+    return host if on_device else host.to(device)
 
 
 # ---------------------------------------------------------------------------

@@ -476,14 +476,18 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
 
 def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
                          E_label: Tensor, F_label: Tensor, w_energy: float = 1.0,
-                         w_force: float = 100.0):
+                         w_force: float = 100.0, forces: Optional[Tensor] = None):
     """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and
-    its exact gradient with respect to every parameter in ``p``."""
+    its exact gradient with respect to every parameter in ``p``.
+    ``forces``: the model's forces at ``x`` when already known (the fused
+    evaluation computes them far faster than the unit adjoint pass here)."""
     E, cache = forward(p, Z, x, g, rc)
     B = E.shape[0]
     ones = torch.ones_like(E)
-    # forces (and the tangent machinery) come from the unit-weighted adjoint
-    _, F, _ = reverse(p, cache, g, ones, lam=None, want_forces=True, want_grads=False)
+    if forces is None:
+        _, F, _ = reverse(p, cache, g, ones, lam=None, want_forces=True, want_grads=False)
+    else:
+        F = forces.to(E.dtype)
     eE = E - E_label
     eF = F - F_label
     loss = w_energy * (eE ** 2).mean() + w_force * (eF ** 2).mean()
@@ -524,9 +528,13 @@ class ForceMatchingTrainer:
     def loss_and_grads(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None):
         g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
         p = {k: v.detach() for k, v in self.model.state_dict().items()}
+        F = None
+        if x.dtype == torch.float32 and x.device.type == "cuda":
+            # forces of the current weights from the fused HIP evaluation
+            _, F, _ = self.model.evaluate(Z, x.detach(), np.asarray(sizes), cells, pbc)
         with torch.no_grad():
             return force_matching_grads(p, Z, x.detach(), g, self.model.cutoff, E_label, F_label,
-                                        self.w_energy, self.w_force)
+                                        self.w_energy, self.w_force, forces=F)
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)
