@@ -280,6 +280,7 @@ def config3(args, pkg, synth, mdmod, dev):
                  "edges": int(cl.size)}, W, ei, sh)
 
     acc0 = accuracy_at(md.pos32.cpu().numpy())[0]
+    tuning = md.autotune()
     md.run(args.warmup, graph=True)
     torch.cuda.synchronize()
     c = md.plan.read_counts()
@@ -307,6 +308,9 @@ def config3(args, pkg, synth, mdmod, dev):
                    "1000 waters (3000 atoms, seed 3) in a 31 A periodic box, PaiNN H128/L3/R20 "
                    "seed 0, cell-free minimum image, hipGraph NVE dt 0.5 fs",
                    edges_start=int(c[0]),
+                   autotune={"persistent_blocks": int(tuning["blocks"]),
+                             "evaluation_us": {str(k): round(v, 1)
+                                               for k, v in tuning["timings_us"].items()}},
                    accuracy={"t0": acc0, "final": acc1},
                    nve={"simulated_ps": steps * 0.5e-3,
                         "drift_per_atom_eV": float(s["energy_drift_per_atom"][0]),

@@ -140,7 +140,7 @@ typedef struct mlff_config {
   int32_t edge_capacity;      /* directed edges */
   int32_t pair_capacity;      /* unordered pairs */
   int32_t any_periodic;       /* non-zero when some structure is periodic */
-  int32_t persistent_blocks;  /* 0 = library default (2 workgroups per CU) */
+  int32_t persistent_blocks;  /* 0 = library default (6 workgroups per CU) */
   int32_t cell_capacity;      /* cells reserved for cell-list neighbour search (0 = none) */
 } mlff_config;
 

@@ -34,7 +34,7 @@ def _key(plan) -> str:
 def default_candidates(cus: Optional[int] = None) -> Iterable[int]:
     if cus is None:
         cus = torch.cuda.get_device_properties(0).multi_processor_count
-    return [cus, 2 * cus, 3 * cus, 4 * cus, 6 * cus]
+    return [k * cus for k in (2, 3, 4, 6, 8, 12, 16)]
 
 
 def time_stage(plan, stage: int, layer: int = 0, reps: int = 10) -> float:

@@ -295,7 +295,7 @@ int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
       if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
         cus = prop.multiProcessorCount;
     }
-    blocks = 2 * cus;
+    blocks = 6 * cus;   // the persistent pair kernels: three rounds of two resident blocks per CU
   }
   pl->buf.blocks = (blocks + 7) / 8 * 8;
   pl->n_streams = 2;

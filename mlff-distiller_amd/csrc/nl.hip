@@ -140,6 +140,57 @@ __device__ __forceinline__ void for_each_cell_candidate(const CellList& cl, int 
   }
 }
 
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// The same 27 cells, walked by a whole wave: lane q < 27 holds cell q's first
+// slot in cell_atoms and the number of candidates before it; candidate f of
+// the flattened list is found by a wave-uniform pass over the 27 cells.
+struct CellWalk {
+  int start, off, total;
+};
+
+__device__ __forceinline__ CellWalk cell_walk(const CellList& cl, int i, int b) {
+  const int lane = threadIdx.x & 63;
+  const int4 g = cl.grid[b];
+  int start = 0, cnt = 0;
+  if (lane < 27) {
+    const int lc = cl.atom_cell[i] - g.w;
+    const int c2 = lc % g.z, c1 = (lc / g.z) % g.y, c0 = lc / (g.y * g.z);
+    const int n0 = (c0 + lane / 9 - 1 + g.x) % g.x;
+    const int n1 = (c1 + (lane / 3) % 3 - 1 + g.y) % g.y;
+    const int n2 = (c2 + lane % 3 - 1 + g.z) % g.z;
+    const int cell = g.w + (n0 * g.y + n1) * g.z + n2;
+    start = cl.cell_start[cell];
+    cnt = cl.cell_start[cell + 1] - start;
+  }
+  const int incl = wave_inclusive_scan(cnt);
+  return CellWalk{start, incl - cnt, __shfl(incl, 63, 64)};
+}
+
+// atom index of candidate f (0 <= f < total); every lane of the wave calls it
+__device__ __forceinline__ int cell_candidate(const CellList& cl, const CellWalk& w, int f) {
+  int slot = 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int oq = __shfl(w.off, q, 64), sq = __shfl(w.start, q, 64);
+    if (f >= oq) slot = sq + (f - oq);
+  }
+  return f < w.total ? cl.cell_atoms[slot] : -1;
+}
+
+// cell-list rows of up to this many neighbours are sorted in LDS by a wave;
+// longer rows take the one-lane path
+constexpr int kCellRowMax = 256;
+constexpr int kNlBlock = 256;
+
 __global__ void cl_zero_kernel(int n, int32_t* __restrict__ a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -235,8 +286,8 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
                                 const float* __restrict__ cells, const uint8_t* __restrict__ pbc,
                                 float rc, int32_t* __restrict__ deg, int32_t* __restrict__ udeg,
                                 CellList cl) {
-  // one wave per atom: the lanes test 64 candidates at a time (brute force
-  // within the structure); a cell-list row is walked by lane 0
+  // one wave per atom: the lanes test 64 candidates at a time, the atoms of
+  // the structure (brute force) or of the 27 cells around the atom's cell
   const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
@@ -253,13 +304,12 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
                   vx, vy, vz, d, s0, s1, s2);
   };
   if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
-    if (lane == 0) {
-      for_each_cell_candidate(cl, i, b, [&](int j) {
-        if (j != i && hit(j)) {
-          ++n;
-          nu += (j > i);
-        }
-      });
+    const CellWalk w = cell_walk(cl, i, b);
+    for (int fb = 0; fb < w.total; fb += 64) {
+      const int j = cell_candidate(cl, w, fb + lane);
+      const bool h = (j >= 0) && (j != i) && hit(j);
+      n += __popcll(__ballot(h));
+      nu += __popcll(__ballot(h && j > i));
     }
   } else {
     const int j0 = mol_ptr[b], j1 = mol_ptr[b + 1];
@@ -282,16 +332,6 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
 // over the 16 wave totals, coalesced stores, and a carry into the next chunk.
 constexpr int kScanPer = 8;
 constexpr int kScanChunk = 1024 * kScanPer;
-
-__device__ __forceinline__ int wave_inclusive_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(v, off, 64);
-    if (lane >= off) v += u;
-  }
-  return v;
-}
 
 __global__ __launch_bounds__(1024) void nl_scan_kernel(const int32_t* __restrict__ deg,
                                                        const int32_t* __restrict__ udeg, int N,
@@ -364,6 +404,7 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
                                float4* __restrict__ geom, int32_t pcap, CellList cl,
                                int32_t* __restrict__ tmp_shift) {
   // one wave per atom, as the count pass
+  __shared__ int4 hits_lds[(kNlBlock / 64) * kCellRowMax];
   const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
@@ -375,9 +416,70 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   const float xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   int e = row_ptr[i];
   int p = pair_ptr[i];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int deg = row_ptr[i + 1] - e;
+  if (cl.grid != nullptr && cl.atom_cell[i] >= 0 && deg <= kCellRowMax) {
+    // cell-list row: the wave collects the hits {j, shift} in candidate order
+    // into LDS, then every hit's place in the row is the number of hits with
+    // a smaller neighbour index (the neighbours of a row are distinct)
+    int4* hl = hits_lds + (threadIdx.x >> 6) * kCellRowMax;
+    const CellWalk w = cell_walk(cl, i, b);
+    int k = 0;
+    for (int fb = 0; fb < w.total; fb += 64) {
+      const int j = cell_candidate(cl, w, fb + lane);
+      float vx, vy, vz, d;
+      int s0 = 0, s1 = 0, s2 = 0;
+      const bool h = (j >= 0) && (j != i) &&
+                     within(pos[3 * j] - xi, pos[3 * j + 1] - yi, pos[3 * j + 2] - zi, rc,
+                            periodic, ci, vx, vy, vz, d, s0, s1, s2);
+      const uint64_t mh = __ballot(h);
+      const int slot = k + __popcll(mh & below);
+      if (h && slot < kCellRowMax) hl[slot] = make_int4(j, s0, s1, s2);
+      k += __popcll(mh);
+    }
+    k = min(k, kCellRowMax);
+    // the LDS writes above are visible to every lane of the wave below
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int n_lower = deg - (pair_ptr[i + 1] - p);   // neighbours j < i
+    for (int q = lane; q < k; q += 64) {
+      const int4 hq = hl[q];
+      int rank = 0;
+      for (int m = 0; m < k; ++m) rank += hl[m].x < hq.x;
+      const int j = hq.x, s0 = hq.y, s1 = hq.z, s2 = hq.w;
+      const int ek = e + rank;
+      const bool e_ok = ek < ecap;
+      if (e_ok) {
+        col[ek] = j;
+        if (edge_shift) {
+          edge_shift[3 * ek] = s0; edge_shift[3 * ek + 1] = s1; edge_shift[3 * ek + 2] = s2;
+        }
+      }
+      if (j > i) {
+        const int pk = p + rank - n_lower;
+        if (pair_lo != nullptr && pk < pcap) {
+          // the pair vector rounds exactly as inside within()
+          float sx, sy, sz;
+          shift_vec(ci, s0, s1, s2, sx, sy, sz);
+          const float vx = (pos[3 * j] - xi) + sx, vy = (pos[3 * j + 1] - yi) + sy,
+                      vz = (pos[3 * j + 2] - zi) + sz;
+          pair_lo[pk] = i;
+          pair_hi[pk] = j;
+          if (pair_shift) {
+            pair_shift[3 * pk] = s0; pair_shift[3 * pk + 1] = s1; pair_shift[3 * pk + 2] = s2;
+          }
+          geom[pk] = make_float4(vx, vy, vz, norm3_exact(vx, vy, vz));
+        }
+        if (edge_pair != nullptr && e_ok) edge_pair[ek] = (pk < pcap) ? pk : -1;
+      }
+    }
+    return;
+  }
   if (cl.grid != nullptr && cl.atom_cell[i] >= 0) {
     if (lane != 0) return;
-    // cell-list row: collect in cell order, sort by neighbour index, then emit
+    // a row longer than the LDS list: lane 0 collects in cell order, sorts by
+    // neighbour index, then emits
     const int e0 = e;
     for_each_cell_candidate(cl, i, b, [&](int j) {
       if (j == i) return;
@@ -435,7 +537,6 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   }
   // 64 candidates per step; a hit's slot is the number of hits in lower
   // lanes, so the row stays in ascending neighbour order
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int jb = j0; jb < j1; jb += 64) {
     const int j = jb + lane;
     float vx = 0.f, vy = 0.f, vz = 0.f, d = 0.f;
@@ -528,12 +629,12 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
     cl = CellList{a.grid, a.cell_start, a.cell_atoms, a.atom_cell};
   }
   // count and fill: one wave per atom
-  const int gw = (int)(((int64_t)a.N * 64 + 255) / 256);
-  nl_count_kernel<<<gw, 256, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
+  const int gw = (int)(((int64_t)a.N * 64 + kNlBlock - 1) / kNlBlock);
+  nl_count_kernel<<<gw, kNlBlock, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                    a.deg, a.udeg, cl);
   nl_scan_kernel<<<1, 1024, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr, a.counts, a.ecap,
                                     build_pairs ? a.pcap : 0x7fffffff);
-  nl_fill_kernel<<<gw, 256, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
+  nl_fill_kernel<<<gw, kNlBlock, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                   a.row_ptr, a.pair_ptr, a.col, a.edge_shift,
                                   build_pairs ? a.edge_pair : nullptr, a.ecap,
                                   build_pairs ? a.pair_lo : nullptr, a.pair_hi, a.pair_shift,

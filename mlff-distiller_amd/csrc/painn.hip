@@ -33,7 +33,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
 constexpr int kPairWaves = kPairBlock / 64;
-constexpr int kMaxRbf = 32;
+constexpr int kMaxRbf = 32;          // radial basis size limit (padded to 16: 16 or 32)
 constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
 constexpr int kUpdBlock = 64 * kUpdWaves;
 #ifndef MLFF_MSG_UNROLL_FWD
@@ -43,7 +43,7 @@ constexpr int kUpdBlock = 64 * kUpdWaves;
 #define MLFF_MSG_UNROLL_BWD 4
 #endif
 constexpr int kMsgUnrollFwd = MLFF_MSG_UNROLL_FWD;   // edges in flight per row (message forward)
-constexpr int kMsgUnrollBwd = MLFF_MSG_UNROLL_BWD;   // (message adjoint)           // radial basis size limit (padded to 16: 16 or 32)
+constexpr int kMsgUnrollBwd = MLFF_MSG_UNROLL_BWD;   // (message adjoint)
 
 __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
                              const float* __restrict__ emb, float* __restrict__ S0,

@@ -118,6 +118,45 @@ def test_cell_list_equals_brute_force(gpu, skew):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("skew", [0.0, 2.5])
+def test_cell_list_long_rows_equal_brute_force(gpu, skew):
+    """A periodic box with a dense slab: rows of more than 256 neighbours
+    (walked by one lane) beside shorter rows (walked by the whole wave and
+    sorted in LDS); energies, forces and virials are bit-identical to the
+    brute-force scan."""
+    note(f"cell list, dense slab (rows above and below 256 neighbours), skew {skew}")
+    engine = pkg_mod("engine")
+    L = 16.5
+    cell = np.diag([L, L, L]).astype(np.float64)
+    cell[1, 0] += skew
+    cell[2, 1] -= 0.5 * skew
+    rng = np.random.default_rng(23)
+    frac = np.concatenate([rng.uniform([0, 0, 0], [1 / 3, 1, 1], size=(1400, 3)),
+                           rng.uniform([1 / 3, 0, 0], [1, 1, 1], size=(1100, 3))])
+    pos32 = (frac @ cell).astype(np.float32)
+    cell32 = cell.astype(np.float32)
+    pbc = np.array([True, True, True])
+    numbers = rng.choice([1, 6, 7, 8], size=len(pos32))
+    rp, _, _ = nl_ref.neighbor_csr(pos32, np.array([0, len(pos32)]), 5.0, cell32[None], pbc[None])
+    deg = np.diff(rp)
+    assert deg.max() > 256 and deg.min() < 256, (deg.min(), deg.max())
+    Z = torch.tensor(numbers, device=gpu)
+    P = torch.tensor(pos32, device=gpu)
+    m1 = _model(64, 2, 20, seed=0)
+    out_cl = m1.energy_forces_virial(Z, P, torch.tensor(cell32), torch.tensor(pbc))
+    assert next(iter(m1._plans.values()))["plan"].cell_grid[0, 0] >= 3
+    old = engine.CELL_LIST_MIN_ATOMS
+    engine.CELL_LIST_MIN_ATOMS = 10 ** 9
+    try:
+        m2 = _model(64, 2, 20, seed=0)
+        out_bf = m2.energy_forces_virial(Z, P, torch.tensor(cell32), torch.tensor(pbc))
+        assert next(iter(m2._plans.values()))["plan"].cell_grid[0, 0] == 0
+    finally:
+        engine.CELL_LIST_MIN_ATOMS = old
+    for a, b in zip(out_cl, out_bf):
+        assert torch.equal(a, b)
+
+
 def _check_ef(model, Z, P32, sizes, E, F, label, absolute=True):
     """Compare with the float64 oracle.  ``absolute``: the task's bars
     (1e-5 eV/atom, 1e-4 eV/A).  Otherwise (weights scaled up so forces reach
