@@ -248,6 +248,38 @@ def test_autograd_forces_and_batched_api(gpu):
         assert (f1 - F[sl]).abs().max().item() < 1e-5
 
 
+def test_structures_without_edges_and_locality(gpu):
+    """A batch with a single atom, two atoms beyond the cutoff, an atom left
+    alone inside a molecule and two ordinary molecules: energies and forces
+    match the oracle, atoms without neighbours feel exactly zero force, and
+    every structure evaluated alone gives bit-identical forces (nothing leaks
+    between the rows of a shared tile)."""
+    note("sparse rows and structure locality")
+    model = _model(128, 3, 20, seed=0, bias_scale=0.05)
+    a, b = _mols(2, 20, 40, seed=51)
+    Pb = b.positions.copy()
+    Pb[-1] += np.array([25.0, 0.0, 0.0])          # an atom far from the rest of its molecule
+    structs = [(np.array([8]), np.zeros((1, 3))),
+               (np.array([6, 1]), np.array([[0.0, 0.0, 0.0], [6.0, 0.0, 0.0]])),
+               (a.numbers, a.positions),
+               (b.numbers, Pb)]
+    Z = np.concatenate([z for z, _ in structs])
+    P32 = np.concatenate([p for _, p in structs]).astype(np.float32)
+    sizes = np.array([len(z) for z, _ in structs])
+    ptr = np.concatenate([[0], np.cumsum(sizes)])
+    Zt, Pt = torch.tensor(Z, device=gpu), torch.tensor(P32, device=gpu)
+    e64, f, _ = model.evaluate(Zt, Pt, sizes)
+    _check_ef(model, Z, P32, sizes, e64.cpu().numpy(), f.cpu().numpy(), "sparse rows",
+              absolute=False)
+    lonely = [0, 1, 2, ptr[4] - 1]
+    assert torch.count_nonzero(f[lonely]).item() == 0
+    for s in range(len(structs)):
+        sl = slice(ptr[s], ptr[s + 1])
+        e1, f1, _ = model.evaluate(Zt[sl], Pt[sl], sizes[s:s + 1])
+        assert torch.equal(f1, f[sl]), s
+        assert abs(e1.item() - e64[s].item()) <= 1e-12 * max(1.0, abs(e1.item())), s
+
+
 def test_runs_are_bit_reproducible(gpu):
     note("determinism")
     model = _model(128, 3, 20, seed=0)
@@ -284,6 +316,53 @@ def test_periodic_water_energy_forces_virial(gpu):
     assert abs(e64.item() - E.item()) / n < E_TOL
     assert np.abs(f.cpu().numpy() - F.numpy()).max() < F_TOL
     assert np.abs(vir.cpu().numpy()[0] - W.numpy()[0]).max() < 1e-3 * max(1.0, np.abs(W.numpy()).max())
+
+
+def test_mixed_periodic_batch(gpu):
+    """One batch with per-structure cells and pbc: a periodic water box (cell
+    list), a molecule, a slab periodic in x and y only, and a skewed box
+    (27-image minimum image): energies, forces and virials of every
+    structure match the oracle run on that structure alone."""
+    note("mixed batch: periodic box, molecule, slab, skewed box")
+    synth = pkg_mod("synth")
+    model = _model(64, 2, 20, seed=0)
+    box = synth.water_box(5, seed=3)
+    slab = synth.water_box(4, seed=5)
+    skew = synth.water_box(4, seed=7)
+    skew_cell = skew.cell.copy()
+    skew_cell[1, 0] += 2.0
+    skew_cell[2, 1] -= 1.0
+    mol = _mols(1, 30, 30, seed=61)[0]
+    structs = [(box.numbers, box.positions, box.cell, np.array([True, True, True])),
+               (mol.numbers, mol.positions, np.zeros((3, 3)), np.array([False, False, False])),
+               (slab.numbers, slab.positions, slab.cell, np.array([True, True, False])),
+               (skew.numbers, skew.positions, skew_cell, np.array([True, True, True]))]
+    Z = np.concatenate([s[0] for s in structs])
+    P32 = np.concatenate([s[1] for s in structs]).astype(np.float32)
+    cells = np.stack([s[2] for s in structs]).astype(np.float32)
+    pbc = np.stack([s[3] for s in structs])
+    sizes = np.array([len(s[0]) for s in structs])
+    ptr = np.concatenate([[0], np.cumsum(sizes)])
+    e64, f, vir = model.evaluate(torch.tensor(Z, device=gpu), torch.tensor(P32, device=gpu),
+                                 sizes, cells, pbc, need_virial=True)
+    sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    for s, (z, _, cell, per) in enumerate(structs):
+        p = P32[ptr[s]:ptr[s + 1]]
+        periodic = bool(per.any())
+        rp, cl, sh = nl_ref.neighbor_csr(p, np.array([0, len(p)]), 5.0,
+                                         cells[s][None] if periodic else None,
+                                         per[None] if periodic else None)
+        ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(len(p)), np.diff(rp))])
+        E, F, W = R.energy_and_forces(sd, z, p.astype(np.float64), cutoff=5.0, edge_index=ei,
+                                      shifts=sh if periodic else None,
+                                      cell=cells[s].astype(np.float64) if periodic else None,
+                                      virial=True)
+        e_err = abs(e64[s].item() - E.item()) / len(p)
+        f_err = np.abs(f[ptr[s]:ptr[s + 1]].cpu().numpy() - F.numpy()).max()
+        w_err = np.abs(vir[s].cpu().numpy() - W.numpy()[0]).max()
+        note(f"structure {s}: |dE|/N={e_err:.2e} |dF|={f_err:.2e} |dW|={w_err:.2e}")
+        assert e_err < E_TOL and f_err < F_TOL, s
+        assert w_err < 1e-3 * max(1.0, np.abs(W.numpy()).max()), s
 
 
 def test_filter_stream_overlap_is_bit_identical(gpu):
