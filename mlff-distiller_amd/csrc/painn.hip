@@ -34,6 +34,8 @@ constexpr int kBlock = 256;
 constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
 constexpr int kPairWaves = kPairBlock / 64;
 constexpr int kMaxRbf = 32;          // radial basis size limit (padded to 16: 16 or 32)
+constexpr int kFwdPairs = 64;        // pairs per tile, filter forward
+constexpr int kBwdPairs = 32;        // pairs per tile, filter adjoint
 constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
 constexpr int kUpdBlock = 64 * kUpdWaves;
 #ifndef MLFF_MSG_UNROLL_FWD
@@ -44,6 +46,14 @@ constexpr int kUpdBlock = 64 * kUpdWaves;
 #endif
 constexpr int kMsgUnrollFwd = MLFF_MSG_UNROLL_FWD;   // edges in flight per row (message forward)
 constexpr int kMsgUnrollBwd = MLFF_MSG_UNROLL_BWD;   // (message adjoint)
+
+// Grid of a pair kernel: the plan's persistent grid, capped at the number of
+// tiles the pair capacity allows (a multiple of 8 for the XCD-interleaved tile
+// order), so small batches do not dispatch thousands of idle workgroups.
+inline int pair_grid(const Buffers& b, int pairs_per_tile) {
+  const int tiles = (b.pcap + pairs_per_tile - 1) / pairs_per_tile;
+  return std::max(8, std::min(b.blocks, (tiles + 7) / 8 * 8));
+}
 
 __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
                              const float* __restrict__ emb, float* __restrict__ S0,
@@ -67,7 +77,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
     const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout,
     const uint16_t* __restrict__ w2bf) {
-  constexpr int TM = 64, RT = 4;
+  constexpr int TM = kFwdPairs, RT = 4;
   constexpr int LDE = kMaxRbf + 4;
   constexpr int LDHB = H + 8;          // bf16 row stride of the split hidden activations
   constexpr int PHB = TM * LDHB;       // plane stride
@@ -597,7 +607,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     const float* __restrict__ s_in, const float* __restrict__ v_in,
     const float* __restrict__ gs1, const float* __restrict__ gv1, float4* __restrict__ pacc,
     float4* __restrict__ gvec, int first, int last) {
-  constexpr int TM = 32, RT = 2;
+  constexpr int TM = kBwdPairs, RT = 2;
   constexpr int LDE = kMaxRbf + 4, LDG = 3 * H + 4, LDA = H + 4;
   constexpr int C4 = H / 4;                    // lanes per pair in the gather (4 channels each)
   constexpr int NPART = 1;                     // one dE/du partial per pair
@@ -898,7 +908,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   // filter stream while the caller's stream walks the atom-side chain
   st.order(s, st.filt, 0);
   for (int l = 0; l < b.L; ++l) {
-    filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, st.filt>>>(
+    filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs), kPairBlock, 0, st.filt>>>(
         b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc,
         b.Wf + (size_t)l * b.pcap * 3 * H, b.wbf + wbf_f2w(H, l));
     if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
@@ -956,7 +966,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
         b.G12 + (size_t)l * 2 * nh(b), gS1, gV1);
     st.order(s, st.filt, b.L + 1 + l);
-    filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, st.filt>>>(
+    filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs), kPairBlock, 0, st.filt>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
         b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), gS1, gV1, b.pacc,
         b.gvec, l == b.L - 1, l == 0);
@@ -989,7 +999,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
   switch (stage) {
     case MLFF_STAGE_FILTER_FWD:
-      filter_fwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+      filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs), kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
                                                        widths, b.R, mw.RP, b.rc, Wl,
                                                        b.wbf + wbf_f2w(H, l));
       break;
@@ -1017,7 +1027,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
       break;
     case MLFF_STAGE_FILTER_BWD:
       // first=1 overwrites the pair accumulator, so repeated launches agree
-      filter_bwd_kernel<H><<<b.blocks, kPairBlock, 0, s>>>(
+      filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs), kPairBlock, 0, s>>>(
           b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
           b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1 + (size_t)l * nh(b),
           b.gV1 + (size_t)l * 3 * nh(b), b.pacc, b.gvec, 1, 0);
