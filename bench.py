@@ -219,7 +219,8 @@ def config1(args, pkg, synth, mdmod, dev):
     model = pkg.StudentForceField().to(dev).eval()
     mol = synth.single_molecule(60, seed=1)
     steps = args.steps if args.steps != 500 else 1000
-    md = mdmod.BatchedNVE(model, [mol], seed=1, max_steps=steps + args.warmup + 1)
+    md = mdmod.BatchedNVE(model, [mol], seed=1, max_steps=steps + args.warmup + 1,
+                          streams=args.streams)
     md.run(args.warmup, graph=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
