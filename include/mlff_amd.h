@@ -9,8 +9,8 @@
  * ASE VelocityVerlet in NVEMDHarness
  * (src/mlff_distiller/testing/nve_harness.py:173-261).  It has no FFI of its
  * own; the entry points below are what its model/calculator would bind
- * through ctypes to replace those calls (the binding is shown in
- * INTEGRATION.md).
+ * through ctypes to replace those calls (the package's own ctypes binding is
+ * mlff-distiller_amd/_capi.py).
  *
  * Conventions
  *  - Every pointer argument named *_dev is device memory owned by the
