@@ -16,7 +16,10 @@ from .model import (CosineCutoff, GaussianRBF, PaiNNInteraction, PaiNNMessage, P
 from .graph import radius_graph, radius_graph_native, radius_graph_csr
 from .calculator import StudentForceFieldCalculator
 from .md import BatchedNVE, NVEMDHarness
+from .dataset import (DistillationWrapper, FrameDataset, FrameDatasetWriter,
+                      distillation_collate)
 
 __all__ = ["StudentForceField", "PaiNNInteraction", "PaiNNMessage", "PaiNNUpdate",
            "GaussianRBF", "CosineCutoff", "radius_graph", "radius_graph_native",
-           "radius_graph_csr", "StudentForceFieldCalculator", "BatchedNVE", "NVEMDHarness"]
+           "radius_graph_csr", "StudentForceFieldCalculator", "BatchedNVE", "NVEMDHarness",
+           "DistillationWrapper", "FrameDataset", "FrameDatasetWriter", "distillation_collate"]

@@ -105,3 +105,51 @@ def test_training_reduces_loss(gpu):
     losses = [tr.step(*args) for _ in range(20)]
     note(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.7 * losses[0]
+
+
+def test_distillation_wrapper_on_collated_dataset(gpu, tmp_path):
+    """Frames written in the reference's dataset layout, read back, collated
+    and run through the DistillationWrapper: eval mode returns the fused
+    evaluation's energies and forces; training mode keeps the graph, and a
+    force-matching loss gives the trainer's parameter gradients."""
+    note("dataset -> collate -> DistillationWrapper")
+    ds = pkg_mod("dataset")
+    synth = pkg_mod("synth")
+    torch.manual_seed(1)
+    teacher = pkg().StudentForceField()
+    torch.manual_seed(0)
+    student = pkg().StudentForceField(hidden_dim=64, num_interactions=2).to(gpu)
+    mols = synth.molecule_set(4, 20, 40, seed=9)
+    Z, P, sizes, b, El, Fl = _labels({k: v.double() for k, v in teacher.state_dict().items()}, mols)
+    ptr = np.concatenate([[0], np.cumsum(sizes)])
+    path = str(tmp_path / "frames.npz")
+    with ds.FrameDatasetWriter(path) as w:
+        for m in range(len(mols)):
+            w.add_structure(energy=float(El[m]), forces=Fl.numpy()[ptr[m]:ptr[m + 1]],
+                            numbers=Z[ptr[m]:ptr[m + 1]], positions=P[ptr[m]:ptr[m + 1]])
+    r = ds.FrameDataset(path)
+    batch = ds.distillation_collate([r[i] for i in range(len(r))], device=gpu)
+    wrap = ds.DistillationWrapper(student)
+    wrap.eval()
+    with torch.no_grad():
+        for p in student.parameters():
+            p.requires_grad_(False)
+        out = wrap(batch)
+        e64, f, _ = student.evaluate(batch["atomic_numbers"], batch["positions"], sizes)
+        for p in student.parameters():
+            p.requires_grad_(True)
+    assert torch.allclose(out["energy"].double(), e64, rtol=1e-6, atol=1e-5)
+    assert (out["forces"] - f).abs().max().item() < 1e-5
+    wrap.train()
+    out = wrap(batch)
+    loss = (((out["energy"] - batch["energy"].float()) ** 2).mean()
+            + 100.0 * ((out["forces"] - batch["forces"]) ** 2).mean())
+    student.zero_grad()
+    loss.backward()
+    tr = pkg_mod("train").ForceMatchingTrainer(student)
+    _, _, _, grads = tr.loss_and_grads(batch["atomic_numbers"], batch["positions"], sizes,
+                                       batch["energy"].float(), batch["forces"])
+    for n, p in student.named_parameters():
+        ref = grads[n].reshape(p.shape)
+        rel = ((p.grad - ref).norm() / (ref.norm() + 1e-30)).item()
+        assert rel < 1e-4, (n, rel)
