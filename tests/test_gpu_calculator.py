@@ -110,3 +110,30 @@ def test_nve_harness_api(gpu, ckpt):
     assert abs(out["energy_drift_abs"]) / 20 < 1e-3
     with pytest.raises(ValueError):
         h.run_simulation(0)
+
+
+def test_torchscript_export_runs_on_the_kernels(gpu, ckpt, tmp_path):
+    """The reference's TorchScript export (SimpleWrapper traced on a 50-atom
+    molecule): the loaded file evaluates other sizes on the fused kernels with
+    the same bits as the model, autograd gives the adjoint forces, and the
+    calculator's use_jit path reads the same file."""
+    note("TorchScript export")
+    jit = pkg_mod("jit")
+    path, sd = ckpt
+    model = pkg().StudentForceField.load(path, device="cuda")
+    out = tmp_path / "student_jit.pt"
+    jit.export_to_torchscript(model, out, num_atoms=50, device="cuda")
+    m = jit.load_torchscript(out, "cuda")
+    assert any(k.startswith("model.interactions.0.message") for k in m.state_dict())
+    atoms = _mol(37, 12)
+    z = torch.tensor(atoms.numbers, device=gpu)
+    x = torch.tensor(atoms.positions, dtype=torch.float32, device=gpu, requires_grad=True)
+    E = m(z, x)
+    F = -torch.autograd.grad(E, x)[0]
+    model.eval()
+    E2, F2 = model.predict_energy_and_forces(z, x.detach())
+    assert E.item() == E2.item()
+    assert torch.equal(F, F2)
+    calc = pkg().StudentForceFieldCalculator(None, device="cuda", use_jit=True, jit_path=out)
+    atoms.calc = calc
+    assert np.array_equal(atoms.get_forces(), F2.cpu().numpy().astype(np.float64))
