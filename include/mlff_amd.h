@@ -151,7 +151,10 @@ int64_t mlff_plan_workspace_bytes(const mlff_plan* plan);
 /* Bind the workspace and the per-batch constants.  numbers_dev [N] int64,
  * mol_ptr_dev [B+1], atom_mol_dev [N] (structure index of each atom),
  * cells_dev [B,3,3] (rows = lattice vectors) and pbc_dev [B,3] may be NULL
- * for non-periodic batches.  weights_dev is the packed blob. */
+ * for non-periodic batches.  weights_dev is the packed blob.  Binding is
+ * synchronous: it clears the plan's counters and splits the filter weights
+ * into their bf16 planes inside the workspace, then waits for the device;
+ * it also creates the plan's second stream on the current device. */
 int mlff_plan_bind(mlff_plan* plan, void* workspace_dev, int64_t workspace_bytes,
                    const float* weights_dev, const int64_t* numbers_dev,
                    const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
