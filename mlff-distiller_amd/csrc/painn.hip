@@ -159,7 +159,9 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
       float bias[NT2];
 #pragma unroll
       for (int ct = 0; ct < NT2; ++ct) bias[ct] = f2b[cb[ct] + r];
-      // lane (r, g) holds rows rt*16 + 4g + i, columns cb[ct] + r
+      // lane (r, g) holds rows rt*16 + 4g + i, columns cb[ct] + r.  W_p is
+      // written once and read by a later kernel: non-temporal stores keep it
+      // from displacing L2 lines.
       float* out = Wout + (size_t)(pbase + 4 * g) * 3 * H + r;
       if (pbase + TM <= P) {   // full tile: no per-element bounds checks
 #pragma unroll
@@ -168,7 +170,8 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
           for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int ct = 0; ct < NT2; ++ct)
-              out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]] = acc[rt][ct][i] + bias[ct];
+              __builtin_nontemporal_store(acc[rt][ct][i] + bias[ct],
+                                          &out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]]);
       } else {
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -177,7 +180,8 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
             if (pbase + rt * 16 + 4 * g + i >= P) continue;
 #pragma unroll
             for (int ct = 0; ct < NT2; ++ct)
-              out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]] = acc[rt][ct][i] + bias[ct];
+              __builtin_nontemporal_store(acc[rt][ct][i] + bias[ct],
+                                          &out[(size_t)(rt * 16 + i) * 3 * H + cb[ct]]);
           }
       }
     }
