@@ -800,14 +800,12 @@ __global__ void force_gather_kernel(int N, const int32_t* __restrict__ row_ptr,
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float vr[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const int e0 = min(row_ptr[i], ecap), e1 = min(row_ptr[i + 1], ecap);
-  for (int e = e0; e < e1; ++e) {
-    const int j = col[e], p = edge_pair[e];
-    if (p < 0) continue;
-    const float4 G = gvec[p];
+  // one edge's contribution, in edge order
+  auto add = [&](int j, int p, const float4& G, const float4& v) {
+    if (p < 0) return;
     if (j > i) {
       fx += G.x; fy += G.y; fz += G.z;
       if (vir_atom != nullptr) {
-        const float4 v = geom[p];
         vr[0] += v.x * G.x; vr[1] += v.x * G.y; vr[2] += v.x * G.z;
         vr[3] += v.y * G.x; vr[4] += v.y * G.y; vr[5] += v.y * G.z;
         vr[6] += v.z * G.x; vr[7] += v.z * G.y; vr[8] += v.z * G.z;
@@ -815,6 +813,33 @@ __global__ void force_gather_kernel(int N, const int32_t* __restrict__ row_ptr,
     } else {
       fx -= G.x; fy -= G.y; fz -= G.z;
     }
+  };
+  // four edges' loads in flight before any is added (one thread per atom
+  // leaves about one wave per CU, so the loop is latency bound)
+  constexpr int U = 4;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  int e = e0;
+  for (; e + U - 1 < e1; e += U) {
+    int jj[U], pp[U];
+    float4 GG[U], VV[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      jj[u] = col[e + u];
+      pp[u] = edge_pair[e + u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int pc = pp[u] >= 0 ? pp[u] : 0;
+      GG[u] = gvec[pc];
+      VV[u] = vir_atom != nullptr ? geom[pc] : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) add(jj[u], pp[u], GG[u], VV[u]);
+  }
+  for (; e < e1; ++e) {
+    const int p = edge_pair[e];
+    const int pc = p >= 0 ? p : 0;
+    add(col[e], p, gvec[pc], vir_atom != nullptr ? geom[pc] : zero);
   }
   if (forces != nullptr) {
     forces[3 * i] = fx;
