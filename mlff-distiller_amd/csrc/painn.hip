@@ -35,7 +35,10 @@ constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, ma
 constexpr int kPairWaves = kPairBlock / 64;
 constexpr int kMaxRbf = 32;          // radial basis size limit (padded to 16: 16 or 32)
 constexpr int kFwdPairs = 64;        // pairs per tile, filter forward
-constexpr int kBwdPairs = 32;        // pairs per tile, filter adjoint
+// pairs per tile, filter adjoint: 64 for the compact hidden size (its tiles are
+// short, so fewer, longer ones), 32 above (two blocks per CU fit the LDS)
+template <int H>
+constexpr int kBwdPairs = H <= 32 ? 64 : 32;
 constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
 constexpr int kUpdBlock = 64 * kUpdWaves;
 #ifndef MLFF_MSG_UNROLL_FWD
@@ -594,7 +597,7 @@ __device__ __forceinline__ void gd_partial(const float* ga, int lda, const float
 }
 
 // ---------------------------------------------------------------------------
-// filter backward, 32 pairs per tile.
+// filter backward, kBwdPairs<H> pairs per tile.
 // For pair p = (a < b): edge b->a carries s_b*Ws, v_b*Wv1 + u (x) Wv2 to a
 // and edge a->b carries s_a*Ws, v_a*Wv1 - u (x) Wv2 to b, with
 // u = (p_b - p_a)/(d + 1e-8).  The tile forms dL/dW_p from both directions,
@@ -611,7 +614,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     const float* __restrict__ s_in, const float* __restrict__ v_in,
     const float* __restrict__ gs1, const float* __restrict__ gv1, float4* __restrict__ pacc,
     float4* __restrict__ gvec, int first, int last) {
-  constexpr int TM = kBwdPairs, RT = 2;
+  constexpr int TM = kBwdPairs<H>, RT = TM / 16;
   constexpr int LDE = kMaxRbf + 4, LDG = 3 * H + 4, LDA = H + 4;
   constexpr int C4 = H / 4;                    // lanes per pair in the gather (4 channels each)
   constexpr int NPART = 1;                     // one dE/du partial per pair
@@ -712,25 +715,36 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     }
     __syncthreads();
     {
-      constexpr int CT = H / 16, NT = (CT + kPairWaves - 1) / kPairWaves;
-      if (wave < CT / NT) {
+      // When the tile has at most one 16x16 output block per wave, each wave
+      // takes one (row tile, column tile); otherwise every wave takes all row
+      // tiles of its column tiles.  Each output element sums over K in the
+      // same order either way.
+      constexpr int CT = H / 16;
+      constexpr bool kOneBlock = CT * RT <= kPairWaves;
+      constexpr int RTW = kOneBlock ? 1 : RT;
+      constexpr int NT = kOneBlock ? 1 : (CT + kPairWaves - 1) / kPairWaves;
+      constexpr int NWB = kOneBlock ? CT * RT : CT / NT;
+      if (wave < NWB) {
+        const int r0 = kOneBlock ? wave / CT : 0;
+        const int c0 = kOneBlock ? wave % CT : wave * NT;
         int cb[NT];
-        f4 pre[RT][NT], acc[RT][NT];
+        f4 pre[RTW][NT], acc[RTW][NT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+        for (int ct = 0; ct < NT; ++ct) cb[ct] = (c0 + ct) * 16;
         zero_acc(pre);
         zero_acc(acc);
-        mma_tile<RT, NT>(e_lds, LDE, w + lw.f1w, RP, RP, cb, pre);
-        mma_tile_k2<RT, NT, 3 * H, 2, true>(gw_lds, LDG, w + lw.f2t, 3 * H, cb, acc);
+        mma_tile<RTW, NT>(e_lds + r0 * 16 * LDE, LDE, w + lw.f1w, RP, RP, cb, pre);
+        mma_tile_k2<RTW, NT, 3 * H, 2, true>(gw_lds + r0 * 16 * LDG, LDG, w + lw.f2t, 3 * H, cb,
+                                             acc);
         const float* b1 = w + lw.f1b;
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
+        for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
           for (int ct = 0; ct < NT; ++ct) {
             const int c = cb[ct] + r;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              const int m = rt * 16 + 4 * g + k;
+              const int m = (r0 + rt) * 16 + 4 * g + k;
               ga_lds[m * LDA + c] = acc[rt][ct][k] * silu_grad(pre[rt][ct][k] + b1[c]);
             }
           }
@@ -995,7 +1009,7 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
         b.G12 + (size_t)l * 2 * nh(b), gS1, gV1);
     st.order(s, st.filt, b.L + 1 + l);
-    filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs), kPairBlock, 0, st.filt>>>(
+    filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs<H>), kPairBlock, 0, st.filt>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
         b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), gS1, gV1, b.pacc,
         b.gvec, l == b.L - 1, l == 0);
@@ -1056,7 +1070,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
       break;
     case MLFF_STAGE_FILTER_BWD:
       // first=1 overwrites the pair accumulator, so repeated launches agree
-      filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs), kPairBlock, 0, s>>>(
+      filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs<H>), kPairBlock, 0, s>>>(
           b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
           b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1 + (size_t)l * nh(b),
           b.gV1 + (size_t)l * 3 * nh(b), b.pacc, b.gvec, 1, 0);
