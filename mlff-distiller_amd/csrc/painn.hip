@@ -34,7 +34,9 @@ constexpr int kBlock = 256;
 constexpr int kPairBlock = 512;       // 8 waves: each owns few output tiles, many in flight
 constexpr int kPairWaves = kPairBlock / 64;
 constexpr int kMaxRbf = 32;          // radial basis size limit (padded to 16: 16 or 32)
-constexpr int kFwdPairs = 64;        // pairs per tile, filter forward
+// pairs per tile, filter forward: 128 for the compact hidden size, 64 above
+template <int H>
+constexpr int kFwdPairs = H <= 32 ? 128 : 64;
 // pairs per tile, filter adjoint: 64 for the compact hidden size (its tiles are
 // short, so fewer, longer ones), 32 above (two blocks per CU fit the LDS)
 template <int H>
@@ -72,7 +74,7 @@ __global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
 }
 
 // ---------------------------------------------------------------------------
-// filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for 64 pairs per tile
+// filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for kFwdPairs<H> pairs per tile
 // ---------------------------------------------------------------------------
 template <int H>
 __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
     const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout,
     const uint16_t* __restrict__ w2bf) {
-  constexpr int TM = kFwdPairs, RT = 4;
+  constexpr int TM = kFwdPairs<H>, RT = TM / 16;
   constexpr int LDE = kMaxRbf + 4;
   constexpr int LDHB = H + 8;          // bf16 row stride of the split hidden activations
   constexpr int PHB = TM * LDHB;       // plane stride
@@ -951,7 +953,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   // filter stream while the caller's stream walks the atom-side chain
   st.order(s, st.filt, 0);
   for (int l = 0; l < b.L; ++l) {
-    filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs), kPairBlock, 0, st.filt>>>(
+    filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs<H>), kPairBlock, 0, st.filt>>>(
         b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc,
         b.Wf + (size_t)l * b.pcap * 3 * H, b.wbf + wbf_f2w(H, l));
     if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
@@ -1042,7 +1044,7 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
   switch (stage) {
     case MLFF_STAGE_FILTER_FWD:
-      filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs), kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
+      filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs<H>), kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
                                                        widths, b.R, mw.RP, b.rc, Wl,
                                                        b.wbf + wbf_f2w(H, l));
       break;
