@@ -415,3 +415,46 @@ def test_weight_update_refreshes_split_planes(gpu):
     e1, f1, _ = model.evaluate(Zt, Pt, sizes)
     assert not torch.equal(f0, f1)
     _check_ef(model, Z, P32, sizes, e1.cpu().numpy(), f1.cpu().numpy(), "after weight update")
+
+
+def test_invariances_and_central_differences_on_the_kernels(gpu):
+    """The reference's model checks (tests/unit/test_student_model.py:205-221,
+    294-346, 377-411) on the HIP path itself: translation, permutation,
+    extensivity over far-apart copies, and forces against central differences
+    of the kernels' own energy.  Positions sit on a 2^-10 A grid, so shifts by
+    whole multiples of 16 A are exact in float32 and leave every edge vector
+    unchanged: those cases must agree bit for bit."""
+    note("invariances and finite differences on the kernels")
+    model = _model(128, 3, 20, seed=0, bias_scale=0.05)
+    mol = _mols(1, 24, 24, seed=71)[0]
+    Z = mol.numbers
+    P = (np.round(mol.positions * 1024.0) / 1024.0).astype(np.float32)
+    n = len(Z)
+
+    def ef(z, p):
+        e64, f, _ = model.evaluate(torch.tensor(z, device=gpu),
+                                   torch.tensor(np.ascontiguousarray(p, dtype=np.float32),
+                                                device=gpu), np.array([len(z)]))
+        return e64.item(), f.cpu().numpy()
+
+    e0, f0 = ef(Z, P)
+    exact = np.array([64.0, -32.0, 16.0], dtype=np.float32)
+    assert np.array_equal((P + exact) - exact, P)
+    ex, fx = ef(Z, P + exact)
+    assert ex == e0 and np.array_equal(fx, f0)
+    et, ft = ef(Z, P + np.array([3.1, -2.2, 7.7], dtype=np.float32))
+    assert abs(et - e0) / n < 1e-6 and np.abs(ft - f0).max() < 1e-4
+    perm = np.random.default_rng(5).permutation(n)
+    ep, fp = ef(Z[perm], P[perm])
+    assert abs(ep - e0) / n < 1e-6 and np.abs(fp - f0[perm]).max() < 1e-4
+    e2, f2 = ef(np.concatenate([Z, Z]), np.concatenate([P, P + np.float32(64.0)]))
+    assert abs(e2 - 2.0 * e0) <= 1e-12 * abs(e0)
+    assert np.array_equal(f2[:n], f0) and np.array_equal(f2[n:], f0)
+    h = np.float32(2e-3)
+    for atom, k in [(0, 0), (5, 1), (11, 2), (23, 0)]:
+        pp, pm = P.copy(), P.copy()
+        pp[atom, k] += h
+        pm[atom, k] -= h
+        step = float(pp[atom, k]) - float(pm[atom, k])
+        fd = -(ef(Z, pp)[0] - ef(Z, pm)[0]) / step
+        assert abs(fd - f0[atom, k]) < 5e-3 * max(1.0, abs(f0[atom, k])), (atom, k, fd, f0[atom, k])
