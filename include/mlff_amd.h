@@ -151,7 +151,8 @@ int64_t mlff_plan_workspace_bytes(const mlff_plan* plan);
 /* Bind the workspace and the per-batch constants.  numbers_dev [N] int64,
  * mol_ptr_dev [B+1], atom_mol_dev [N] (structure index of each atom),
  * cells_dev [B,3,3] (rows = lattice vectors) and pbc_dev [B,3] may be NULL
- * for non-periodic batches.  weights_dev is the packed blob.  Binding is
+ * for non-periodic batches.  weights_dev is the packed blob.  The workspace
+ * and the weights must be 16-byte aligned (MLFF_ERR_INVALID otherwise).  Binding is
  * synchronous: it clears the plan's counters and splits the filter weights
  * into their bf16 planes inside the workspace, then waits for the device;
  * it also creates the plan's second stream on the current device. */

@@ -3,6 +3,7 @@
 // and the weight layout; no device memory is allocated here.
 
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -328,6 +329,11 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   }
   if (!weights_dev || !numbers_dev || !mol_ptr_dev || !atom_mol_dev) {
     set_error("mlff_plan_bind: null pointer");
+    return MLFF_ERR_INVALID;
+  }
+  // the kernels read and write these with 16-byte vector accesses
+  if ((reinterpret_cast<uintptr_t>(ws) | reinterpret_cast<uintptr_t>(weights_dev)) & 15) {
+    set_error("mlff_plan_bind: workspace and weights must be 16-byte aligned");
     return MLFF_ERR_INVALID;
   }
   if (pl->cfg.any_periodic && (!cells_dev || !pbc_dev)) {

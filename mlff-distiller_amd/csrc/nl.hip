@@ -7,7 +7,8 @@
 //
 // Passes (all deterministic, no atomics):
 //   count : per atom, number of neighbours (deg) and of neighbours j > i (udeg)
-//   scan  : exclusive prefix sums -> row_ptr (edges) and pair_ptr (pairs)
+//   scan  : exclusive prefix sums -> row_ptr (edges) and pair_ptr (pairs), per
+//           1024-atom slice, then the offsets of the slices before
 //   fill  : per atom, neighbours in ascending j -> col; for j > i the pair
 //           (lo=i, hi=j) gets its index, lattice shift and geometry
 //           {p_hi - p_lo (+ shift @ cell), d}
@@ -326,69 +327,113 @@ __global__ void nl_count_kernel(const float* __restrict__ pos, int N,
   }
 }
 
-// Single-workgroup exclusive scan of deg and udeg (any N), in chunks of
-// 1024 x kScanPer elements: coalesced loads into LDS, a contiguous scan of
-// kScanPer elements per thread, wave prefix sums by shuffles, one LDS pass
-// over the 16 wave totals, coalesced stores, and a carry into the next chunk.
-constexpr int kScanPer = 8;
-constexpr int kScanChunk = 1024 * kScanPer;
+// Exclusive scan of deg and udeg in two passes over 1024-atom slices, one
+// workgroup per slice (any N; a slice's work is a handful of loads, so the
+// scan costs two short launches at any N instead of a chain of barriers in
+// one workgroup):
+//   nl_scan_slice_kernel  : slice-local exclusive sums -> row_ptr, pair_ptr;
+//                           the slice totals are parked in deg[first atom of
+//                           the slice] and udeg[...] (deg and udeg are not
+//                           read after the scan)
+//   nl_scan_offset_kernel : adds the sum of the totals of the slices before,
+//                           and writes row_ptr[N], pair_ptr[N] and the counts.
+constexpr int kScanThreads = 256;
+constexpr int kScanSlice = 1024;
+constexpr int kScanPer = kScanSlice / kScanThreads;   // 4 consecutive atoms per thread
 
-__global__ __launch_bounds__(1024) void nl_scan_kernel(const int32_t* __restrict__ deg,
-                                                       const int32_t* __restrict__ udeg, int N,
-                                                       int32_t* __restrict__ row_ptr,
-                                                       int32_t* __restrict__ pair_ptr,
-                                                       int32_t* __restrict__ counts,
-                                                       int32_t ecap, int32_t pcap) {
-  __shared__ int b1[kScanChunk];
-  __shared__ int b2[kScanChunk];
-  __shared__ int w1[16], w2[16];
+__global__ __launch_bounds__(kScanThreads) void nl_scan_slice_kernel(
+    int32_t* __restrict__ deg, int32_t* __restrict__ udeg, int N, int32_t* __restrict__ row_ptr,
+    int32_t* __restrict__ pair_ptr) {
+  __shared__ __align__(16) int b1[kScanSlice];
+  __shared__ __align__(16) int b2[kScanSlice];
+  __shared__ int w1[kScanThreads / 64], w2[kScanThreads / 64];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  int carry1 = 0, carry2 = 0;
-  for (int base = 0; base < N; base += kScanChunk) {
-    const int n = min(kScanChunk, N - base);
-    for (int i = t; i < kScanChunk; i += 1024) {
-      b1[i] = i < n ? deg[base + i] : 0;
-      b2[i] = i < n ? udeg[base + i] : 0;
-    }
-    __syncthreads();
-    int x1[kScanPer], x2[kScanPer];
-    int s1 = 0, s2 = 0;
+  const int base = blockIdx.x * kScanSlice;
+  const int n = min(kScanSlice, N - base);
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-      x1[k] = s1; s1 += b1[t * kScanPer + k];
-      x2[k] = s2; s2 += b2[t * kScanPer + k];
-    }
-    const int i1 = wave_inclusive_scan(s1), i2 = wave_inclusive_scan(s2);
-    if (lane == 63) { w1[wv] = i1; w2[wv] = i2; }
-    __syncthreads();
-    int o1 = carry1, o2 = carry2;
-    for (int q = 0; q < wv; ++q) { o1 += w1[q]; o2 += w2[q]; }
-    o1 += i1 - s1;
-    o2 += i2 - s2;
-    int tot1 = carry1, tot2 = carry2;
-    for (int q = 0; q < 16; ++q) { tot1 += w1[q]; tot2 += w2[q]; }
-    __syncthreads();   // every thread has read its chunk elements and the wave totals
+  for (int k = 0; k < kScanPer; ++k) {
+    const int i = k * kScanThreads + t;
+    b1[i] = i < n ? deg[base + i] : 0;
+    b2[i] = i < n ? udeg[base + i] : 0;
+  }
+  __syncthreads();
+  const int4 a1 = reinterpret_cast<const int4*>(b1)[t];
+  const int4 a2 = reinterpret_cast<const int4*>(b2)[t];
+  const int s1 = a1.x + a1.y + a1.z + a1.w, s2 = a2.x + a2.y + a2.z + a2.w;
+  const int i1 = wave_inclusive_scan(s1), i2 = wave_inclusive_scan(s2);
+  if (lane == 63) { w1[wv] = i1; w2[wv] = i2; }
+  __syncthreads();
+  int o1 = i1 - s1, o2 = i2 - s2, tot1 = 0, tot2 = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-      b1[t * kScanPer + k] = o1 + x1[k];
-      b2[t * kScanPer + k] = o2 + x2[k];
-    }
-    __syncthreads();
-    for (int i = t; i < n; i += 1024) {
+  for (int q = 0; q < kScanThreads / 64; ++q) {
+    if (q < wv) { o1 += w1[q]; o2 += w2[q]; }
+    tot1 += w1[q];
+    tot2 += w2[q];
+  }
+  reinterpret_cast<int4*>(b1)[t] = make_int4(o1, o1 + a1.x, o1 + a1.x + a1.y,
+                                             o1 + a1.x + a1.y + a1.z);
+  reinterpret_cast<int4*>(b2)[t] = make_int4(o2, o2 + a2.x, o2 + a2.x + a2.y,
+                                             o2 + a2.x + a2.y + a2.z);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int i = k * kScanThreads + t;
+    if (i < n) {
       row_ptr[base + i] = b1[i];
       pair_ptr[base + i] = b2[i];
     }
-    carry1 = tot1;
-    carry2 = tot2;
-    __syncthreads();   // the next chunk overwrites b1, b2, w1, w2
   }
+  // thread 0 read deg[base] itself before the first barrier; no other
+  // workgroup touches this slice
   if (t == 0) {
-    const int E = carry1, P = carry2;
-    row_ptr[N] = E;
-    pair_ptr[N] = P;
-    counts[MLFF_COUNT_EDGES] = E;
-    counts[MLFF_COUNT_PAIRS] = P;
-    if (E > ecap || P > pcap) counts[MLFF_COUNT_OVERFLOW] = 1;
+    deg[base] = tot1;
+    udeg[base] = tot2;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void nl_scan_offset_kernel(
+    const int32_t* __restrict__ deg, const int32_t* __restrict__ udeg, int N,
+    int32_t* __restrict__ row_ptr, int32_t* __restrict__ pair_ptr, int32_t* __restrict__ counts,
+    int32_t ecap, int32_t pcap) {
+  __shared__ int w1[kScanThreads / 64], w2[kScanThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int g = blockIdx.x, G = gridDim.x;
+  // totals of the slices before g (the last workgroup also adds its own:
+  // that sum is the number of edges and pairs)
+  const int upto = g == G - 1 ? G : g;
+  int s1 = 0, s2 = 0;
+  for (int q = t; q < upto; q += kScanThreads) {
+    s1 += deg[q * kScanSlice];
+    s2 += udeg[q * kScanSlice];
+  }
+  s1 = wave_inclusive_scan(s1);
+  s2 = wave_inclusive_scan(s2);
+  if (lane == 63) { w1[wv] = s1; w2[wv] = s2; }
+  __syncthreads();
+  int all1 = 0, all2 = 0;
+#pragma unroll
+  for (int q = 0; q < kScanThreads / 64; ++q) { all1 += w1[q]; all2 += w2[q]; }
+  int o1 = all1, o2 = all2;
+  if (g == G - 1) {   // own slice total out of the offset
+    o1 -= deg[g * kScanSlice];
+    o2 -= udeg[g * kScanSlice];
+  }
+  const int base = g * kScanSlice;
+  const int n = min(kScanSlice, N - base);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int i = k * kScanThreads + t;
+    if (i < n) {
+      row_ptr[base + i] += o1;
+      pair_ptr[base + i] += o2;
+    }
+  }
+  if (g == G - 1 && t == 0) {
+    row_ptr[N] = all1;
+    pair_ptr[N] = all2;
+    counts[MLFF_COUNT_EDGES] = all1;
+    counts[MLFF_COUNT_PAIRS] = all2;
+    if (all1 > ecap || all2 > pcap) counts[MLFF_COUNT_OVERFLOW] = 1;
   }
 }
 
@@ -632,8 +677,11 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
   const int gw = (int)(((int64_t)a.N * 64 + kNlBlock - 1) / kNlBlock);
   nl_count_kernel<<<gw, kNlBlock, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                    a.deg, a.udeg, cl);
-  nl_scan_kernel<<<1, 1024, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr, a.counts, a.ecap,
-                                    build_pairs ? a.pcap : 0x7fffffff);
+  const int slices = (a.N + kScanSlice - 1) / kScanSlice;
+  nl_scan_slice_kernel<<<slices, kScanThreads, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr);
+  nl_scan_offset_kernel<<<slices, kScanThreads, 0, s>>>(a.deg, a.udeg, a.N, a.row_ptr, a.pair_ptr,
+                                                        a.counts, a.ecap,
+                                                        build_pairs ? a.pcap : 0x7fffffff);
   nl_fill_kernel<<<gw, kNlBlock, 0, s>>>(a.pos, a.N, a.mol_ptr, a.atom_mol, a.cells, a.pbc, a.rc,
                                   a.row_ptr, a.pair_ptr, a.col, a.edge_shift,
                                   build_pairs ? a.edge_pair : nullptr, a.ecap,

@@ -60,17 +60,24 @@ inline int pair_grid(const Buffers& b, int pairs_per_tile) {
   return std::max(8, std::min(b.blocks, (tiles + 7) / 8 * 8));
 }
 
-__global__ void embed_kernel(int N, int H, const int64_t* __restrict__ Z,
+// S0 = emb[Z] and V0 = 0, four channels per thread: thread t of an atom's
+// H/4 threads writes S0 channels 4t..4t+3 and the same channels of the
+// three V0 components (16-byte stores; rows are 16-byte aligned since H % 4 == 0)
+template <int H>
+__global__ void embed_kernel(int N, const int64_t* __restrict__ Z,
                              const float* __restrict__ emb, float* __restrict__ S0,
                              float* __restrict__ V0) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)N * H;
-  if (idx >= total) return;
-  const int i = (int)(idx / H), c = (int)(idx % H);
-  S0[idx] = emb[Z[i] * H + c];
-  V0[(int64_t)i * 3 * H + c] = 0.f;
-  V0[(int64_t)i * 3 * H + H + c] = 0.f;
-  V0[(int64_t)i * 3 * H + 2 * H + c] = 0.f;
+  constexpr int Q = H / 4;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * Q) return;
+  const int i = idx / Q, c = idx % Q;
+  const float4 e = reinterpret_cast<const float4*>(emb + Z[i] * H)[c];
+  reinterpret_cast<float4*>(S0)[idx] = e;
+  float4* v = reinterpret_cast<float4*>(V0) + (size_t)i * 3 * Q + c;
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  v[0] = z;
+  v[Q] = z;
+  v[2 * Q] = z;
 }
 
 // ---------------------------------------------------------------------------
@@ -959,9 +966,9 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
     if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
   }
   {
-    const int64_t total = (int64_t)N * H;
-    embed_kernel<<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        N, H, b.numbers, b.w + mw.emb, b.S, b.V);
+    const int total = N * (H / 4);
+    embed_kernel<H><<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        N, b.numbers, b.w + mw.emb, b.S, b.V);
   }
   const int rows_per_block = kBlock / (H / 4);
   const int msg_grid = ((N + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;

@@ -73,6 +73,24 @@ def test_neighbor_list_batched_molecules(gpu):
     np.testing.assert_array_equal(col.cpu().numpy(), cl)
 
 
+@pytest.mark.parametrize("n", [1023, 1024, 1025, 2049, 5 * 1024])
+def test_neighbor_list_scan_slice_boundaries(gpu, n):
+    """The row offsets are scanned per 1024-atom slice: atom counts at and
+    one either side of a slice boundary give the oracle's CSR exactly."""
+    note(f"NL scan slices, {n} atoms in two structures")
+    rng = np.random.default_rng(n)
+    side = (n / 0.03) ** (1 / 3)            # about 0.03 atoms per A^3: ~15 neighbours
+    pos32 = rng.uniform(0, side, size=(n, 3)).astype(np.float32)
+    cut = n // 3
+    mol_ptr = np.array([0, cut, n])
+    batch = np.repeat([0, 1], [cut, n - cut])
+    row_ptr, col, _ = pkg().radius_graph_csr(torch.tensor(pos32, device=gpu), 5.0,
+                                             torch.tensor(batch, device=gpu))
+    rp, cl, _ = nl_ref.neighbor_csr(pos32, mol_ptr, 5.0)
+    np.testing.assert_array_equal(row_ptr.cpu().numpy(), rp)
+    np.testing.assert_array_equal(col.cpu().numpy(), cl)
+
+
 def test_neighbor_list_periodic_water(gpu):
     note("NL periodic water box 1000 molecules")
     synth = pkg_mod("synth")
