@@ -299,18 +299,31 @@ __global__ __launch_bounds__(kUpdBlock) void update_fwd_kernel(
   constexpr int NTC = NT1, NT2 = 3 * NTC;
   __shared__ __align__(16) float x_lds[TM * LDX];
   __shared__ __align__(16) float h_lds[TM * LDH];
+  constexpr int PER = TM * H / kUpdBlock;   // tile elements per thread
+  static_assert(PER * kUpdBlock == TM * H, "tile split");
   const int base = blockIdx.x * TM;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
-    const int m = idx / H, c = idx - m * H, i = base + m;
-    float sv = 0.f, nv = 0.f;
-    if (i < N) {
-      sv = s_mid[(size_t)i * H + c];
+  {
+    // every load of the tile in flight at once (rows past N read atom N-1
+    // and are zeroed), then the norms
+    float sv[PER], x0[PER], x1[PER], x2[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = threadIdx.x + k * kUpdBlock, m = idx / H, c = idx % H;
+      const int i = min(base + m, N - 1);
+      sv[k] = s_mid[(size_t)i * H + c];
       const float* vv = v_mid + (size_t)i * 3 * H;
-      nv = norm3_exact(vv[c], vv[H + c], vv[2 * H + c]);
+      x0[k] = vv[c];
+      x1[k] = vv[H + c];
+      x2[k] = vv[2 * H + c];
     }
-    x_lds[m * LDX + c] = sv;
-    x_lds[m * LDX + H + c] = nv;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = threadIdx.x + k * kUpdBlock, m = idx / H, c = idx % H;
+      const bool in = base + m < N;
+      x_lds[m * LDX + c] = in ? sv[k] : 0.f;
+      x_lds[m * LDX + H + c] = in ? norm3_exact(x0[k], x1[k], x2[k]) : 0.f;
+    }
   }
   __syncthreads();
   if (wave < CT1 / NT1) {
@@ -343,13 +356,32 @@ __global__ __launch_bounds__(kUpdBlock) void update_fwd_kernel(
     for (int q = 0; q < NTC; ++q)
 #pragma unroll
       for (int sp = 0; sp < 3; ++sp) cb[sp * NTC + q] = sp * H + (wave * NTC + q) * 16;
-    zero_acc(acc);
-    mma_tile_k<RT, NT2, H>(h_lds, LDH, w + lw.u2w, H, cb, acc);
+    // the epilogue's operands (biases, mixing matrix, the tile's vectors)
+    // are loaded before the product, so their latency hides behind it
     const float* b2 = w + lw.u2b;
     const float* U = w + lw.mix;
     float u[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) u[k] = U[k];
+    float bias[NTC][3];
+    float xv[RT][NTC][4][3];
+#pragma unroll
+    for (int q = 0; q < NTC; ++q) {
+      const int c = (wave * NTC + q) * 16 + r;
+#pragma unroll
+      for (int sp = 0; sp < 3; ++sp) bias[q][sp] = b2[sp * H + c];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = min(base + rt * 16 + 4 * g + k, N - 1);
+          const float* vv = v_mid + (size_t)i * 3 * H;
+#pragma unroll
+          for (int sp = 0; sp < 3; ++sp) xv[rt][q][k][sp] = vv[sp * H + c];
+        }
+    }
+    zero_acc(acc);
+    mma_tile_k<RT, NT2, H>(h_lds, LDH, w + lw.u2w, H, cb, acc);
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -359,12 +391,11 @@ __global__ __launch_bounds__(kUpdBlock) void update_fwd_kernel(
         for (int k = 0; k < 4; ++k) {
           const int m = rt * 16 + 4 * g + k, i = base + m;
           if (i >= N) continue;
-          const float ds = acc[rt][q][k] + b2[c];
-          const float gt1 = acc[rt][NTC + q][k] + b2[H + c];
-          const float gt2 = acc[rt][2 * NTC + q][k] + b2[2 * H + c];
+          const float ds = acc[rt][q][k] + bias[q][0];
+          const float gt1 = acc[rt][NTC + q][k] + bias[q][1];
+          const float gt2 = acc[rt][2 * NTC + q][k] + bias[q][2];
           s_out[(size_t)i * H + c] = x_lds[m * LDX + c] + ds;
-          const float* vv = v_mid + (size_t)i * 3 * H;
-          const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
+          const float x0 = xv[rt][q][k][0], x1 = xv[rt][q][k][1], x2 = xv[rt][q][k][2];
           float* vo = v_out + (size_t)i * 3 * H;
           vo[c] = x0 * gt1 + (u[0] * x0 + u[1] * x1 + u[2] * x2) * gt2;
           vo[H + c] = x1 * gt1 + (u[3] * x0 + u[4] * x1 + u[5] * x2) * gt2;
@@ -394,9 +425,20 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(int N, const float* __r
   __shared__ __align__(16) float g2_lds[TM * LD4];
   const int base = blockIdx.x * TM;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  for (int idx = threadIdx.x; idx < TM * H; idx += kBlock) {
-    const int m = idx / H, c = idx - m * H, i = base + m;
-    x_lds[m * LDX + c] = (i < N) ? sL[(size_t)i * H + c] : 0.f;
+  {
+    constexpr int PER = TM * H / kBlock;
+    static_assert(PER * kBlock == TM * H, "tile split");
+    float xs[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = threadIdx.x + k * kBlock, m = idx / H, c = idx % H;
+      xs[k] = sL[(size_t)min(base + m, N - 1) * H + c];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = threadIdx.x + k * kBlock, m = idx / H, c = idx % H;
+      x_lds[m * LDX + c] = (base + m < N) ? xs[k] : 0.f;
+    }
   }
   __syncthreads();
   if (wave < HP2 / 16) {   // Linear(H, H/2) + SiLU, one column tile per wave
@@ -498,24 +540,41 @@ __global__ __launch_bounds__(kUpdBlock) void update_bwd_kernel(
   float u[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) u[k] = U[k];
-  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
-    const int m = idx / H, c = idx - m * H, i = base + m;
-    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
-    if (i < N) {
-      y0 = gs2[(size_t)i * H + c];
+  // loads in batches of kB elements per thread, all of a batch in flight at
+  // once (rows past N read atom N-1 and are zeroed)
+  constexpr int PER = TM * H / kUpdBlock;
+  constexpr int kB = PER < 4 ? PER : 4;
+  static_assert(PER * kUpdBlock == TM * H && PER % kB == 0, "tile split");
+#pragma unroll
+  for (int kb = 0; kb < PER; kb += kB) {
+    float y0[kB], x0[kB], x1[kB], x2[kB], q0[kB], q1[kB], q2[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      const int idx = threadIdx.x + (kb + k) * kUpdBlock, m = idx / H, c = idx % H;
+      const int i = min(base + m, N - 1);
+      y0[k] = gs2[(size_t)i * H + c];
       if (gv2 != nullptr) {
         const float* vv = v_mid + (size_t)i * 3 * H;
         const float* gg = gv2 + (size_t)i * 3 * H;
-        const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
-        const float q0 = gg[c], q1 = gg[H + c], q2 = gg[2 * H + c];
-        y1 = q0 * x0 + q1 * x1 + q2 * x2;
-        y2 = q0 * (u[0] * x0 + u[1] * x1 + u[2] * x2) + q1 * (u[3] * x0 + u[4] * x1 + u[5] * x2) +
-             q2 * (u[6] * x0 + u[7] * x1 + u[8] * x2);
+        x0[k] = vv[c]; x1[k] = vv[H + c]; x2[k] = vv[2 * H + c];
+        q0[k] = gg[c]; q1[k] = gg[H + c]; q2[k] = gg[2 * H + c];
       }
     }
-    y_lds[m * LDY + c] = y0;
-    y_lds[m * LDY + H + c] = y1;
-    y_lds[m * LDY + 2 * H + c] = y2;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      const int idx = threadIdx.x + (kb + k) * kUpdBlock, m = idx / H, c = idx % H;
+      const bool in = base + m < N;
+      float y1 = 0.f, y2 = 0.f;
+      if (gv2 != nullptr) {
+        y1 = q0[k] * x0[k] + q1[k] * x1[k] + q2[k] * x2[k];
+        y2 = q0[k] * (u[0] * x0[k] + u[1] * x1[k] + u[2] * x2[k]) +
+             q1[k] * (u[3] * x0[k] + u[4] * x1[k] + u[5] * x2[k]) +
+             q2[k] * (u[6] * x0[k] + u[7] * x1[k] + u[8] * x2[k]);
+      }
+      y_lds[m * LDY + c] = in ? y0[k] : 0.f;
+      y_lds[m * LDY + H + c] = in ? y1 : 0.f;
+      y_lds[m * LDY + 2 * H + c] = in ? y2 : 0.f;
+    }
   }
   __syncthreads();
   {
@@ -525,6 +584,16 @@ __global__ __launch_bounds__(kUpdBlock) void update_bwd_kernel(
       f4 acc[RT][NT];
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) cb[ct] = (wave * NT + ct) * 16;
+      float av[RT][NT][4];   // pre-activations, loaded before the product
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int i = min(base + rt * 16 + 4 * g + k, N - 1);
+            av[rt][ct][k] = a_upd[(size_t)i * H + cb[ct] + r];
+          }
       zero_acc(acc);
       mma_tile_k<RT, NT, 3 * H, 2, true>(y_lds, LDY, w + lw.u2t, 3 * H, cb, acc);
 #pragma unroll
@@ -534,7 +603,7 @@ __global__ __launch_bounds__(kUpdBlock) void update_bwd_kernel(
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             const int m = rt * 16 + 4 * g + k, i = base + m, c = cb[ct] + r;
-            const float a = (i < N) ? a_upd[(size_t)i * H + c] : 0.f;
+            const float a = (i < N) ? av[rt][ct][k] : 0.f;
             a_lds[m * LDA + c] = acc[rt][ct][k] * silu_grad(a);
           }
     }
@@ -559,28 +628,43 @@ __global__ __launch_bounds__(kUpdBlock) void update_bwd_kernel(
     }
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < TM * H; idx += kUpdBlock) {
-    const int m = idx / H, c = idx - m * H, i = base + m;
-    if (i >= N) continue;
-    gs1[(size_t)i * H + c] = gs2[(size_t)i * H + c] + x_lds[m * LDX + c];
-    const float gn = x_lds[m * LDX + H + c];
-    const float* vv = v_mid + (size_t)i * 3 * H;
-    const float x0 = vv[c], x1 = vv[H + c], x2 = vv[2 * H + c];
-    const float n = norm3_exact(x0, x1, x2);
-    const float sc = (n > 0.f) ? gn / n : 0.f;
-    float o0 = sc * x0, o1 = sc * x1, o2 = sc * x2;
-    if (gv2 != nullptr) {
-      const float gt1 = g12[(size_t)i * 2 * H + c], gt2 = g12[(size_t)i * 2 * H + H + c];
-      const float* gg = gv2 + (size_t)i * 3 * H;
-      const float q0 = gg[c], q1 = gg[H + c], q2 = gg[2 * H + c];
-      o0 += q0 * gt1 + (u[0] * q0 + u[3] * q1 + u[6] * q2) * gt2;
-      o1 += q1 * gt1 + (u[1] * q0 + u[4] * q1 + u[7] * q2) * gt2;
-      o2 += q2 * gt1 + (u[2] * q0 + u[5] * q1 + u[8] * q2) * gt2;
+#pragma unroll
+  for (int kb = 0; kb < PER; kb += kB) {
+    float s2[kB], x0[kB], x1[kB], x2[kB], t1[kB], t2[kB], q0[kB], q1[kB], q2[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      const int idx = threadIdx.x + (kb + k) * kUpdBlock, m = idx / H, c = idx % H;
+      const int i = min(base + m, N - 1);
+      s2[k] = gs2[(size_t)i * H + c];
+      const float* vv = v_mid + (size_t)i * 3 * H;
+      x0[k] = vv[c]; x1[k] = vv[H + c]; x2[k] = vv[2 * H + c];
+      if (gv2 != nullptr) {
+        t1[k] = g12[(size_t)i * 2 * H + c];
+        t2[k] = g12[(size_t)i * 2 * H + H + c];
+        const float* gg = gv2 + (size_t)i * 3 * H;
+        q0[k] = gg[c]; q1[k] = gg[H + c]; q2[k] = gg[2 * H + c];
+      }
     }
-    float* go = gv1 + (size_t)i * 3 * H;
-    go[c] = o0;
-    go[H + c] = o1;
-    go[2 * H + c] = o2;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      const int idx = threadIdx.x + (kb + k) * kUpdBlock, m = idx / H, c = idx % H;
+      const int i = base + m;
+      if (i >= N) continue;
+      gs1[(size_t)i * H + c] = s2[k] + x_lds[m * LDX + c];
+      const float gn = x_lds[m * LDX + H + c];
+      const float n = norm3_exact(x0[k], x1[k], x2[k]);
+      const float sc = (n > 0.f) ? gn / n : 0.f;
+      float o0 = sc * x0[k], o1 = sc * x1[k], o2 = sc * x2[k];
+      if (gv2 != nullptr) {
+        o0 += q0[k] * t1[k] + (u[0] * q0[k] + u[3] * q1[k] + u[6] * q2[k]) * t2[k];
+        o1 += q1[k] * t1[k] + (u[1] * q0[k] + u[4] * q1[k] + u[7] * q2[k]) * t2[k];
+        o2 += q2[k] * t1[k] + (u[2] * q0[k] + u[5] * q1[k] + u[8] * q2[k]) * t2[k];
+      }
+      float* go = gv1 + (size_t)i * 3 * H;
+      go[c] = o0;
+      go[H + c] = o1;
+      go[2 * H + c] = o2;
+    }
   }
 }
 
