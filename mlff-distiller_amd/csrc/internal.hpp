@@ -55,7 +55,7 @@ struct Buffers {
   const float* w;                   // weight blob
   const ModelW* mw;
   // bf16 three-plane copies of the filter's second layer, per block l:
-  // W2 [3H][H] at wbf + wbf_f2w(l), W2^T [H][3H] at wbf + wbf_f2t(l); plane stride 3H^2
+  // W2 [3H][H] at wbf + wbf_f2w(l); plane stride 3H^2
   uint16_t* wbf;
   const int64_t* numbers;
   const int32_t* mol_ptr;
@@ -81,9 +81,8 @@ struct Buffers {
   hipEvent_t* ev;
 };
 __host__ __device__ inline int64_t wbf_plane(int H) { return 3 * (int64_t)H * H; }
-__host__ __device__ inline int64_t wbf_f2w(int H, int l) { return (int64_t)l * 6 * wbf_plane(H); }
-__host__ __device__ inline int64_t wbf_f2t(int H, int l) { return wbf_f2w(H, l) + 3 * wbf_plane(H); }
-__host__ __device__ inline int64_t wbf_elems(int H, int L) { return (int64_t)L * 6 * wbf_plane(H); }
+__host__ __device__ inline int64_t wbf_f2w(int H, int l) { return (int64_t)l * 3 * wbf_plane(H); }
+__host__ __device__ inline int64_t wbf_elems(int H, int L) { return (int64_t)L * 3 * wbf_plane(H); }
 // Split the filter weights into their bf16 planes (once per bind; synchronous).
 hipError_t split_filter_weights(const Buffers& b);
 hipError_t launch_forward(const Buffers& b, hipStream_t s);

@@ -993,21 +993,19 @@ __global__ void copy_f32_kernel(int n, const float* __restrict__ src, float* __r
   if (i < n) dst[i] = src[i];
 }
 
-// bf16 planes of W2 [3H][H] and W2^T [H][3H] of every block (both dense in the blob)
+// bf16 planes of W2 [3H][H] of every block (dense in the blob)
 __global__ void split_filter_weights_kernel(const float* __restrict__ w, ModelW mw,
                                             uint16_t* __restrict__ wbf) {
   const int H = mw.H;
   const int64_t S = wbf_plane(H);
-  const int64_t total = (int64_t)mw.L * 2 * S;
+  const int64_t total = (int64_t)mw.L * S;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int l = (int)(idx / (2 * S));
-    const int mat = (int)((idx / S) % 2);
+    const int l = (int)(idx / S);
     const int64_t e = idx % S;
-    const float x = w[(mat == 0 ? mw.layer[l].f2w : mw.layer[l].f2t) + e];
     uint16_t h, m, lo;
-    split3_bf16(x, h, m, lo);
-    uint16_t* dst = wbf + (mat == 0 ? wbf_f2w(H, l) : wbf_f2t(H, l)) + e;
+    split3_bf16(w[mw.layer[l].f2w + e], h, m, lo);
+    uint16_t* dst = wbf + wbf_f2w(H, l) + e;
     dst[0] = h;
     dst[S] = m;
     dst[2 * S] = lo;
