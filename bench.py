@@ -449,9 +449,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=80)
-    ap.add_argument("--frame-stride", type=int, default=100,
-                    help="MD steps between training frames (config 5)")
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--frame-stride", type=int, default=250,
+                    help="MD steps between training frames (config 5; 80 x 250 steps = 10 ps)")
+    ap.add_argument("--train-steps", type=int, default=5000)
     ap.add_argument("--mts-outer", type=int, default=5000)
     ap.add_argument("--mts-threshold", type=float, default=5.0)
     ap.add_argument("--mts-inner", type=int, default=4,
