@@ -65,6 +65,24 @@ def test_plan_workspace_accounting():
     C.lib().mlff_plan_destroy(h)
 
 
+def test_plan_bind_rejects_misaligned_buffers():
+    """The kernels access the workspace and the weights with 16-byte vector
+    loads; bind refuses other alignments before it touches the device (the
+    pointers below are never dereferenced)."""
+    cfg = C.Config(32, 2, 16, 118, 5.0, 100, 2, 4000, 2000, 0, 64)
+    h = C.vp()
+    C.check("create", C.lib().mlff_plan_create(ctypes.byref(h), ctypes.byref(cfg)))
+    nbytes = C.lib().mlff_plan_workspace_bytes(h)
+    ok, odd = 1 << 20, (1 << 20) + 4
+    args = (ctypes.c_void_p(ok),) * 4                       # numbers, mol_ptr, atom_mol
+    for ws, w in ((odd, ok), (ok, odd)):
+        rc = C.lib().mlff_plan_bind(h, ctypes.c_void_p(ws), nbytes, ctypes.c_void_p(w),
+                                    *args[:3], None, None)
+        assert rc == -1
+        assert b"16-byte aligned" in C.lib().mlff_last_error()
+    C.lib().mlff_plan_destroy(h)
+
+
 def test_plan_create_rejects_bad_sizes():
     cfg = C.Config(128, 3, 20, 118, 5.0, 0, 1, 10, 10, 0, 0)
     h = C.vp()
