@@ -268,16 +268,29 @@ __global__ void cl_place_kernel(int N, const int32_t* __restrict__ atom_cell,
 }
 
 // ascending atom order inside every cell (placement order is not deterministic)
+// One wave per cell.  A cell of at most 64 atoms is ranked in registers: lane
+// q holds the cell's q-th atom and its rank is the number of smaller indices
+// (indices are distinct), so each lane writes its atom to slot rank; every
+// lane has loaded its atom before any lane stores.  A larger cell is sorted
+// by one lane (insertion sort).
 __global__ void cl_sort_cells_kernel(int C, const int32_t* __restrict__ cell_start,
                                      int32_t* __restrict__ cell_atoms) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= C) return;
-  const int lo = cell_start[c], hi = cell_start[c + 1];
-  for (int k = lo + 1; k < hi; ++k) {
-    const int v = cell_atoms[k];
-    int m = k - 1;
-    while (m >= lo && cell_atoms[m] > v) { cell_atoms[m + 1] = cell_atoms[m]; --m; }
-    cell_atoms[m + 1] = v;
+  const int lo = cell_start[c], n = cell_start[c + 1] - lo;
+  if (n <= 64) {
+    const int v = lane < n ? cell_atoms[lo + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (__shfl(v, j, 64) < v) ? 1 : 0;
+    if (lane < n) cell_atoms[lo + rank] = v;
+  } else if (lane == 0) {
+    for (int k = lo + 1; k < lo + n; ++k) {
+      const int v = cell_atoms[k];
+      int m = k - 1;
+      while (m >= lo && cell_atoms[m] > v) { cell_atoms[m + 1] = cell_atoms[m]; --m; }
+      cell_atoms[m + 1] = v;
+    }
   }
 }
 
@@ -670,7 +683,8 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
                                    a.cell_count);
     cl_scan_kernel<<<1, 1024, 0, s>>>(a.cell_count, C, a.cell_start);
     cl_place_kernel<<<g, tb, 0, s>>>(a.N, a.atom_cell, a.cell_start, a.cell_count, a.cell_atoms);
-    cl_sort_cells_kernel<<<(C + 127) / 128, 128, 0, s>>>(C, a.cell_start, a.cell_atoms);
+    cl_sort_cells_kernel<<<(unsigned)(((int64_t)C * 64 + 255) / 256), 256, 0, s>>>(
+        C, a.cell_start, a.cell_atoms);
     cl = CellList{a.grid, a.cell_start, a.cell_atoms, a.atom_cell};
   }
   // count and fill: one wave per atom

@@ -91,6 +91,23 @@ def test_neighbor_list_scan_slice_boundaries(gpu, n):
     np.testing.assert_array_equal(col.cpu().numpy(), cl)
 
 
+def test_neighbor_list_many_slices(gpu):
+    """400 molecules (about 22 scan slices): every slice's offset is the sum
+    of all slice totals before it."""
+    note("NL 400 molecules 10..100 (many scan slices)")
+    mols = _mols(400, 10, 100, seed=13)
+    synth = pkg_mod("synth")
+    Z, P, mol_ptr = synth.flatten(mols)
+    assert mol_ptr[-1] > 16 * 1024
+    pos32 = P.astype(np.float32)
+    row_ptr, col, _ = pkg().radius_graph_csr(
+        torch.tensor(pos32, device=gpu), 5.0,
+        torch.tensor(np.repeat(np.arange(len(mols)), np.diff(mol_ptr)), device=gpu))
+    rp, cl, _ = nl_ref.neighbor_csr(pos32, mol_ptr, 5.0)
+    np.testing.assert_array_equal(row_ptr.cpu().numpy(), rp)
+    np.testing.assert_array_equal(col.cpu().numpy(), cl)
+
+
 def test_neighbor_list_periodic_water(gpu):
     note("NL periodic water box 1000 molecules")
     synth = pkg_mod("synth")
