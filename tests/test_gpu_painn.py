@@ -429,6 +429,38 @@ def test_filter_stream_overlap_is_bit_identical(gpu):
         assert torch.equal(a, b)
 
 
+def test_persistent_grid_size_is_bit_identical(gpu):
+    """The autotuner picks the persistent grid of the pair kernels; every
+    pair's arithmetic is independent of which workgroup runs its tile, so
+    any grid (including sizes that are not a multiple of 8, which take the
+    plain strided tile order) gives the same bits."""
+    note("persistent grid 8 / 100 / 1536 / 4096")
+    engine = pkg_mod("engine")
+    synth = pkg_mod("synth")
+    mols = _mols(24, 10, 100, seed=53)
+    Z, P, mol_ptr = synth.flatten(mols)
+    sizes = np.diff(mol_ptr)
+    dev = torch.device(gpu)
+    lay = engine.make_layout(sizes, dev)
+    Zt = torch.tensor(Z, dtype=torch.int64, device=dev)
+    Pt = torch.tensor(P.astype(np.float32), device=dev)
+    model = _model(64, 2, 20, seed=0)
+    w = model.packed_weights(dev)
+    cap = engine.edge_capacity_bound(sizes, False)
+    outs = []
+    C = pkg_mod("_capi")
+    for blocks in (8, 100, 1536, 4096):
+        plan = engine.Plan(64, 2, 20, model.max_z, 5.0, lay, Zt, w, cap)
+        C.check("set_blocks", C.lib().mlff_plan_set_blocks(plan.handle, blocks))
+        e = torch.empty(len(sizes), dtype=torch.float64, device=dev)
+        f = torch.empty(len(Z), 3, dtype=torch.float32, device=dev)
+        plan.energy_forces(Pt, e, f)
+        torch.cuda.synchronize()
+        outs.append((e.clone(), f.clone()))
+    for e, f in outs[1:]:
+        assert torch.equal(e, outs[0][0]) and torch.equal(f, outs[0][1])
+
+
 def test_weight_update_refreshes_split_planes(gpu):
     """The filter's second-layer weights also live as bf16 planes split at
     bind; changing the weights in place must rebind the plan and refresh
