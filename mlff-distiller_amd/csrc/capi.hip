@@ -148,10 +148,13 @@ void carve(mlff_plan* pl) {
   sz[S_PLO] = 4 * P;
   sz[S_PHI] = 4 * P;
   sz[S_PSHIFT] = c.any_periodic ? 12 * P : 0;
-  sz[S_GEOM] = 16 * P;
+  // at least one pair row: edges whose pair is switched off (p = -1) read
+  // row 0 of the pair geometry and of the filter outputs
+  const int64_t P1 = P > 0 ? P : 1;
+  sz[S_GEOM] = 16 * P1;
   sz[S_PACC] = 16 * P;
   sz[S_GVEC] = 16 * P;
-  sz[S_WF] = 4 * L * P * 3 * H;
+  sz[S_WF] = 4 * L * P1 * 3 * H;
   sz[S_S] = 4 * (L + 1) * N * H;
   sz[S_V] = 4 * (L + 1) * N * 3 * H;
   sz[S_SMID] = 4 * L * N * H;

@@ -230,10 +230,8 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
     float ux, uy, uz;
     bool on;
   };
-  auto load = [&](int e) {
+  auto load = [&](int j, int p) {
     Edge q;
-    const int j = col[e];
-    const int p = edge_pair[e];
     q.on = p >= 0;
     const int pc = q.on ? p : 0;
     const float* wp = Wf + (size_t)pc * 3 * H + c;
@@ -267,15 +265,33 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
     }
   };
   constexpr int kUnroll = FWD ? kMsgUnrollFwd : kMsgUnrollBwd;
-  int e = e0;
-  for (; e + kUnroll - 1 < e1; e += kUnroll) {
-    Edge q[kUnroll];
+  // The row's lanes load the neighbour and pair indices of a batch of
+  // kBatch edges together (one edge per lane), the next batch's while this
+  // one is processed; each group of kUnroll edges takes its indices from the
+  // row's lanes by shuffle, so its operand loads wait for no index load.
+  // Past the row's end, edges are switched off (p = -1).
+  constexpr int kBatch = (LPR / kUnroll) * kUnroll;
+  static_assert(kBatch >= kUnroll, "a batch holds at least one group");
+  const int rbase = (threadIdx.x & 63) - sub;   // the row's first lane in the wave
+  int jn = i, pn = -1;
+  if (sub < kBatch && e0 + sub < e1) { jn = col[e0 + sub]; pn = edge_pair[e0 + sub]; }
+  for (int bs = e0; bs < e1; bs += kBatch) {
+    const int jc = jn, pcur = pn;
+    jn = i; pn = -1;
+    if (sub < kBatch && bs + kBatch + sub < e1) {
+      jn = col[bs + kBatch + sub];
+      pn = edge_pair[bs + kBatch + sub];
+    }
+    const int nb = min(kBatch, e1 - bs);
+    for (int k = 0; k < nb; k += kUnroll) {
+      Edge q[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) q[u] = load(e + u);
+      for (int u = 0; u < kUnroll; ++u)
+        q[u] = load(__shfl(jc, rbase + k + u, 64), __shfl(pcur, rbase + k + u, 64));
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) accumulate(q[u]);
+      for (int u = 0; u < kUnroll; ++u) accumulate(q[u]);
+    }
   }
-  for (; e < e1; ++e) accumulate(load(e));
   const float* vi = v_in + (size_t)i * 3 * H + c;
   float* vo = v_out + (size_t)i * 3 * H + c;
   *reinterpret_cast<f4*>(s_out + (size_t)i * H + c) =
