@@ -44,7 +44,7 @@ constexpr int kBwdPairs = H <= 32 ? 64 : 32;
 constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
 constexpr int kUpdBlock = 64 * kUpdWaves;
 #ifndef MLFF_MSG_UNROLL_FWD
-#define MLFF_MSG_UNROLL_FWD 3
+#define MLFF_MSG_UNROLL_FWD 5
 #endif
 #ifndef MLFF_MSG_UNROLL_BWD
 #define MLFF_MSG_UNROLL_BWD 4
