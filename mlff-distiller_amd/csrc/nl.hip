@@ -12,8 +12,8 @@
 //   fill  : per atom, neighbours in ascending j -> col; for j > i the pair
 //           (lo=i, hi=j) gets its index, lattice shift and geometry
 //           {p_hi - p_lo (+ shift @ cell), d}
-//   link  : per atom, edges with j < i find their pair in row j by binary
-//           search among row j's sorted upper neighbours.
+//   link  : per atom (one wave), edges with j < i find their pair in row j
+//           by binary search among row j's sorted upper neighbours.
 // Every write is guarded by the edge / pair capacity; a run that exceeds a
 // capacity sets counts[MLFF_COUNT_OVERFLOW] and never writes out of bounds.
 //
@@ -631,41 +631,37 @@ __global__ void nl_fill_kernel(const float* __restrict__ pos, int N,
   }
 }
 
-// One thread per edge (grid-stride over the device-side edge count).
+// One wave per row i.  Row i's lower neighbours j < i are its first
+// deg_i - udeg_i entries (rows are sorted ascending); the lanes take them in
+// parallel and find the pair (j, i) by binary search for i among row j's
+// upper neighbours, the last udeg_j entries of row j.  Edges past the edge
+// capacity were not written and are skipped; pairs past the pair capacity
+// get -1.
 __global__ void nl_link_kernel(int N, const int32_t* __restrict__ row_ptr,
                                const int32_t* __restrict__ pair_ptr,
                                const int32_t* __restrict__ col, int32_t* __restrict__ edge_pair,
-                               int32_t ecap, int32_t pcap, const int32_t* __restrict__ counts) {
-  const int E = min(counts[MLFF_COUNT_EDGES], ecap);
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-    // row of edge e: last i with row_ptr[i] <= e
-    int lo_r = 0, hi_r = N;
-    while (hi_r - lo_r > 1) {
-      const int mid = (lo_r + hi_r) >> 1;
-      if (row_ptr[mid] <= e) lo_r = mid; else hi_r = mid;
-    }
-    const int i = lo_r;
+                               int32_t ecap, int32_t pcap) {
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
+  const int r0 = row_ptr[i];
+  const int n_lower = (row_ptr[i + 1] - r0) - (pair_ptr[i + 1] - pair_ptr[i]);
+  for (int q = lane; q < n_lower; q += 64) {
+    const int e = r0 + q;
+    if (e >= ecap) break;
     const int j = col[e];
-    if (j >= i) continue;   // upper neighbours got their pair in the fill pass
-    // binary search for i among row j (sorted ascending)
-    int lo = row_ptr[j], hi = min(row_ptr[j + 1], ecap);
-    const int rstart = lo;
-    int q = -1;
+    const int end_j = row_ptr[j + 1];
+    const int up_j = end_j - (pair_ptr[j + 1] - pair_ptr[j]);   // first upper entry of row j
+    int lo = up_j, hi = min(end_j, ecap);
+    int p = -1;
     while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      int v = col[mid];
+      const int mid = (lo + hi) >> 1;
+      const int v = col[mid];
       if (v < i) lo = mid + 1;
       else if (v > i) hi = mid;
-      else { q = mid; break; }
+      else { p = pair_ptr[j] + (mid - up_j); break; }
     }
-    int p = -1;
-    if (q >= 0) {
-      const int deg_j = row_ptr[j + 1] - row_ptr[j];
-      const int udeg_j = pair_ptr[j + 1] - pair_ptr[j];
-      p = pair_ptr[j] + (q - rstart) - (deg_j - udeg_j);
-      if (p >= pcap) p = -1;
-    }
-    edge_pair[e] = p;
+    edge_pair[e] = (p < pcap) ? p : -1;
   }
 }
 
@@ -702,9 +698,8 @@ hipError_t launch_neighbor_list(const NLArgs& a, hipStream_t s, bool build_pairs
                                   build_pairs ? a.pair_lo : nullptr, a.pair_hi, a.pair_shift,
                                   a.geom, a.pcap, cl, a.edge_tmp_shift);
   if (build_pairs) {
-    const int lg = min(4096, max(1, (a.ecap + 255) / 256));
-    nl_link_kernel<<<lg, 256, 0, s>>>(a.N, a.row_ptr, a.pair_ptr, a.col, a.edge_pair, a.ecap,
-                                      a.pcap, a.counts);
+    nl_link_kernel<<<gw, kNlBlock, 0, s>>>(a.N, a.row_ptr, a.pair_ptr, a.col, a.edge_pair,
+                                            a.ecap, a.pcap);
   }
   return hipGetLastError();
 }
