@@ -272,6 +272,22 @@ def ctypes_byref(x):
     return ctypes.byref(x)
 
 
+def count_edges(positions: torch.Tensor, layout: BatchLayout, cutoff: float) -> int:
+    """Number of directed edges of the current positions (the neighbour-list
+    count and scan passes only: no edge is written; one device sync)."""
+    lib = C.lib()
+    N = layout.n_atoms
+    dev = positions.device
+    row_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+    counts = torch.zeros(C.MLFF_COUNT_SLOTS, dtype=torch.int32, device=dev)
+    ws = torch.empty(int(lib.mlff_neighbor_workspace_bytes(N)), dtype=torch.uint8, device=dev)
+    C.check("mlff_neighbor_list", lib.mlff_neighbor_list(
+        C.stream_handle(), C.ptr(positions), N, layout.n_structures, C.ptr(layout.mol_ptr),
+        C.ptr(layout.atom_mol), C.ptr(layout.cells), C.ptr(layout.pbc), float(cutoff),
+        C.ptr(row_ptr), None, None, 0, C.ptr(counts), C.ptr(ws)))
+    return int(counts[C.MLFF_COUNT_EDGES].item())
+
+
 def neighbor_list(positions: torch.Tensor, layout: BatchLayout, cutoff: float,
                   edge_capacity: Optional[int] = None, with_shifts: bool = False):
     """Receiver-CSR neighbour list on the device.

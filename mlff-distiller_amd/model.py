@@ -306,7 +306,8 @@ class StudentForceField(nn.Module):
                 entry["plan"].set_weights(self._packed)
         return self._packed
 
-    def _plan_for(self, sizes: np.ndarray, device, periodic: bool, cells, pbc) -> dict:
+    def _plan_for(self, sizes: np.ndarray, device, periodic: bool, cells, pbc,
+                  positions: Optional[torch.Tensor] = None) -> dict:
         key = (tuple(int(x) for x in sizes), periodic, str(device))
         entry = self._plans.get(key)
         if entry is not None:
@@ -336,6 +337,13 @@ class StudentForceField(nn.Module):
         weights = self.packed_weights(device)
         numbers = torch.zeros(layout.n_atoms, dtype=torch.int64, device=device)
         cap = engine.edge_capacity_bound(sizes, periodic)
+        if positions is not None:
+            # size the edge capacity from the first positions (25 % headroom)
+            # instead of the worst case, which for large structures reserves
+            # far more filter rows than exist; a later call that needs more
+            # regrows the plan (evaluate)
+            edges = engine.count_edges(positions, layout, self.cutoff)
+            cap = min(cap, max(int(edges * 1.25) + 64, 16))
         plan = engine.Plan(self.hidden_dim, self.num_interactions, self.num_rbf, self.max_z,
                            self.cutoff, layout, numbers, weights, cap)
         entry = {"layout": layout, "plan": plan, "numbers": numbers}
@@ -373,10 +381,10 @@ class StudentForceField(nn.Module):
         zmax = int(atomic_numbers.max()) if atomic_numbers.numel() else 1
         if zmin < 0 or zmax > self.max_z:
             raise ValueError(f"atomic numbers must lie in [0, {self.max_z}]")
-        entry = self._plan_for(sizes, device, periodic, cells, pbc)
+        pos = positions.detach().to(torch.float32).contiguous()
+        entry = self._plan_for(sizes, device, periodic, cells, pbc, pos)
         self.packed_weights(device)
         entry["numbers"].copy_(atomic_numbers.to(torch.int64))
-        pos = positions.detach().to(torch.float32).contiguous()
         B = len(sizes)
         N = pos.shape[0]
         for _attempt in range(2):

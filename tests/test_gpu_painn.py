@@ -525,3 +525,41 @@ def test_invariances_and_central_differences_on_the_kernels(gpu):
         step = float(pp[atom, k]) - float(pm[atom, k])
         fd = -(ef(Z, pp)[0] - ef(Z, pm)[0]) / step
         assert abs(fd - f0[atom, k]) < 5e-3 * max(1.0, abs(f0[atom, k])), (atom, k, fd, f0[atom, k])
+
+
+def test_reference_memory_and_speed_budgets(gpu):
+    """The reference's budgets (tests/unit/test_student_model.py:448-505):
+    a forward pass over 1000 random atoms within 500 MB of peak device
+    memory (measured here above what is already allocated), and under 50 ms
+    per inference of 100 atoms."""
+    import gc
+    import time
+    note("1000-atom memory budget, 100-atom inference time")
+    gc.collect()
+    torch.cuda.synchronize()
+    model = _model(128, 3, 20, seed=0)
+    g = torch.Generator().manual_seed(0)
+    Z = torch.randint(1, 10, (1000,), generator=g).to(gpu)
+    P = (torch.randn(1000, 3, generator=g) * 10.0).to(gpu)
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    with torch.no_grad():
+        E = model(Z, P)
+    torch.cuda.synchronize()
+    assert torch.isfinite(E).all()
+    peak_mb = (torch.cuda.max_memory_allocated() - base) / 2 ** 20
+    note(f"peak above baseline: {peak_mb:.1f} MB")
+    assert peak_mb < 500, peak_mb
+    Z = torch.randint(1, 10, (100,), generator=g).to(gpu)
+    P = (torch.randn(100, 3, generator=g) * 5.0).to(gpu)
+    with torch.no_grad():
+        for _ in range(10):
+            model(Z, P)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(100):
+            model(Z, P)
+        torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / 100 * 1e3
+    note(f"100 atoms: {ms:.2f} ms per inference")
+    assert ms < 50, ms
