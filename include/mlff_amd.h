@@ -117,8 +117,9 @@ int mlff_weights_layout(int hidden_dim, int num_interactions, int num_rbf, int m
  * shift_dev (may be NULL) receives the integer lattice shift of each edge:
  * edge vector p[col] - p[row] + shift @ cell.  counts_dev[MLFF_COUNT_EDGES]
  * receives the edge count; entries beyond edge_capacity are not written and
- * MLFF_COUNT_OVERFLOW is set.  workspace_dev must hold
- * mlff_neighbor_workspace_bytes(n_atoms) bytes.
+ * MLFF_COUNT_OVERFLOW is set; edge_capacity = 0 with col_dev = NULL only
+ * counts.  workspace_dev must hold mlff_neighbor_workspace_bytes(n_atoms)
+ * bytes; its contents are scratch.
  * ------------------------------------------------------------------------- */
 int64_t mlff_neighbor_workspace_bytes(int n_atoms);
 int mlff_neighbor_list(void* stream, const float* pos_dev, int n_atoms, int n_structures,
