@@ -108,6 +108,28 @@ def test_neighbor_list_many_slices(gpu):
     np.testing.assert_array_equal(col.cpu().numpy(), cl)
 
 
+def test_neighbor_list_count_only(gpu):
+    """The count-only call (capacity 0, no edge buffers) that sizes new plans
+    reports the oracle's edge count, for molecules and a periodic box."""
+    note("NL count-only mode")
+    engine = pkg_mod("engine")
+    synth = pkg_mod("synth")
+    mols = _mols(40, 10, 100, seed=17)
+    Z, P, mol_ptr = synth.flatten(mols)
+    pos32 = P.astype(np.float32)
+    lay = engine.make_layout(np.diff(mol_ptr), torch.device(gpu))
+    n = engine.count_edges(torch.tensor(pos32, device=gpu), lay, 5.0)
+    rp, _, _ = nl_ref.neighbor_csr(pos32, mol_ptr, 5.0)
+    assert n == rp[-1]
+    box = synth.water_box(6, seed=3)
+    bpos = box.positions.astype(np.float32)
+    blay = engine.make_layout([len(bpos)], torch.device(gpu), box.cell[None], box.pbc[None])
+    nb = engine.count_edges(torch.tensor(bpos, device=gpu), blay, 5.0)
+    rpb, _, _ = nl_ref.neighbor_csr(bpos, np.array([0, len(bpos)]), 5.0, box.cell[None],
+                                    box.pbc[None])
+    assert nb == rpb[-1]
+
+
 def test_neighbor_list_periodic_water(gpu):
     note("NL periodic water box 1000 molecules")
     synth = pkg_mod("synth")
