@@ -179,9 +179,12 @@ class MTSNVE:
     def _make_plans(self, edges: float, step: int = 1):
         bound = engine.edge_capacity_bound(self.sizes, False)
         cap = int(min(bound, int(edges) + 64))
+        # one stream per plan: an outer step runs five evaluations, and their
+        # fork/join events cost more than the overlap of the filter kernels
+        # gains (measured 3.01 against 3.07 ms per outer step on 512 molecules)
         mk = lambda m: engine.Plan(m.hidden_dim, m.num_interactions, m.num_rbf, m.max_z,
                                    m.cutoff, self.layout, self.numbers,
-                                   m.packed_weights(self.device), cap)
+                                   m.packed_weights(self.device), cap, streams=1)
         self.plan_full, self.plan_fast = mk(self.full), mk(self.fast)
         C.check("mlff_mts_bind", self.lib.mlff_mts_bind(
             self.plan_full.handle, self.plan_fast.handle, C.ptr(self.x), C.ptr(self.p),
