@@ -255,7 +255,8 @@ def config3(args, pkg, synth, mdmod, dev):
     model = pkg.StudentForceField().to(dev).eval()
     box = synth.water_box(10, seed=3)
     steps = args.steps if args.steps != 500 else 20000
-    md = mdmod.BatchedNVE(model, [box], seed=3, max_steps=steps + args.warmup + 1)
+    md = mdmod.BatchedNVE(model, [box], seed=3, max_steps=steps + args.warmup + 1,
+                          streams=args.streams)
     sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
 
     def accuracy_at(P32):
