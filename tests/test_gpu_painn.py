@@ -585,3 +585,26 @@ def test_reference_memory_and_speed_budgets(gpu):
     ms = (time.perf_counter() - t) / 100 * 1e3
     note(f"100 atoms: {ms:.2f} ms per inference")
     assert ms < 50, ms
+
+
+def test_coincident_atoms_follow_the_reference(gpu):
+    """Two atoms at the same position: the pair is inside the cutoff (d = 0)
+    and the reference's direction u = vec / (d + 1e-8) makes its forces huge
+    (1e6 eV/A) but finite.  The kernels reproduce them to float32 relative
+    precision, and the other atoms' forces to the usual absolute bar."""
+    note("coincident atoms (d = 0 pair)")
+    model = _model(128, 3, 20, seed=0)
+    Z = np.array([6, 1, 8, 1])
+    P = np.array([[0., 0., 0.], [0., 0., 0.], [1.2, 0.1, -0.3], [2.0, 1.0, 0.5]])
+    e64, f, _ = model.evaluate(torch.tensor(Z, device=gpu),
+                               torch.tensor(P.astype(np.float32), device=gpu), np.array([4]))
+    sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    E, F = R.energy_and_forces(sd, Z, P.astype(np.float32).astype(np.float64))
+    F = F.numpy()
+    f = f.cpu().numpy()
+    assert np.isfinite(f).all()
+    assert abs(e64.item() - E.item()) / 4 < E_TOL
+    big = np.abs(F[:2]).max()
+    assert big > 1e4
+    assert np.abs(f[:2] - F[:2]).max() < 1e-5 * big
+    assert np.abs(f[2:] - F[2:]).max() < F_TOL
