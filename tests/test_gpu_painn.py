@@ -237,9 +237,14 @@ def _check_ef(model, Z, P32, sizes, E, F, label, absolute=True):
     assert f_err < f_tol, label
 
 
-@pytest.mark.parametrize("H,L,R_", [(128, 3, 20), (64, 2, 20), (32, 2, 16)])
+@pytest.mark.parametrize("H,L,R_", [(128, 3, 20), (64, 2, 20), (32, 2, 16), (128, 1, 32),
+                                    (64, 8, 1)])
 def test_energy_forces_vs_fp64_oracle(gpu, H, L, R_):
-    """Seed-0 random initialisation (zero biases), as the benchmarks use."""
+    """Seed-0 random initialisation (zero biases), as the benchmarks use; the
+    supported range's ends too (one block and 32 basis functions; eight
+    blocks and one basis function).  Eight random blocks give forces of order
+    1e4 eV/A, beyond what float32 resolves to 1e-4: that case takes the
+    relative bar (three times the reference algorithm's own float32 error)."""
     note(f"E+F H{H} L{L} R{R_}")
     model = _model(H, L, R_, seed=0)
     mols = _mols(12, 10, 100, seed=21)
@@ -248,7 +253,8 @@ def test_energy_forces_vs_fp64_oracle(gpu, H, L, R_):
     sizes = np.diff(mol_ptr)
     P32 = P.astype(np.float32)
     e64, f, _ = model.evaluate(torch.tensor(Z, device=gpu), torch.tensor(P32, device=gpu), sizes)
-    _check_ef(model, Z, P32, sizes, e64.cpu().numpy(), f.cpu().numpy(), f"H{H}")
+    _check_ef(model, Z, P32, sizes, e64.cpu().numpy(), f.cpu().numpy(), f"H{H} L{L} R{R_}",
+              absolute=L <= 3)
 
 
 @pytest.mark.parametrize("H,L,R_", [(128, 3, 20), (32, 2, 16)])
