@@ -381,6 +381,14 @@ class StudentForceField(nn.Module):
         zmax = int(atomic_numbers.max()) if atomic_numbers.numel() else 1
         if zmin < 0 or zmax > self.max_z:
             raise ValueError(f"atomic numbers must lie in [0, {self.max_z}]")
+        if positions.shape[0] == 0:
+            # no atoms: every sum is empty, as in the reference's forward
+            # (student_model.py:734-757), so no plan is needed
+            B = max(len(sizes), 1)
+            return (torch.zeros(B, dtype=torch.float64, device=device),
+                    torch.zeros(0, 3, dtype=torch.float32, device=device) if need_forces else None,
+                    torch.zeros(B, 3, 3, dtype=torch.float64, device=device)
+                    if need_virial else None)
         pos = positions.detach().to(torch.float32).contiguous()
         entry = self._plan_for(sizes, device, periodic, cells, pbc, pos)
         self.packed_weights(device)
@@ -421,6 +429,10 @@ class StudentForceField(nn.Module):
                 cell: Optional[torch.Tensor] = None, pbc: Optional[torch.Tensor] = None,
                 batch: Optional[torch.Tensor] = None) -> torch.Tensor:
         n = atomic_numbers.shape[0]
+        if n == 0:
+            # no atoms: the reference's sums over atoms are empty and give 0
+            # (student_model.py:734-757); differentiable in the positions
+            return positions.sum() * 0.0
         sizes, order, bnp, multi = self._batch_sizes(batch, n)
         cells_np, pbc_np = self._cell_args(cell, pbc, len(sizes))
         Z = atomic_numbers

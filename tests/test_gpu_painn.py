@@ -614,3 +614,22 @@ def test_coincident_atoms_follow_the_reference(gpu):
     assert big > 1e4
     assert np.abs(f[:2] - F[:2]).max() < 1e-5 * big
     assert np.abs(f[2:] - F[2:]).max() < F_TOL
+
+
+def test_empty_structure_gives_zero_energy(gpu):
+    """No atoms: the reference's forward sums over nothing and returns 0
+    (student_model.py:734-757); forces are an empty [0, 3] array, also
+    through autograd in training mode."""
+    note("empty structure")
+    model = _model(32, 2, 16, seed=0).eval()
+    Z = torch.zeros(0, dtype=torch.int64, device=gpu)
+    P = torch.zeros(0, 3, dtype=torch.float32, device=gpu)
+    with torch.no_grad():
+        assert model(Z, P).item() == 0.0
+    E, F = model.predict_energy_and_forces(Z, P)
+    assert E.item() == 0.0 and tuple(F.shape) == (0, 3)
+    model.train()
+    P2 = P.clone().requires_grad_(True)
+    E2 = model(Z, P2)
+    g = torch.autograd.grad(E2, P2)[0]
+    assert E2.item() == 0.0 and tuple(g.shape) == (0, 3)
