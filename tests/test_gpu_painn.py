@@ -633,3 +633,23 @@ def test_empty_structure_gives_zero_energy(gpu):
     E2 = model(Z, P2)
     g = torch.autograd.grad(E2, P2)[0]
     assert E2.item() == 0.0 and tuple(g.shape) == (0, 3)
+
+
+def test_batch_with_an_empty_structure(gpu):
+    """A batch index that skips a structure (batch = [0, 0, 0, 2, 2, 2, 2]):
+    the reference's per-structure loop (student_model.py:734-757) gives the
+    skipped structure energy 0; the others equal their own evaluations."""
+    note("batch with an empty structure")
+    model = _model(64, 2, 20, seed=0).eval()
+    Z = torch.tensor([6, 1, 1, 8, 1, 1, 6], device=gpu)
+    P = torch.tensor([[0.0, 0.0, 0.0], [1.1, 0.0, 0.0], [-0.4, 1.0, 0.0],
+                      [10.0, 0.0, 0.0], [10.9, 0.3, 0.0], [9.6, 0.8, 0.2], [11.5, -1.2, 0.4]],
+                     dtype=torch.float32, device=gpu)
+    batch = torch.tensor([0, 0, 0, 2, 2, 2, 2], device=gpu)
+    with torch.no_grad():
+        E = model(Z, P, batch=batch)
+        E0 = model(Z[:3], P[:3])
+        E2 = model(Z[3:], P[3:])
+    assert tuple(E.shape) == (3,)
+    assert E[1].item() == 0.0
+    assert abs(E[0].item() - E0.item()) < 1e-6 and abs(E[2].item() - E2.item()) < 1e-6
