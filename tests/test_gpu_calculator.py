@@ -137,3 +137,20 @@ def test_torchscript_export_runs_on_the_kernels(gpu, ckpt, tmp_path):
     calc = pkg().StudentForceFieldCalculator(None, device="cuda", use_jit=True, jit_path=out)
     atoms.calc = calc
     assert np.array_equal(atoms.get_forces(), F2.cpu().numpy().astype(np.float64))
+
+
+def test_reset_and_repr(gpu, ckpt):
+    """The reference's calculator-state tests
+    (tests/integration/test_ase_calculator.py:314-334): after reset() the
+    next call computes again, and repr names the class and the checkpoint."""
+    note("calculator reset and repr")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    atoms = _mol(12, 5)
+    atoms.calc = calc
+    e1 = atoms.get_potential_energy()
+    calc.reset()
+    e2 = atoms.get_potential_energy()
+    assert np.isfinite(e2) and e1 == e2 and calc.n_calls == 2
+    r = repr(calc)
+    assert "StudentForceFieldCalculator" in r and path.name in r
