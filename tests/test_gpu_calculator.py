@@ -154,3 +154,22 @@ def test_reset_and_repr(gpu, ckpt):
     assert np.isfinite(e2) and e1 == e2 and calc.n_calls == 2
     r = repr(calc)
     assert "StudentForceFieldCalculator" in r and path.name in r
+
+
+def test_steepest_descent_lowers_the_energy(gpu, ckpt):
+    """The reference's geometry-optimisation check
+    (tests/integration/test_ase_calculator.py:158-175, BFGS through ASE,
+    which is absent here) as plain steepest descent: small steps along the
+    calculator's forces lower the energy at every step."""
+    note("steepest descent with the calculator's forces")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    atoms = _mol(16, 9)
+    atoms.calc = calc
+    energies = [atoms.get_potential_energy()]
+    for _ in range(10):
+        f = atoms.get_forces()
+        step = 0.01 / max(np.abs(f).max(), 1e-12)     # largest displacement 0.01 A
+        atoms.positions = atoms.positions + step * f
+        energies.append(atoms.get_potential_energy())
+    assert np.all(np.diff(energies) < 0), energies
