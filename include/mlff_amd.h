@@ -222,6 +222,15 @@ int mlff_plan_set_streams(mlff_plan* plan, int n_streams);
 int mlff_md_bind(mlff_plan* plan, double* x_dev, double* p_dev, const double* inv_mass_dev,
                  float* pos32_dev, float* forces_dev, double* energy_dev,
                  double* ledger_dev, int32_t ledger_len, double dt);
+/* Per-step frames, the positions, velocities and forces the reference's
+ * NVEMDHarness._collect_data copies through the host every step
+ * (nve_harness.py:263-287).  With n_frames > 0, every step also stores x and
+ * p (float64) and F (float32), each [N, 3], into row step % n_frames of
+ * x_frames / p_frames / f_frames ([n_frames, N, 3]); the row index is the
+ * ledger's step counter, so graph replays record too.  n_frames = 0 unbinds.
+ * Needs mlff_md_bind first; a later mlff_md_bind keeps the binding. */
+int mlff_md_bind_frames(mlff_plan* plan, double* x_frames_dev, double* p_frames_dev,
+                        float* f_frames_dev, int32_t n_frames);
 /* Forces at the current positions (call once before the first step). */
 int mlff_md_init_forces(mlff_plan* plan, void* stream);
 /* Enqueue n_steps steps (eager launches). */

@@ -38,8 +38,8 @@ EXPORTS = (
     "mlff_version", "mlff_last_error", "mlff_weights_layout", "mlff_neighbor_workspace_bytes",
     "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
-    "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run", "mlff_launch_stage",
-    "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
+    "mlff_md_bind_frames", "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
+    "mlff_launch_stage", "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
     "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams",
 )
 
@@ -106,6 +106,8 @@ def _declare(lib):
     lib.mlff_energy_forces.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.mlff_md_bind.restype = ctypes.c_int
     lib.mlff_md_bind.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_double]
+    lib.mlff_md_bind_frames.restype = ctypes.c_int
+    lib.mlff_md_bind_frames.argtypes = [vp, vp, vp, vp, i32]
     lib.mlff_md_init_forces.restype = ctypes.c_int
     lib.mlff_md_init_forces.argtypes = [vp, vp]
     lib.mlff_md_run.restype = ctypes.c_int

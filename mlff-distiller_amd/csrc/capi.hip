@@ -105,6 +105,8 @@ struct mlff_plan {
   bool md_bound;
   double* x; double* p; const double* inv_mass; float* pos32; float* forces; double* energy;
   double* ledger; int ledger_len; double dt;
+  double* frame_x = nullptr; double* frame_p = nullptr; float* frame_f = nullptr;
+  int n_frames = 0;
   hipGraph_t graph; hipGraphExec_t graph_exec; int graph_steps; hipStream_t graph_stream;
   int4* grid_dev;
   // multiple time step (this plan is the outer one)
@@ -529,6 +531,22 @@ int mlff_md_bind(mlff_plan* pl, double* x_dev, double* p_dev, const double* inv_
   return MLFF_OK;
 }
 
+int mlff_md_bind_frames(mlff_plan* pl, double* x_frames_dev, double* p_frames_dev,
+                        float* f_frames_dev, int32_t n_frames) {
+  if (!pl || !pl->md_bound) { set_error("mlff_md_bind_frames: MD not bound"); return MLFF_ERR_STATE; }
+  if (n_frames < 0 || (n_frames > 0 && (!x_frames_dev || !p_frames_dev || !f_frames_dev))) {
+    set_error("mlff_md_bind_frames: invalid argument");
+    return MLFF_ERR_INVALID;
+  }
+  pl->frame_x = n_frames > 0 ? x_frames_dev : nullptr;
+  pl->frame_p = n_frames > 0 ? p_frames_dev : nullptr;
+  pl->frame_f = n_frames > 0 ? f_frames_dev : nullptr;
+  pl->n_frames = n_frames;
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
 int mlff_md_init_forces(mlff_plan* pl, void* stream) {
   if (!pl || !pl->md_bound) { set_error("mlff_md_init_forces: MD not bound"); return MLFF_ERR_STATE; }
   return run_ef(pl, static_cast<hipStream_t>(stream), pl->pos32, pl->energy, pl->forces, nullptr,
@@ -542,7 +560,8 @@ static int md_step(mlff_plan* pl, hipStream_t s) {
   int rc = run_ef(pl, s, pl->pos32, pl->energy, pl->forces, nullptr, nullptr);
   if (rc != MLFF_OK) return rc;
   e = launch_vv_second(b.N, b.B, b.mol_ptr, pl->p, pl->inv_mass, pl->forces, pl->dt, pl->energy,
-                       pl->ledger, pl->ledger_len, b.counts, s);
+                       pl->ledger, pl->ledger_len, b.counts, pl->x, pl->frame_x, pl->frame_p,
+                       pl->frame_f, pl->n_frames, s);
   if (e != hipSuccess) return hip_fail(e, "vv second half");
   return MLFF_OK;
 }

@@ -99,7 +99,8 @@ hipError_t launch_vv_first(int N, double* x, double* p, const double* inv_mass, 
 hipError_t launch_vv_second(int N, int B, const int32_t* mol_ptr, double* p,
                             const double* inv_mass, const float* F, double dt,
                             const double* epot, double* ledger, int ledger_len,
-                            int32_t* counts, hipStream_t s);
+                            int32_t* counts, const double* x, double* fx, double* fp, float* fF,
+                            int n_frames, hipStream_t s);
 
 hipError_t launch_mts_kick(int N, double* p, const float* Fa, const float* Fb, double coef,
                            hipStream_t s);

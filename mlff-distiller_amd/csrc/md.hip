@@ -4,7 +4,10 @@
 // State (x, p) is float64 like ASE's; the model reads a float32 copy of x.
 // The second half-kick also sums the kinetic energy of each structure in
 // FP64 (sum p^2 / 2m, as Atoms.get_kinetic_energy) and appends
-// (E_pot, E_kin) to the ledger row of the current step.
+// (E_pot, E_kin) to the ledger row of the current step.  When frame buffers
+// are bound it also stores (x, p, F) of the step in the frame row of the
+// same step counter, the per-step data the reference harness copies through
+// the host (nve_harness.py:263-287).
 
 #include "common.hpp"
 #include "internal.hpp"
@@ -33,11 +36,15 @@ __global__ __launch_bounds__(256) void vv_second_kernel(
     int B, const int32_t* __restrict__ mol_ptr, double* __restrict__ p,
     const double* __restrict__ inv_mass, const float* __restrict__ F, double dt,
     const double* __restrict__ epot, double* __restrict__ ledger, int ledger_len,
-    const int32_t* __restrict__ counts) {
+    const int32_t* __restrict__ counts, int N, const double* __restrict__ x,
+    double* __restrict__ fx, double* __restrict__ fp, float* __restrict__ fF, int n_frames) {
   __shared__ double sh[256];
   const int b = blockIdx.x;
   if (b >= B) return;
   const int i0 = mol_ptr[b], i1 = mol_ptr[b + 1];
+  // frame row of this step (all three frame buffers are bound together)
+  const size_t fbase = n_frames > 0
+      ? (size_t)(counts[MLFF_COUNT_STEP] % n_frames) * (size_t)N * 3 : 0;
   double ke = 0.0;
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     double s = 0.0;
@@ -46,6 +53,11 @@ __global__ __launch_bounds__(256) void vv_second_kernel(
       const double pa = p[3 * i + a] + 0.5 * dt * (double)F[3 * i + a];
       p[3 * i + a] = pa;
       s += pa * pa;
+      if (n_frames > 0) {
+        fx[fbase + 3 * i + a] = x[3 * i + a];
+        fp[fbase + 3 * i + a] = pa;
+        fF[fbase + 3 * i + a] = F[3 * i + a];
+      }
     }
     ke += 0.5 * s * inv_mass[i];
   }
@@ -171,11 +183,11 @@ hipError_t launch_vv_first(int N, double* x, double* p, const double* inv_mass, 
 hipError_t launch_vv_second(int N, int B, const int32_t* mol_ptr, double* p,
                             const double* inv_mass, const float* F, double dt,
                             const double* epot, double* ledger, int ledger_len,
-                            int32_t* counts, hipStream_t s) {
-  (void)N;
+                            int32_t* counts, const double* x, double* fx, double* fp, float* fF,
+                            int n_frames, hipStream_t s) {
   if (B > 0)
     vv_second_kernel<<<B, 256, 0, s>>>(B, mol_ptr, p, inv_mass, F, dt, epot, ledger, ledger_len,
-                                       counts);
+                                       counts, N, x, fx, fp, fF, n_frames);
   step_advance_kernel<<<1, 64, 0, s>>>(counts);
   return hipGetLastError();
 }

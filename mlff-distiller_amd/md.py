@@ -31,7 +31,8 @@ class BatchedNVE:
     def __init__(self, model, structures: Sequence["synth.Structure"], timestep_fs: float = 0.5,
                  temperature_K: float = 300.0, seed: int = 0, momenta: Optional[np.ndarray] = None,
                  max_steps: int = 100000, headroom: float = 1.3, device="cuda:0",
-                 persistent_blocks: int = 0, streams: Optional[int] = None):
+                 persistent_blocks: int = 0, streams: Optional[int] = None,
+                 record_frames: bool = False):
         self.model = model
         self.device = torch.device(device)
         self.structures = list(structures)
@@ -66,6 +67,13 @@ class BatchedNVE:
         self.energy = torch.zeros(self.B, dtype=torch.float64, device=dev)
         self.max_steps = int(max_steps)
         self.ledger = torch.zeros(self.max_steps + 1, self.B, 2, dtype=torch.float64, device=dev)
+        # per-step frames (x, p, F), written by the second half-kick on the device
+        self.frame_x = self.frame_p = self.frame_f = None
+        if record_frames:
+            shape = (self.max_steps + 1, self.n_atoms, 3)
+            self.frame_x = torch.zeros(shape, dtype=torch.float64, device=dev)
+            self.frame_p = torch.zeros(shape, dtype=torch.float64, device=dev)
+            self.frame_f = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.layout = engine.make_layout(self.sizes, dev, cells, pbc)
         self.weights = model.packed_weights(dev)
         self._headroom = headroom
@@ -93,6 +101,10 @@ class BatchedNVE:
             0, self.layout.atom_mol.long(), ke0)
         self.ledger[0, :, 0] = self.energy
         self.ledger[0, :, 1] = ke0
+        if self.frame_x is not None:
+            self.frame_x[0] = self.x
+            self.frame_p[0] = self.p
+            self.frame_f[0] = self.forces
         self.tuning = None
 
     def autotune(self, **kw):
@@ -124,6 +136,10 @@ class BatchedNVE:
             self.plan.handle, C.ptr(self.x), C.ptr(self.p), C.ptr(self.inv_mass),
             C.ptr(self.pos32), C.ptr(self.forces), C.ptr(self.energy), C.ptr(self.ledger),
             self.max_steps + 1, float(self.dt)))
+        if self.frame_x is not None:
+            C.check("mlff_md_bind_frames", self.lib.mlff_md_bind_frames(
+                self.plan.handle, C.ptr(self.frame_x), C.ptr(self.frame_p),
+                C.ptr(self.frame_f), self.max_steps + 1))
         # the ledger row index is the device step counter (+1: row 0 holds t = 0)
         self.plan.counts[C.MLFF_COUNT_STEP] = 1
 
@@ -179,6 +195,17 @@ class BatchedNVE:
         led = self.ledger[:self.steps_done + 1].cpu().numpy()
         return led[..., 0], led[..., 1]
 
+    def frames(self):
+        """Positions, velocities (ASE units) and forces of every step so far,
+        each [steps+1, N, 3] float64 on the host; row 0 is t = 0.  Needs
+        record_frames=True."""
+        if self.frame_x is None:
+            raise RuntimeError("frames were not recorded (record_frames=False)")
+        n = self.steps_done + 1
+        v = self.frame_p[:n] * self.inv_mass[None, :, None]
+        return (self.frame_x[:n].cpu().numpy(), v.cpu().numpy(),
+                self.frame_f[:n].double().cpu().numpy())
+
     def summary(self):
         epot, ekin = self.energies()
         etot = epot + ekin
@@ -208,12 +235,47 @@ class BatchedNVE:
         return (self.p * self.inv_mass[:, None]).cpu().numpy()
 
 
+# element symbols by atomic number (index 0 unused), for extended-XYZ output
+_SYMBOLS = ("X H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu "
+            "Zn Ga Ge As Se Br Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La "
+            "Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At "
+            "Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm Md No Lr Rf Db Sg Bh Hs Mt Ds Rg Cn Nh "
+            "Fl Mc Lv Ts Og").split()
+
+
+def write_extxyz(path, numbers, positions, velocities, forces, info, cell=None, pbc=None):
+    """Extended-XYZ frames (the format ase.io.write(format='extxyz') reads
+    back): per frame the atom count, a comment line of key=value pairs
+    (Lattice/pbc when periodic, then `info`'s scalars), then one line per atom
+    with symbol, position, velocity and force.  `info` maps a key to one
+    value per frame."""
+    n_frames = len(positions)
+    with open(path, "w") as fh:
+        for t in range(n_frames):
+            head = ['Properties=species:S:1:pos:R:3:vel:R:3:forces:R:3']
+            if cell is not None and pbc is not None and np.any(pbc):
+                head.insert(0, 'Lattice="%s"' % " ".join(f"{v:.10f}" for v in
+                                                         np.asarray(cell).reshape(-1)))
+                head.insert(2, 'pbc="%s"' % " ".join("T" if b else "F" for b in pbc))
+            head += [f"{k}={float(v[t]):.12g}" for k, v in info.items()]
+            fh.write(f"{len(numbers)}\n{' '.join(head)}\n")
+            for z, x, v, f in zip(numbers, positions[t], velocities[t], forces[t]):
+                fh.write(f"{_SYMBOLS[int(z)]:<2} " + " ".join(f"{c:.10f}" for c in x) + " "
+                         + " ".join(f"{c:.10f}" for c in v) + " "
+                         + " ".join(f"{c:.10f}" for c in f) + "\n")
+
+
 class NVEMDHarness:
     """The reference's NVE harness API (nve_harness.py:58-446) on the device
-    integrator: same constructor, initialize_velocities, run_simulation and
-    summary keys.  Energies/temperatures are recorded every step (from the
-    device ledger); positions, velocities and forces every save_interval
-    steps (the reference copies them every step through the host)."""
+    integrator: same constructor, initialize_velocities, run_simulation,
+    summary keys, reset, is_complete, n_atoms, get_trajectory_array and
+    save_trajectory.  Every step's time, positions, velocities, forces and
+    energies are collected, as the reference's _collect_data(interval=1)
+    does (nve_harness.py:228-231, 263-287); here the second half-kick writes
+    them into device frame buffers inside the replayed HIP graph and they
+    cross to the host once per run.  Runs without reset accumulate, each
+    starting its time at 0 (a new dynamics object per run in the
+    reference)."""
 
     def __init__(self, atoms, calculator, temperature: float = 300.0, timestep: float = 0.5,
                  trajectory_file=None, log_interval: int = 10, remove_com_motion: bool = True,
@@ -257,8 +319,21 @@ class NVEMDHarness:
         self._momenta = synth.maxwell_boltzmann(s.numbers, s.positions, T, self._rng,
                                                 remove_com_translation, remove_com_rotation)
 
+    @property
+    def is_complete(self) -> bool:
+        """nve_harness.py:379-382."""
+        return self._simulation_complete
+
+    @property
+    def n_atoms(self) -> int:
+        """nve_harness.py:384-387."""
+        return len(self.atoms)
+
     def run_simulation(self, steps: int, initialize_velocities: bool = True,
                        save_interval: Optional[int] = None):
+        """nve_harness.py:173-261.  save_interval (default log_interval)
+        sets how often trajectory_file gets a frame, as the reference's
+        attached Trajectory writer; the in-memory data holds every step."""
         import time as _time
         if steps <= 0:
             raise ValueError(f"steps must be positive, got {steps}")
@@ -269,24 +344,28 @@ class NVEMDHarness:
                 self.initialize_velocities()
             md = BatchedNVE(self.calculator.model, [self._structure()],
                             timestep_fs=self.timestep_fs, momenta=self._momenta,
-                            max_steps=steps, device=self.calculator.device)
-            self._record(md)
-            done = 0
-            while done < steps:
-                k = min(save_interval, steps - done)
-                md.run(k, graph=k >= 10, steps_per_graph=10 if k >= 10 else 1)
-                done += k
-                self._record(md)
+                            max_steps=steps, device=self.calculator.device,
+                            record_frames=True)
+            md.run(steps, graph=steps >= 10, steps_per_graph=10)
             epot, ekin = md.energies()
+            x, v, f = md.frames()
             n = md.n_atoms
             td = self.trajectory_data
-            td["time"] = list(np.arange(steps + 1) * self.timestep_fs / 1000.0)
-            td["potential_energy"] = list(epot[:, 0])
-            td["kinetic_energy"] = list(ekin[:, 0])
-            td["total_energy"] = list(epot[:, 0] + ekin[:, 0])
-            td["temperature"] = list(2 * ekin[:, 0] / (3 * n * synth.KB_EV))
-            self.atoms.set_positions(md.positions()) if hasattr(self.atoms, "set_positions") else None
+            td["time"] += list(np.arange(steps + 1) * self.timestep_fs / 1000.0)
+            td["positions"] += list(x)
+            td["velocities"] += list(v)
+            td["forces"] += list(f)
+            td["potential_energy"] += list(epot[:, 0])
+            td["kinetic_energy"] += list(ekin[:, 0])
+            td["total_energy"] += list(epot[:, 0] + ekin[:, 0])
+            td["temperature"] += list(2 * ekin[:, 0] / (3 * n * synth.KB_EV))
+            if hasattr(self.atoms, "set_positions"):
+                self.atoms.set_positions(x[-1])
             self._momenta = md.p.cpu().numpy()
+            if self.trajectory_file:
+                rows = np.arange(0, steps + 1, save_interval)
+                k0 = len(td["time"]) - (steps + 1)
+                self._write(self.trajectory_file, k0 + rows, None)
             self._n_steps = steps
             self._wall_time = _time.perf_counter() - t0
             self._simulation_complete = True
@@ -294,11 +373,6 @@ class NVEMDHarness:
         except Exception as e:
             raise RuntimeError(f"NVE MD simulation failed after "
                                f"{len(self.trajectory_data['time'])} steps: {e}") from e
-
-    def _record(self, md):
-        self.trajectory_data["positions"].append(md.positions())
-        self.trajectory_data["velocities"].append(md.velocities())
-        self.trajectory_data["forces"].append(md.forces.cpu().numpy().astype(np.float64))
 
     def _compute_summary(self):
         """nve_harness.py:311-360."""
@@ -322,7 +396,56 @@ class NVEMDHarness:
         }
 
     def reset(self):
+        """nve_harness.py:362-377."""
         self.trajectory_data = {k: [] for k in self.trajectory_data}
         self._simulation_complete = False
         self._n_steps = 0
         self._wall_time = 0.0
+
+    def get_trajectory_array(self, key: str) -> np.ndarray:
+        """nve_harness.py:389-406."""
+        if key not in self.trajectory_data:
+            raise KeyError(f"Unknown trajectory key: {key}")
+        data = self.trajectory_data[key]
+        if not data:
+            raise RuntimeError("No trajectory data collected")
+        return np.array(data)
+
+    def save_trajectory(self, filename, format: str = "traj"):
+        """nve_harness.py:408-443: every collected frame with its time and
+        energies.  Written through ase.io when ASE is installed; without it
+        'xyz' and 'extxyz' are written as extended XYZ and other formats
+        raise ValueError."""
+        import warnings
+        if not self._simulation_complete:
+            warnings.warn("Simulation not complete, saving partial trajectory")
+        if not self.trajectory_data["time"]:
+            raise RuntimeError("No trajectory data to save")
+        self._write(filename, np.arange(len(self.trajectory_data["time"])), format)
+
+    def _write(self, filename, rows, format):
+        td = self.trajectory_data
+        keys = ("time", "potential_energy", "kinetic_energy", "total_energy", "temperature")
+        try:
+            from ase.io import write as ase_write
+        except ImportError:
+            ase_write = None
+        if ase_write is not None:
+            frames = []
+            for i in rows:
+                a = self.atoms.copy()
+                a.set_positions(td["positions"][i])
+                a.set_velocities(td["velocities"][i])
+                for k in keys:
+                    a.info[k] = td[k][i]
+                a.arrays["forces"] = td["forces"][i]
+                frames.append(a)
+            ase_write(str(filename), frames, format=format)
+            return
+        fmt = format or str(filename).rsplit(".", 1)[-1]
+        if fmt not in ("xyz", "extxyz"):
+            raise ValueError(f"format {fmt!r} needs ASE (not installed); use 'xyz' or 'extxyz'")
+        s = self._structure()
+        write_extxyz(filename, s.numbers, [td["positions"][i] for i in rows],
+                     [td["velocities"][i] for i in rows], [td["forces"][i] for i in rows],
+                     {k: [td[k][i] for i in rows] for k in keys}, s.cell, s.pbc)

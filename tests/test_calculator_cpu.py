@@ -74,3 +74,53 @@ def test_cells_narrower_than_twice_cutoff_are_rejected():
     with pytest.raises(ValueError, match="half the periodic cell"):
         engine.check_cells(np.eye(3) * 9.0, np.ones(3, bool), 5.0)
     engine.check_cells(np.eye(3) * 10.5, np.ones(3, bool), 5.0)
+
+
+class _CalcStub:
+    """Just what the harness constructor reads: a calculator with a model."""
+    model = None
+    device = "cpu"
+
+
+def test_harness_state_before_any_run():
+    """nve_harness tests: test_initialization, test_get_trajectory_array_invalid_key,
+    test_get_trajectory_array_before_simulation; save before a run raises too."""
+    from atoms_standin import Atoms
+    h = pkg().NVEMDHarness(Atoms([8, 1, 1], np.eye(3)), _CalcStub(), temperature=300.0,
+                           timestep=0.5)
+    assert h.n_atoms == 3 and not h.is_complete
+    assert h.temperature == 300.0 and h.timestep_fs == 0.5
+    assert len(h.trajectory_data["time"]) == 0
+    with pytest.raises(KeyError, match="Unknown trajectory key"):
+        h.get_trajectory_array("nope")
+    with pytest.raises(RuntimeError, match="No trajectory data collected"):
+        h.get_trajectory_array("positions")
+    with pytest.warns(UserWarning):
+        with pytest.raises(RuntimeError, match="No trajectory data to save"):
+            h.save_trajectory("unused.xyz", format="xyz")
+
+
+def test_extxyz_writer_reads_back(tmp_path):
+    """The extended-XYZ writer, parsed back by hand: species, positions,
+    velocities, forces and per-frame info for two frames of a periodic cell."""
+    md = pkg_mod("md")
+    rng = np.random.default_rng(0)
+    z = np.array([8, 1, 1, 79])
+    x, v, f = (rng.normal(size=(2, 4, 3)) for _ in range(3))
+    info = {"time": [0.0, 0.0005], "total_energy": [-1.25, -1.5]}
+    cell = np.diag([10.0, 11.0, 12.0])
+    path = tmp_path / "t.xyz"
+    md.write_extxyz(path, z, x, v, f, info, cell, np.array([True, True, False]))
+    lines = path.read_text().splitlines()
+    assert len(lines) == 2 * (2 + 4)
+    for t in range(2):
+        head = lines[t * 6 + 1]
+        assert int(lines[t * 6]) == 4
+        assert 'pbc="T T F"' in head and "Lattice=" in head
+        assert f"total_energy={info['total_energy'][t]:.12g}" in head
+        rows = [ln.split() for ln in lines[t * 6 + 2:t * 6 + 6]]
+        assert [r[0] for r in rows] == ["O", "H", "H", "Au"]
+        vals = np.array([[float(c) for c in r[1:]] for r in rows])
+        np.testing.assert_allclose(vals[:, 0:3], x[t], atol=1e-9)
+        np.testing.assert_allclose(vals[:, 3:6], v[t], atol=1e-9)
+        np.testing.assert_allclose(vals[:, 6:9], f[t], atol=1e-9)

@@ -106,10 +106,61 @@ def test_nve_harness_api(gpu, ckpt):
               "initial_energy", "final_energy", "trajectory_data", "n_steps"):
         assert k in out
     td = out["trajectory_data"]
-    assert len(td["total_energy"]) == 41 and len(td["positions"]) == 5
+    assert len(td["total_energy"]) == 41 and len(td["positions"]) == 41
     assert abs(out["energy_drift_abs"]) / 20 < 1e-3
     with pytest.raises(ValueError):
         h.run_simulation(0)
+
+
+def test_nve_harness_collects_every_step(gpu, ckpt, tmp_path):
+    """The reference harness collects positions, velocities and forces every
+    step (test_run_simulation_basic, test_trajectory_data_shapes,
+    test_get_trajectory_array, test_multiple_runs_without_reset, test_reset,
+    test_trajectory_file_writing).  Here the frames come from the device, 40
+    steps through replayed graphs plus 3 eager ones; consecutive frames must
+    satisfy the Velocity Verlet update itself, and each frame's forces must
+    be the calculator's forces at that frame's positions."""
+    note("NVEMDHarness per-step frames")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    atoms = _mol(20, 2)
+    xyz = tmp_path / "auto.xyz"
+    h = pkg().NVEMDHarness(atoms, calc, temperature=300.0, timestep=0.5, seed=0,
+                           trajectory_file=xyz, log_interval=10)
+    h.run_simulation(43, save_interval=10)
+    assert h.is_complete and h.n_atoms == 20
+    x = h.get_trajectory_array("positions")
+    v = h.get_trajectory_array("velocities")
+    f = h.get_trajectory_array("forces")
+    t = h.get_trajectory_array("time")
+    assert x.shape == v.shape == f.shape == (44, 20, 3) and t.shape == (44,)
+    # trajectory_file holds steps 0, 10, 20, 30, 40
+    assert len(xyz.read_text().splitlines()) == 5 * (2 + 20)
+    synth = pkg_mod("synth")
+    m = synth.masses_of(atoms.get_atomic_numbers())[None, :, None]
+    dt = 0.5 * synth.FS
+    np.testing.assert_allclose(x[1:], x[:-1] + dt * v[:-1] + 0.5 * dt * dt * f[:-1] / m,
+                               rtol=0, atol=1e-9)
+    np.testing.assert_allclose(v[1:], v[:-1] + 0.5 * dt * (f[:-1] + f[1:]) / m, rtol=0,
+                               atol=1e-9)
+    np.testing.assert_allclose(x[-1], h.atoms.get_positions(), rtol=0, atol=0)
+    for k in (0, 7, 25, 43):
+        a = atoms.copy()
+        a.set_positions(x[k])
+        a.calc = calc
+        np.testing.assert_allclose(f[k], a.get_forces(), rtol=0, atol=1e-4)
+    ekin = 0.5 * np.sum(m * v * v, axis=(1, 2))
+    np.testing.assert_allclose(ekin, h.get_trajectory_array("kinetic_energy"), rtol=1e-12)
+    # a second run without reset appends its own frames, time from 0 again
+    h.run_simulation(5, initialize_velocities=False)
+    assert len(h.trajectory_data["time"]) == 44 + 6
+    assert h.trajectory_data["time"][44] == 0.0
+    np.testing.assert_allclose(h.trajectory_data["positions"][44], x[-1], rtol=0, atol=0)
+    out = tmp_path / "all.xyz"
+    h.save_trajectory(out, format="xyz")
+    assert len(out.read_text().splitlines()) == 50 * (2 + 20)
+    h.reset()
+    assert not h.is_complete and len(h.trajectory_data["time"]) == 0
 
 
 def test_torchscript_export_runs_on_the_kernels(gpu, ckpt, tmp_path):
