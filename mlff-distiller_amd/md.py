@@ -338,6 +338,8 @@ class NVEMDHarness:
         if steps <= 0:
             raise ValueError(f"steps must be positive, got {steps}")
         save_interval = save_interval or self.log_interval
+        if self.trajectory_file:
+            self._writable(self.trajectory_file, None)     # fail before the run, not after it
         t0 = _time.perf_counter()
         try:
             if initialize_velocities or self._momenta is None:
@@ -423,6 +425,18 @@ class NVEMDHarness:
             raise RuntimeError("No trajectory data to save")
         self._write(filename, np.arange(len(self.trajectory_data["time"])), format)
 
+    @staticmethod
+    def _writable(filename, format):
+        """Without ASE only extended XYZ can be written."""
+        try:
+            import ase.io  # noqa: F401
+            return
+        except ImportError:
+            pass
+        fmt = format or str(filename).rsplit(".", 1)[-1]
+        if fmt not in ("xyz", "extxyz"):
+            raise ValueError(f"format {fmt!r} needs ASE (not installed); use 'xyz' or 'extxyz'")
+
     def _write(self, filename, rows, format):
         td = self.trajectory_data
         keys = ("time", "potential_energy", "kinetic_energy", "total_energy", "temperature")
@@ -442,9 +456,7 @@ class NVEMDHarness:
                 frames.append(a)
             ase_write(str(filename), frames, format=format)
             return
-        fmt = format or str(filename).rsplit(".", 1)[-1]
-        if fmt not in ("xyz", "extxyz"):
-            raise ValueError(f"format {fmt!r} needs ASE (not installed); use 'xyz' or 'extxyz'")
+        self._writable(filename, format)
         s = self._structure()
         write_extxyz(filename, s.numbers, [td["positions"][i] for i in rows],
                      [td["velocities"][i] for i in rows], [td["forces"][i] for i in rows],

@@ -98,6 +98,13 @@ def test_harness_state_before_any_run():
     with pytest.warns(UserWarning):
         with pytest.raises(RuntimeError, match="No trajectory data to save"):
             h.save_trajectory("unused.xyz", format="xyz")
+    try:
+        import ase  # noqa: F401
+    except ImportError:      # a format only ASE writes fails before any step runs
+        h2 = pkg().NVEMDHarness(Atoms([8, 1, 1], np.eye(3)), _CalcStub(),
+                                trajectory_file="run.traj")
+        with pytest.raises(ValueError, match="needs ASE"):
+            h2.run_simulation(10)
 
 
 def test_extxyz_writer_reads_back(tmp_path):
