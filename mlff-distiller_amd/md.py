@@ -16,6 +16,7 @@ near zero.
 from __future__ import annotations
 
 import ctypes
+import logging
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -23,6 +24,8 @@ import torch
 
 from . import _capi as C
 from . import engine, synth
+
+logger = logging.getLogger(__name__)
 
 
 class BatchedNVE:
@@ -371,10 +374,27 @@ class NVEMDHarness:
             self._n_steps = steps
             self._wall_time = _time.perf_counter() - t0
             self._simulation_complete = True
-            return self._compute_summary()
+            k0 = len(td["time"]) - (steps + 1)
+            for r in range(self.log_interval, steps + 1, max(self.log_interval, 1)):
+                self._log_progress(k0 + r)
+            results = self._compute_summary()
+            logger.info(f"Simulation complete: {steps} steps in {self._wall_time:.2f}s "
+                        f"({steps / max(self._wall_time, 1e-12):.1f} steps/s)")
+            logger.info(f"Energy drift: {results['energy_drift_pct']:.4f}%, "
+                        f"Avg T: {results['avg_temperature']:.2f}K")
+            return results
         except Exception as e:
             raise RuntimeError(f"NVE MD simulation failed after "
                                f"{len(self.trajectory_data['time'])} steps: {e}") from e
+
+    def _log_progress(self, k: int):
+        """nve_harness.py:289-309, for collected row k (the reference logs as
+        the run goes; here the rows come back from the device after it)."""
+        td = self.trajectory_data
+        etot, e0 = td["total_energy"][k], td["total_energy"][0]
+        drift_pct = 100.0 * (etot - e0) / abs(e0) if e0 != 0 else 0.0
+        logger.debug(f"Step {k:6d}: t={td['time'][k]:8.3f}ps, E={etot:12.6f}eV, "
+                     f"T={td['temperature'][k]:7.2f}K, drift={drift_pct:+.4f}%")
 
     def _compute_summary(self):
         """nve_harness.py:311-360."""
