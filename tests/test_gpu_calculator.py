@@ -224,3 +224,23 @@ def test_steepest_descent_lowers_the_energy(gpu, ckpt):
         atoms.positions = atoms.positions + step * f
         energies.append(atoms.get_potential_energy())
     assert np.all(np.diff(energies) < 0), energies
+
+
+def test_harness_overhead_and_trajectory_memory(gpu, ckpt):
+    """test_md_framework.py test_harness_overhead and
+    test_trajectory_memory_efficiency: 100 steps of a water molecule finish
+    in under 5 s and the position and force lists stay under 1 MB."""
+    import sys
+    import time
+    note("NVEMDHarness water 100 steps")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    water = Atoms([8, 1, 1], [[0.0, 0.0, 0.119], [0.0, 0.763, -0.477], [0.0, -0.763, -0.477]])
+    h = pkg().NVEMDHarness(water, calc, temperature=100.0, timestep=1.0, seed=0)
+    t0 = time.perf_counter()
+    out = h.run_simulation(steps=100)
+    assert time.perf_counter() - t0 < 5.0
+    assert out["n_steps"] == 100 and len(h.trajectory_data["time"]) == 101
+    td = h.trajectory_data
+    assert sys.getsizeof(td["positions"]) + sys.getsizeof(td["forces"]) < 1_000_000
+    assert not np.any(np.isnan(h.get_trajectory_array("total_energy")))
