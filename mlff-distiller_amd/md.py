@@ -340,7 +340,7 @@ class NVEMDHarness:
         import time as _time
         if steps <= 0:
             raise ValueError(f"steps must be positive, got {steps}")
-        save_interval = save_interval or self.log_interval
+        save_interval = max(int(save_interval or self.log_interval or 1), 1)
         if self.trajectory_file:
             self._writable(self.trajectory_file, None)     # fail before the run, not after it
         t0 = _time.perf_counter()
