@@ -14,6 +14,14 @@
  * cells the shortest of its 27 neighbouring images.  Every float operation
  * is spelled out (fmaf where fused) so the result is reproducible.
  *
+ * Pinned (tests/test_oracle.py): the non-periodic output equals the edge
+ * lists the reference itself wrote (tests/golden/nl_fp32.npz, pairs planted
+ * at 5 A +- 3 ulp, single and batched); the periodic output equals
+ * painn_ref.minimum_image_graph -- itself pinned by explicit replication of
+ * the neighbouring images -- on orthorhombic, skewed and slab cells and on
+ * pairs planted across a periodic boundary at the cutoff and +- 1 ulp; the
+ * norm rounding equals torch.norm on 200k random vectors.
+ *
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared nl_ref.c -o _build/libnl_ref.so -lm
  */
 #include <math.h>

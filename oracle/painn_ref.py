@@ -60,13 +60,14 @@ def radius_graph_dense(positions: Tensor, r: float, batch: Optional[Tensor] = No
 
 
 def minimum_image_graph(positions: np.ndarray, cell: np.ndarray, r: float,
-                        dtype=np.float64) -> Tuple[np.ndarray, np.ndarray]:
+                        dtype=np.float64, pbc=(True, True, True)) -> Tuple[np.ndarray, np.ndarray]:
     """Brute-force periodic neighbour list (the build's PBC definition).
 
     For every ordered pair (i, j), i != j, and every lattice shift
     S in {-1,0,1}^3 applied to the fractionally wrapped difference, the edge
-    (src=j, dst=i) exists when ||p_j - p_i + S_wrapped . cell|| <= r.  With
-    r below half the smallest perpendicular cell width at most one image
+    (src=j, dst=i) exists when ||p_j - p_i + S_wrapped . cell|| <= r.  Along
+    an axis with ``pbc`` false there is no wrap and no shift (slabs, wires).
+    With r below half the smallest perpendicular cell width at most one image
     qualifies.  Returns edge_index [2, E] (src, dst) sorted by (dst, src) and
     the integer shift [E, 3] such that
     edge_vector = p[src] - p[dst] + shift @ cell.
@@ -74,13 +75,15 @@ def minimum_image_graph(positions: np.ndarray, cell: np.ndarray, r: float,
     pos = np.asarray(positions, dtype=dtype)
     cell = np.asarray(cell, dtype=dtype)
     inv = np.linalg.inv(cell)
+    per = np.array([bool(x) for x in pbc])
     n = len(pos)
     src_l, dst_l, sh_l = [], [], []
-    shifts = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)])
+    rng1 = [(-1, 0, 1) if per[a] else (0,) for a in range(3)]
+    shifts = np.array([[a, b, c] for a in rng1[0] for b in rng1[1] for c in rng1[2]])
     for i in range(n):
         d = pos - pos[i]                       # p_j - p_i
         frac = d @ inv
-        base = -np.round(frac)                 # wrap to nearest image
+        base = -np.round(frac) * per           # wrap to nearest image along periodic axes
         for j in range(n):
             if j == i:
                 continue

@@ -140,26 +140,116 @@ def test_total_force_vanishes():
     assert np.abs(F.sum(0)).max() < 1e-10
 
 
-def test_periodic_graph_equals_explicit_images():
-    """Minimum-image edges == edges of the centre cell against its 27 replicas."""
+_CELLS = {
+    "orthorhombic": (np.eye(3) * 11.0, (True, True, True)),
+    "skewed": (np.array([[11.0, 0.0, 0.0], [2.5, 10.5, 0.0], [-1.5, 1.8, 11.5]]),
+               (True, True, True)),
+    "slab": (np.array([[11.0, 0.0, 0.0], [1.2, 10.8, 0.0], [0.0, 0.0, 12.0]]),
+             (True, True, False)),
+}
+
+
+@pytest.mark.parametrize("kind", sorted(_CELLS))
+def test_periodic_graph_equals_explicit_images(kind):
+    """Minimum-image edges == edges of the centre cell against its replicas
+    (27 images; only in-plane ones for the slab, whose third axis is open)."""
+    cell, pbc = _CELLS[kind]
     rng = np.random.default_rng(4)
-    L = 11.0
-    cell = np.eye(3) * L
-    pos = rng.uniform(0, L, size=(30, 3))
-    ei, sh = R.minimum_image_graph(pos, cell, 4.0)
+    n = 30
+    pos = rng.uniform(0, 1, size=(n, 3)) @ cell
+    ei, sh = R.minimum_image_graph(pos, cell, 4.0, pbc=pbc)
     got = {(int(s), int(d), tuple(int(x) for x in v)) for s, d, v in zip(ei[0], ei[1], sh)}
+    assert len(got) == ei.shape[1]
     want = set()
-    for i in range(30):
-        for j in range(30):
-            for a in (-1, 0, 1):
-                for b in (-1, 0, 1):
-                    for c in (-1, 0, 1):
+    imgs = [(-1, 0, 1) if p else (0,) for p in pbc]
+    for i in range(n):
+        for j in range(n):
+            for a in imgs[0]:
+                for b in imgs[1]:
+                    for c in imgs[2]:
                         if i == j and (a, b, c) == (0, 0, 0):
                             continue
                         v = pos[j] - pos[i] + np.array([a, b, c]) @ cell
                         if math.sqrt(v @ v) <= 4.0:
                             want.add((j, i, (a, b, c)))
     assert got == want
+    if kind == "slab":
+        assert all(s[2][2] == 0 for s in got)
+
+
+def _csr_edges(row_ptr, col):
+    rows = np.repeat(np.arange(len(row_ptr) - 1), np.diff(row_ptr))
+    return np.stack([rows.astype(np.int64), col.astype(np.int64)])
+
+
+def test_nl_ref_matches_reference_golden_edges():
+    """oracle/nl_ref.c == the reference's own radius_graph_native output
+    (tests/golden/nl_fp32.npz, written by student_model.py:63-109 with pairs
+    planted at 5 A +- 3 ulp), single structure and two-structure batch."""
+    from oracle import nl_ref
+    g = golden("nl_fp32.npz")
+    pos = g["positions"]
+    n = len(pos)
+    row_ptr, col, _ = nl_ref.neighbor_csr(pos, [0, n], 5.0)
+    np.testing.assert_array_equal(_csr_edges(row_ptr, col), g["edge_index"])
+    b = g["batch"]
+    mol_ptr = np.concatenate([[0], np.cumsum(np.bincount(b))])
+    row_ptr, col, _ = nl_ref.neighbor_csr(pos, mol_ptr, 5.0)
+    np.testing.assert_array_equal(_csr_edges(row_ptr, col), g["edge_index_batched"])
+
+
+def _planted_boundary_box():
+    """A 10.5 A orthorhombic box (rc = 5 A is below half its width) holding
+    random atoms plus three pairs that meet only across the x boundary, at
+    image distance exactly 5 A, 5 A - 1 ulp and 5 A + 1 ulp.  Every operation
+    on the planted coordinates is exact in float32 and float64 (differences
+    of 5.5 +- k 2^-21 and the shift -10.5), so both precisions must agree."""
+    L = 10.5
+    rng = np.random.default_rng(12)
+    base = rng.uniform(0, 1, size=(24, 3)) * L
+    planted, expect = [], []
+    for k, (y, z) in zip((0, 1, -1), ((1.0, 1.0), (4.0, 4.0), (7.0, 7.0))):
+        xi = np.float32(0.25)
+        xj = np.float32(5.75 + k * 2.0 ** -21)
+        planted += [[xi, y, z], [xj, y, z]]
+        expect.append(k >= 0)
+    pos = np.concatenate([base, np.array(planted)]).astype(np.float32)
+    return pos, np.eye(3) * L, expect
+
+
+@pytest.mark.parametrize("kind", sorted(_CELLS) + ["planted"])
+def test_nl_ref_periodic_matches_minimum_image_graph(kind):
+    """oracle/nl_ref.c's periodic branch == painn_ref.minimum_image_graph
+    (pinned by explicit replication above): same (src, dst, shift) edges in
+    the same receiver-major order, on orthorhombic, skewed and slab cells and
+    on pairs planted at the cutoff +- 1 ulp across a periodic boundary."""
+    from oracle import nl_ref
+    if kind == "planted":
+        pos, cell, expect = _planted_boundary_box()
+        pbc = (True, True, True)
+        rc = 5.0
+    else:
+        cell, pbc = _CELLS[kind]
+        rng = np.random.default_rng(7)
+        pos = (rng.uniform(0, 1, size=(60, 3)) @ cell).astype(np.float32)
+        rc = 5.0 if kind != "skewed" else 4.5
+    n = len(pos)
+    row_ptr, col, shift = nl_ref.neighbor_csr(pos, [0, n], rc, cells=cell[None].astype(np.float32),
+                                              pbc=np.array([pbc], dtype=np.uint8))
+    ei, sh = R.minimum_image_graph(pos.astype(np.float64), cell.astype(np.float32).astype(np.float64),
+                                   rc, pbc=pbc)
+    got = _csr_edges(row_ptr, col)
+    np.testing.assert_array_equal(got, ei[::-1])
+    np.testing.assert_array_equal(shift, sh)
+    assert ei.shape[1] > 10 * n
+    if kind == "planted":
+        have = set(zip(got[0].tolist(), got[1].tolist()))
+        for q, inside in enumerate(expect):
+            i, j = 24 + 2 * q, 25 + 2 * q
+            assert ((i, j) in have) == inside and ((j, i) in have) == inside, (q, inside)
+        d = nl_ref.norm3_f32(np.array([[-5.0 + k * 2.0 ** -21, 0, 0] for k in (0, 1, -1)],
+                                      dtype=np.float32))
+        assert d[0] == 5.0 and d[1] < 5.0 and d[2] > 5.0
 
 
 def test_virial_is_strain_derivative():
