@@ -32,6 +32,7 @@ from typing import Any, Dict, List, Optional, Union
 import numpy as np
 import torch
 
+from . import engine
 from .model import StudentForceField
 
 logger = logging.getLogger(__name__)
@@ -111,7 +112,7 @@ class StudentForceFieldCalculator(Calculator):
                  **kwargs):
         super().__init__(**kwargs)
         self.checkpoint_path = Path(checkpoint_path) if checkpoint_path is not None else None
-        self.device = torch.device(device)
+        self.device = engine.device_of(device)
         self.dtype = dtype
         self.enable_stress = enable_stress
         self.batch_size = batch_size

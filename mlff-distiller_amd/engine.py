@@ -15,6 +15,19 @@ import torch
 
 from . import _capi as C
 
+def device_of(device) -> torch.device:
+    """A device with its index: torch.device('cuda') becomes cuda:<current>,
+    so device comparisons and cache keys see one name per GPU."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        try:
+            idx = torch.cuda.current_device()
+        except (RuntimeError, AssertionError):     # no ROCm device visible
+            idx = 0
+        d = torch.device("cuda", idx)
+    return d
+
+
 # ---------------------------------------------------------------------------
 # weights
 # ---------------------------------------------------------------------------
@@ -55,7 +68,7 @@ def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
     training step's weight update costs device copies only)."""
     H, L, R, max_z = hyper_from_state(sd)
     slots, total = C.weights_layout(H, L, R, max_z)
-    device = torch.device(device)
+    device = device_of(device)
     on_device = all(v.device == device for v in sd.values())
     where = device if on_device else torch.device("cpu")
     host = torch.zeros(total, dtype=torch.float32, device=where)

@@ -35,9 +35,9 @@ class BatchedNVE:
                  temperature_K: float = 300.0, seed: int = 0, momenta: Optional[np.ndarray] = None,
                  max_steps: int = 100000, headroom: float = 1.3, device="cuda:0",
                  persistent_blocks: int = 0, streams: Optional[int] = None,
-                 record_frames: bool = False):
+                 record_frames: bool = False, masses: Optional[np.ndarray] = None):
         self.model = model
-        self.device = torch.device(device)
+        self.device = engine.device_of(device)
         self.structures = list(structures)
         numbers, positions, mol_ptr = synth.flatten(self.structures)
         self.sizes = np.diff(mol_ptr)
@@ -45,12 +45,16 @@ class BatchedNVE:
         self.B = len(self.structures)
         self.dt = timestep_fs * synth.FS
         self.timestep_fs = timestep_fs
-        masses = synth.masses_of(numbers)
+        masses = synth.masses_of(numbers) if masses is None else \
+            np.asarray(masses, dtype=np.float64).reshape(-1)
+        if len(masses) != len(numbers) or not np.all(masses > 0):
+            raise ValueError("masses must be positive, one per atom")
         if momenta is None:
             rng = np.random.Generator(np.random.PCG64(seed))
             momenta = np.concatenate([
-                synth.maxwell_boltzmann(s.numbers, s.positions, temperature_K, rng)
-                for s in self.structures])
+                synth.maxwell_boltzmann(s.numbers, s.positions, temperature_K, rng,
+                                        masses=masses[a:b])
+                for s, a, b in zip(self.structures, mol_ptr[:-1], mol_ptr[1:])])
         periodic = any(s.pbc is not None and np.any(s.pbc) for s in self.structures)
         cells = pbc = None
         if periodic:
@@ -128,8 +132,10 @@ class BatchedNVE:
         return max(int(col.numel()), 16)
 
     def _make_plan(self, edges: float):
+        # the bound caps the first guess only: a measured count above it (a
+        # dense box, more than 256 neighbours per atom) is honoured as is
         bound = engine.edge_capacity_bound(self.sizes, self.layout.periodic)
-        cap = int(min(bound, int(edges) + 64))
+        cap = int(min(bound, int(edges) + 64)) if edges <= bound else int(edges) + 64
         self.plan = engine.Plan(self.model.hidden_dim, self.model.num_interactions,
                                 self.model.num_rbf, self.model.max_z, self.model.cutoff,
                                 self.layout, self.numbers, self.weights, cap,
@@ -158,6 +164,7 @@ class BatchedNVE:
             raise ValueError("ledger too short; raise max_steps")
         done = 0
         chunk = max(steps_per_graph, 1) * 10
+        replays = 0
         while done < n_steps:
             k = min(chunk, n_steps - done)
             saved = (self.x.clone(), self.p.clone(), self.forces.clone(), self.energy.clone(),
@@ -169,7 +176,13 @@ class BatchedNVE:
                 self.x.copy_(saved[0]); self.p.copy_(saved[1]); self.forces.copy_(saved[2])
                 self.energy.copy_(saved[3])
                 step = int(saved[4][C.MLFF_COUNT_STEP])
-                self._make_plan(int(exc.args[0]) * max(self._headroom, 1.1))
+                need = int(exc.args[0])
+                replays += 1
+                if replays > 8 or need <= self.plan.ecap:
+                    raise RuntimeError(f"neighbour capacity did not converge: {need} edges "
+                                       f"needed, capacity {self.plan.ecap} after "
+                                       f"{replays} replays")
+                self._make_plan(need * max(self._headroom, 1.1))
                 self.plan.counts[C.MLFF_COUNT_STEP] = step
                 self.pos32.copy_(self.x.to(torch.float32))
                 continue
@@ -299,6 +312,10 @@ class NVEMDHarness:
         self.remove_com_motion = remove_com_motion
         self._rng = np.random.Generator(np.random.PCG64(seed))
         self._momenta = None
+        try:
+            self.atoms.calc = calculator        # nve_harness.py:117
+        except AttributeError:
+            pass
         self.trajectory_data = {k: [] for k in ("time", "positions", "velocities", "forces",
                                                 "potential_energy", "kinetic_energy",
                                                 "total_energy", "temperature")}
@@ -314,13 +331,34 @@ class NVEMDHarness:
                                np.asarray(a.get_cell(), dtype=np.float64) if pbc.any() else None,
                                pbc if pbc.any() else None)
 
+    def _masses(self) -> np.ndarray:
+        if hasattr(self.atoms, "get_masses"):
+            return np.asarray(self.atoms.get_masses(), dtype=np.float64)
+        return synth.masses_of(self.atoms.get_atomic_numbers())
+
+    def _atoms_momenta(self) -> np.ndarray:
+        """The atoms' own momenta (zeros when none were set, as ASE)."""
+        if hasattr(self.atoms, "get_momenta"):
+            p = self.atoms.get_momenta()
+            if p is not None:
+                return np.asarray(p, dtype=np.float64).reshape(-1, 3)
+        if self._momenta is not None:
+            return self._momenta
+        return np.zeros((len(self.atoms), 3))
+
     def initialize_velocities(self, temperature: Optional[float] = None,
                               remove_com_translation: bool = True,
                               remove_com_rotation: bool = True):
+        """nve_harness.py:141-171: Maxwell-Boltzmann momenta at the
+        temperature, centre-of-mass translation and rotation removed, set on
+        the atoms."""
         s = self._structure()
         T = self.temperature if temperature is None else temperature
         self._momenta = synth.maxwell_boltzmann(s.numbers, s.positions, T, self._rng,
-                                                remove_com_translation, remove_com_rotation)
+                                                remove_com_translation, remove_com_rotation,
+                                                masses=self._masses())
+        if hasattr(self.atoms, "set_momenta"):
+            self.atoms.set_momenta(self._momenta)
 
     @property
     def is_complete(self) -> bool:
@@ -345,12 +383,12 @@ class NVEMDHarness:
             self._writable(self.trajectory_file, None)     # fail before the run, not after it
         t0 = _time.perf_counter()
         try:
-            if initialize_velocities or self._momenta is None:
+            if initialize_velocities:
                 self.initialize_velocities()
             md = BatchedNVE(self.calculator.model, [self._structure()],
-                            timestep_fs=self.timestep_fs, momenta=self._momenta,
+                            timestep_fs=self.timestep_fs, momenta=self._atoms_momenta(),
                             max_steps=steps, device=self.calculator.device,
-                            record_frames=True)
+                            record_frames=True, masses=self._masses())
             md.run(steps, graph=steps >= 10, steps_per_graph=10)
             epot, ekin = md.energies()
             x, v, f = md.frames()
@@ -367,6 +405,8 @@ class NVEMDHarness:
             if hasattr(self.atoms, "set_positions"):
                 self.atoms.set_positions(x[-1])
             self._momenta = md.p.cpu().numpy()
+            if hasattr(self.atoms, "set_momenta"):
+                self.atoms.set_momenta(self._momenta)
             if self.trajectory_file:
                 rows = np.arange(0, steps + 1, save_interval)
                 k0 = len(td["time"]) - (steps + 1)

@@ -174,14 +174,19 @@ class _Session:
     def param_grads(self, gE):
         from . import train
         p, g, c = self._prepared()
-        grads, _, _ = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype), want_forces=False)
+        grads, _, _ = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype), want_forces=False,
+                                    rbf_grads=self._rbf_grads())
         return grads
+
+    def _rbf_grads(self) -> bool:
+        return any(n.startswith("rbf.") for n, _ in self.model.named_parameters())
 
     def tangent_grads(self, gE, lam):
         from . import train
         p, g, c = self._prepared()
         _, _, tg = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype),
-                                 lam=self._to_sorted(lam.to(self.pos.dtype)), want_forces=False)
+                                 lam=self._to_sorted(lam.to(self.pos.dtype)), want_forces=False,
+                                 rbf_grads=self._rbf_grads())
         return tg
 
 
@@ -298,6 +303,7 @@ class StudentForceField(nn.Module):
     # device plumbing
     # ------------------------------------------------------------------
     def packed_weights(self, device) -> torch.Tensor:
+        device = engine.device_of(device)
         sig = tuple((t.data_ptr(), t._version) for t in self.state_dict().values())
         if self._packed is None or self._packed_sig != sig or self._packed.device != device:
             self._packed = engine.pack_weights(self.state_dict(), device)

@@ -123,7 +123,7 @@ class MTSNVE:
                  inner_fs: float = 0.5, n_inner: int = 4, temperature_K: float = 300.0,
                  seed: int = 0, threshold: float = 2.0, max_outer: int = 20000,
                  headroom: float = 1.3, device="cuda:0", momenta: Optional[np.ndarray] = None):
-        dev = torch.device(device)
+        dev = engine.device_of(device)
         self.device = dev
         self.full, self.fast = full_model, fast_model
         if full_model.cutoff != fast_model.cutoff:

@@ -165,15 +165,17 @@ def water_box(n_side: int = 10, density_g_cm3: float = 1.0, seed: int = 3,
 
 def maxwell_boltzmann(numbers: np.ndarray, positions: np.ndarray, temperature_K: float,
                       rng: np.random.Generator, zero_com: bool = True,
-                      zero_rotation: bool = True) -> np.ndarray:
+                      zero_rotation: bool = True,
+                      masses: Optional[np.ndarray] = None) -> np.ndarray:
     """Momenta (amu*A/time-unit) drawn at ``temperature_K``.
 
     Mirrors ASE's MaxwellBoltzmannDistribution (p = N(0,1) * sqrt(m kT)),
     Stationary (subtract the centre-of-mass momentum share m_i P / M) and
     ZeroRotation (subtract omega x (r - r_com) with omega = I^-1 L), the
     calls made at src/mlff_distiller/testing/nve_harness.py:158-165.
+    ``masses`` (amu) default to the tabulated masses of ``numbers``.
     """
-    m = masses_of(numbers)
+    m = masses_of(numbers) if masses is None else np.asarray(masses, dtype=np.float64)
     kT = KB_EV * temperature_K
     p = rng.standard_normal((len(m), 3)) * np.sqrt(m * kT)[:, None]
     if zero_com:

@@ -104,6 +104,27 @@ def _edge_features_d2(d, centers, widths, rc):
     return d2phi * fc + 2 * dphi * dfc + phi * d2fc
 
 
+def _edge_feature_param_derivs(d, centers, widths, rc):
+    """Derivatives of e_k(d) = phi_k(d) fc(d) in the basis parameters
+    (learnable_rbf, student_model.py:226-236): de_k/dmu_k, de_k/dw_k, and
+    the d-derivatives of both (for the tangent pass), each [P, R].
+    phi = exp(-(d-mu)^2 / w^2): dphi/dmu = 2 gamma (d-mu) phi,
+    dphi/dw = 2 (d-mu)^2 phi / w^3, gamma = 1/w^2."""
+    diff = d.unsqueeze(-1) - centers
+    gamma = 1.0 / widths ** 2
+    phi = torch.exp(-gamma * diff ** 2)
+    inside = (d < rc).to(d.dtype).unsqueeze(-1)
+    arg = math.pi * d.unsqueeze(-1) / rc
+    fc = 0.5 * (torch.cos(arg) + 1.0) * inside
+    dfc = -0.5 * (math.pi / rc) * torch.sin(arg) * inside
+    w3 = widths ** 3
+    A = 2.0 * gamma * diff * phi
+    Bw = 2.0 * diff ** 2 * phi / w3
+    dA = 2.0 * gamma * phi * (1.0 - 2.0 * gamma * diff ** 2)
+    dBw = (2.0 * diff * phi / w3) * (2.0 - 2.0 * gamma * diff ** 2)
+    return fc * A, fc * Bw, dfc * A + fc * dA, dfc * Bw + fc * dBw
+
+
 def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
     """a.T @ b for tall a [K, M], b [K, N] (a weight gradient, K = pairs or
     atoms): the K reduction is split into chunks computed as one batched
@@ -213,7 +234,7 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float)
 
 def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             lam: Optional[Tensor] = None, want_forces: bool = True, want_grads: bool = True,
-            gw: Optional[Tensor] = None):
+            gw: Optional[Tensor] = None, rbf_grads: bool = False):
     """Adjoint of E weighted by gE [B].
 
     Returns (grads, forces, tgrads): parameter gradients of sum_b gw_b E_b
@@ -227,6 +248,8 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     structure; ``gw`` is applied that way (gE must then be all ones), which
     lets one pass give the unit-weight tangent and the gw-weighted
     gradients together.  ``want_grads=False`` skips the parameter gradients.
+    ``rbf_grads`` adds the gradients of the basis centres and widths
+    (learnable_rbf); they are buffers otherwise and get none.
     """
     H, L, n = c["H"], c["L"], c["n"]
     T = lam is not None
@@ -331,6 +354,9 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     gd = torch.zeros_like(c["d"])
     gu = torch.zeros_like(c["u"])
     u = c["u"]
+    if rbf_grads:
+        e_mu, e_w, de_mu, de_w = _edge_feature_param_derivs(
+            c["d"], p["rbf.centers"].to(dt), p["rbf.widths"].to(dt), c["rc"])
     for k in range(L - 1, -1, -1):
         lay = c["layers"][k]
         upre = f"interactions.{k}.update"
@@ -401,6 +427,9 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         accg(mpre + ".0.bias", lambda: wp(ga1).sum(0))
         ge = ga1 @ W1
         gd = gd + (ge * c["de"]).sum(-1)
+        if rbf_grads:
+            accg("rbf.centers", lambda: (wp(ge) * e_mu).sum(0))
+            accg("rbf.widths", lambda: (wp(ge) * e_w).sum(0))
         gs_new = gs_m + _add_pairs(n, ga, gb, gs_m[gb] * Ws, gs_m[ga] * Ws)
         gv_new = gv_m + _add_pairs(n, ga, gb, gv_m[gb] * Wv1.unsqueeze(1),
                                    gv_m[ga] * Wv1.unsqueeze(1))
@@ -419,6 +448,10 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             dga1 = dgh * _silu1(lay["a1"]) + gh * _silu2(lay["a1"]) * da1
             acc(tg, mpre + ".0.weight", _tmm(dga1, c["e"]) + _tmm(ga1, de_t))
             acc(tg, mpre + ".0.bias", dga1.sum(0))
+            if rbf_grads:
+                dge = dga1 @ W1
+                acc(tg, "rbf.centers", (dge * e_mu + ge * de_mu * dd.unsqueeze(-1)).sum(0))
+                acc(tg, "rbf.widths", (dge * e_w + ge * de_w * dd.unsqueeze(-1)).sum(0))
             dgs = dgs_m + _add_pairs(n, ga, gb, dgs_m[gb] * Ws + gs_m[gb] * dWs,
                                      dgs_m[ga] * Ws + gs_m[ga] * dWs)
             dgv = dgv_m + _add_pairs(n, ga, gb,
@@ -476,7 +509,8 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
 
 def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
                          E_label: Tensor, F_label: Tensor, w_energy: float = 1.0,
-                         w_force: float = 100.0, forces: Optional[Tensor] = None):
+                         w_force: float = 100.0, forces: Optional[Tensor] = None,
+                         rbf_grads: bool = False):
     """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and
     its exact gradient with respect to every parameter in ``p``.
     ``forces``: the model's forces at ``x`` when already known (the fused
@@ -496,7 +530,8 @@ def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGrap
     # one more pass gives both parts:
     #   d/dtheta of wE MSE(E): the unit adjoint's terms weighted by dL/dE (gw)
     #   d/dtheta of wF MSE(F) = d/dtheta < dL/dF, F > = - d/dtheta < dL/dF, grad_x sum E >
-    gE, _, tg = reverse(p, cache, g, ones, lam=dLdF, want_forces=False, gw=dLdE)
+    gE, _, tg = reverse(p, cache, g, ones, lam=dLdF, want_forces=False, gw=dLdE,
+                        rbf_grads=rbf_grads)
     grads = {}
     for name in p:
         a = gE.get(name)
@@ -519,9 +554,8 @@ class ForceMatchingTrainer:
 
     def __init__(self, model, lr: float = 1e-3, weight_decay: float = 0.0,
                  w_energy: float = 1.0, w_force: float = 100.0, grad_clip: float = 1.0):
-        if any(n.startswith("rbf.") for n, _ in model.named_parameters()):
-            raise NotImplementedError("training learnable RBF centres/widths is not supported")
         self.model = model
+        self.rbf_grads = any(n.startswith("rbf.") for n, _ in model.named_parameters())
         self.w_energy, self.w_force, self.grad_clip = w_energy, w_force, grad_clip
         self.opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=weight_decay)
 
@@ -534,7 +568,8 @@ class ForceMatchingTrainer:
             _, F, _ = self.model.evaluate(Z, x.detach(), np.asarray(sizes), cells, pbc)
         with torch.no_grad():
             return force_matching_grads(p, Z, x.detach(), g, self.model.cutoff, E_label, F_label,
-                                        self.w_energy, self.w_force, forces=F)
+                                        self.w_energy, self.w_force, forces=F,
+                                        rbf_grads=self.rbf_grads)
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)

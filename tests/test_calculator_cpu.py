@@ -107,6 +107,39 @@ def test_harness_state_before_any_run():
             h2.run_simulation(10)
 
 
+def test_velocity_initialization():
+    """nve_harness test_velocity_initialization: zero velocities before, then
+    Maxwell-Boltzmann velocities set on the atoms, with a positive
+    temperature, no centre-of-mass momentum, the atoms' own masses, and the
+    calculator attached (nve_harness.py:117, 141-171)."""
+    from atoms_standin import Atoms
+    rng = np.random.default_rng(3)
+    atoms = Atoms([8, 1, 1, 6, 1], rng.normal(size=(5, 3)) * 1.2)
+    calc = _CalcStub()
+    h = pkg().NVEMDHarness(atoms, calc, temperature=300.0, seed=5)
+    assert h.atoms.calc is calc
+    assert np.allclose(h.atoms.get_velocities(), 0.0)
+    h.initialize_velocities()
+    v = h.atoms.get_velocities()
+    assert v.shape == (5, 3) and np.abs(v).max() > 0
+    assert h.atoms.get_temperature() > 0
+    p = h.atoms.get_momenta()
+    assert np.abs(p.sum(0)).max() < 1e-12
+    # heavier hydrogens (deuterium): the momenta follow the atoms' masses
+    heavy = atoms.copy()
+    m = heavy.get_masses()
+    m[heavy.get_atomic_numbers() == 1] = 2.014
+    heavy.set_masses(m)
+    h2 = pkg().NVEMDHarness(heavy, calc, temperature=300.0, seed=5)
+    h2.initialize_velocities(remove_com_translation=False, remove_com_rotation=False)
+    h1 = pkg().NVEMDHarness(atoms, calc, temperature=300.0, seed=5)
+    h1.initialize_velocities(remove_com_translation=False, remove_com_rotation=False)
+    ratio = h2.atoms.get_momenta() / h1.atoms.get_momenta()
+    np.testing.assert_allclose(ratio, np.sqrt(m / atoms.get_masses())[:, None] * np.ones(3),
+                               rtol=1e-12)
+    assert h._masses().tolist() == atoms.get_masses().tolist()
+
+
 def test_extxyz_writer_reads_back(tmp_path):
     """The extended-XYZ writer, parsed back by hand: species, positions,
     velocities, forces and per-frame info for two frames of a periodic cell."""

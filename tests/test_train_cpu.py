@@ -52,13 +52,14 @@ def test_forward_and_forces_match_oracle():
     assert torch.allclose(F, Fref, atol=1e-11, rtol=0)
 
 
-def _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, wE, wF):
+def _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, wE, wF, rbf=False):
     p = {k: v.clone().requires_grad_(True) for k, v in sd.items()}
     pos = x.clone().requires_grad_(True)
     E = R.energy(p, Z, pos, rc, batch=batch, n_structures=int(batch.max()) + 1)
     F = -torch.autograd.grad(E.sum(), pos, create_graph=True)[0]
     loss = wE * ((E - El) ** 2).mean() + wF * ((F - Fl) ** 2).mean()
-    names = [k for k in p if p[k].requires_grad and k not in ("rbf.centers", "rbf.widths")]
+    names = [k for k in p if p[k].requires_grad
+             and (rbf or k not in ("rbf.centers", "rbf.widths"))]
     gr = torch.autograd.grad(loss, [p[k] for k in names], allow_unused=True)
     return loss.detach(), dict(zip(names, gr))
 
@@ -138,3 +139,31 @@ def test_weighted_gradients_equal_weighted_adjoint():
         assert torch.allclose(got[k], ref[k], rtol=1e-11, atol=1e-12), k
     none, F, _ = train.reverse(sd, cache, g, torch.ones_like(E), want_grads=False)
     assert none == {} and F is not None
+
+
+def test_learnable_rbf_gradients_match_autograd_double_backward():
+    """Basis centres and widths as parameters (learnable_rbf,
+    student_model.py:226-236): their loss gradients, first and second order,
+    equal FP64 autograd's; the centres and widths are moved off their
+    defaults so every term of the derivative is exercised."""
+    train = pkg_mod("train")
+    sd, Z, x, batch, rc = _setup(seed=4)
+    gen = torch.Generator().manual_seed(11)
+    sd["rbf.centers"] = sd["rbf.centers"] + 0.1 * torch.randn(sd["rbf.centers"].shape,
+                                                              generator=gen, dtype=torch.float64)
+    sd["rbf.widths"] = sd["rbf.widths"] * (1 + 0.2 * torch.rand(sd["rbf.widths"].shape,
+                                                                generator=gen,
+                                                                dtype=torch.float64))
+    g = _graph(x, batch, rc, 3)
+    El = torch.randn(3, generator=gen, dtype=torch.float64)
+    Fl = torch.randn(x.shape, generator=gen, dtype=torch.float64)
+    _, _, _, grads = train.force_matching_grads(sd, Z, x, g, rc, El, Fl, 1.0, 100.0,
+                                                rbf_grads=True)
+    _, ref = _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, 1.0, 100.0, rbf=True)
+    for k in ("rbf.centers", "rbf.widths"):
+        err = (grads[k] - ref[k]).abs().max().item()
+        scale = max(1.0, ref[k].abs().max().item())
+        assert ref[k].abs().max().item() > 1e-3, k
+        assert err < 1e-9 * scale, (k, err, scale)
+    _, _, _, plain = train.force_matching_grads(sd, Z, x, g, rc, El, Fl, 1.0, 100.0)
+    assert "rbf.centers" not in plain and "rbf.widths" not in plain
