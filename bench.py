@@ -46,6 +46,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_fields():
+    return {"cores": torch.get_num_threads(), "cpu_count": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
+
+
 # ---------------------------------------------------------------------------
 # plain path: eager PyTorch, one molecule at a time (student_model.py:634-795)
 # ---------------------------------------------------------------------------
@@ -145,9 +162,10 @@ def cpu_baseline(sd_cpu, mols, masses_of, dt, steps):
     return atoms / (time.perf_counter() - t0)
 
 
-def accuracy(sd_cpu, md, sample):
+def accuracy(sd_cpu, md, sample=None):
     """max|dF| (eV/A) and max|dE|/N (eV/atom) vs the float64 oracle at the
-    trajectory's current positions, for the first `sample` molecules."""
+    trajectory's current positions, for the first `sample` molecules (all
+    when None)."""
     from oracle import painn_ref as R
     lib = md.plan
     x32 = md.pos32.cpu().numpy().astype(np.float64)
@@ -155,13 +173,31 @@ def accuracy(sd_cpu, md, sample):
     E = md.energy.cpu().numpy()
     mp = np.concatenate([[0], np.cumsum(md.sizes)])
     df = de = 0.0
-    for b in range(min(sample, md.B)):
+    for b in range(md.B if sample is None else min(sample, md.B)):
         sl = slice(mp[b], mp[b + 1])
         Eref, Fref = R.energy_and_forces(sd_cpu, md.structures[b].numbers, x32[sl])
         df = max(df, float(np.abs(F[sl] - Fref.numpy()).max()))
         de = max(de, abs(float(E[b]) - Eref.item()) / md.sizes[b])
     del lib
     return df, de
+
+
+def window_edges(model, mols, args, md_timed):
+    """Mean directed-edge count over the timed MD steps.  The step loop is
+    bit-reproducible, so a second run from the same state, one eager step at
+    a time with the edge count read after each, visits the same trajectory;
+    returns (mean, whether the replay ended on the timed run's positions)."""
+    mdmod = importlib.import_module("mlff-distiller_amd.md")
+    md = mdmod.BatchedNVE(model, mols, timestep_fs=0.5, temperature_K=300.0, seed=args.seed,
+                          max_steps=args.warmup + args.steps + 1, streams=args.streams,
+                          persistent_blocks=md_timed._persistent_blocks)
+    md.run(args.warmup, graph=False)
+    edges = []
+    for _ in range(args.steps):
+        md.run(1, graph=False)
+        edges.append(int(md.plan.read_counts()[0]))
+    same = bool(torch.equal(md.x, md_timed.x) and torch.equal(md.p, md_timed.p))
+    return float(np.mean(edges)), same
 
 
 # ---------------------------------------------------------------------------
@@ -218,7 +254,9 @@ def config1(args, pkg, synth, mdmod, dev):
     torch.manual_seed(0)
     model = pkg.StudentForceField().to(dev).eval()
     mol = synth.single_molecule(60, seed=1)
-    steps = args.steps if args.steps != 500 else 1000
+    steps = args.steps if args.steps is not None else 1000
+    warmup = args.warmup if args.warmup is not None else 20
+    args.warmup = warmup
     md = mdmod.BatchedNVE(model, [mol], seed=1, max_steps=steps + args.warmup + 1,
                           streams=args.streams)
     md.run(args.warmup, graph=True)
@@ -233,7 +271,7 @@ def config1(args, pkg, synth, mdmod, dev):
     s = md.summary()
     sd_dev = {k: v.detach() for k, v in model.state_dict().items()}
     plain = plain_md_rate(sd_dev, 128, 3, [mol], synth.masses_of, 0.5 * synth.FS, 50, dev)
-    cpu_steps = args.cpu_steps if args.cpu_steps != 2 else 300
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else 1000
     cpu = cpu_baseline(sd_cpu, [mol], synth.masses_of, 0.5 * synth.FS, cpu_steps)
     return _common("atom-steps/s (E+F), single 60-atom molecule NVE", value, "atom-steps/s",
                    steps, args.warmup, wall, "f32",
@@ -242,9 +280,9 @@ def config1(args, pkg, synth, mdmod, dev):
                    plain={"value": plain, "unit": "atom-steps/s", "speedup": value / plain},
                    nve={"drift_per_atom_eV": float(s["energy_drift_per_atom"][0]),
                         "drift_pct": float(s["energy_drift_pct"][0])},
-                   cpu_baseline={"value": cpu, "unit": "atom-steps/s",
-                                 "cores": torch.get_num_threads(), "kind": "port",
-                                 "sample": f"the same molecule, {cpu_steps} MD steps"})
+                   cpu_baseline={"value": cpu, "unit": "atom-steps/s", "kind": "port",
+                                 "sample": f"the same molecule, {cpu_steps} MD steps",
+                                 **cpu_fields()})
 
 
 def config3(args, pkg, synth, mdmod, dev):
@@ -254,7 +292,8 @@ def config3(args, pkg, synth, mdmod, dev):
     torch.manual_seed(0)
     model = pkg.StudentForceField().to(dev).eval()
     box = synth.water_box(10, seed=3)
-    steps = args.steps if args.steps != 500 else 20000
+    steps = args.steps if args.steps is not None else 20000
+    args.warmup = args.warmup if args.warmup is not None else 20
     md = mdmod.BatchedNVE(model, [box], seed=3, max_steps=steps + args.warmup + 1,
                           streams=args.streams)
     sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
@@ -321,8 +360,9 @@ def config3(args, pkg, synth, mdmod, dev):
                         "avg_temperature_K": float(s["avg_temperature"][0])},
                    pressure_eV_per_A3=float(-np.trace(W.numpy()[0]) / (3 * vol)),
                    cpu_baseline={"value": cpu_rate, "unit": "atom-evals/s (E+F only)",
-                                 "cores": torch.get_num_threads(), "kind": "port",
-                                 "sample": f"{nrep} float32 E+F evaluations of the final frame"})
+                                 "kind": "port",
+                                 "sample": f"{nrep} float32 E+F evaluations of the final frame",
+                                 **cpu_fields()})
 
 
 def config4(args, pkg, synth, dev):
@@ -356,8 +396,8 @@ def config4(args, pkg, synth, dev):
     gref = torch.autograd.grad(Lref, [sd[k] for k in names], allow_unused=True)
     rel = max(((grads[k].double().cpu() - g).norm() / (g.norm() + 1e-30)).item()
               for k, g in zip(names, gref) if g is not None)
-    steps = args.steps if args.steps != 500 else 50
-    for _ in range(args.warmup if args.warmup != 20 else 3):
+    steps = args.steps if args.steps is not None else 50
+    for _ in range(args.warmup if args.warmup is not None else 3):
         tr.step(Zt, Pt, sizes, Elt, Flt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -383,10 +423,10 @@ def config4(args, pkg, synth, dev):
                    n_atoms=n, train_steps_per_s=steps / wall,
                    accuracy={"max_rel_grad_err_vs_fp64_autograd": rel,
                              "loss_fp32": float(loss), "loss_fp64": Lref.item()},
-                   cpu_baseline={"value": n / cpu_s, "unit": "atoms/s", "cores":
-                                 torch.get_num_threads(), "kind": "port",
+                   cpu_baseline={"value": n / cpu_s, "unit": "atoms/s", "kind": "port",
                                  "sample": "one float32 autograd double-backward step of the "
-                                           "oracle on the same batch"})
+                                           "oracle on the same batch",
+                                 **cpu_fields()})
 
 
 def config5(args, pkg, synth, mdmod, dev):
@@ -458,14 +498,21 @@ def main():
     ap.add_argument("--mts-inner", type=int, default=4,
                     help="inner 0.5 fs steps per outer step (config 5)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20 for the batched workload, the workload's "
+                         "own length for the others)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 5)")
     ap.add_argument("--molecules", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--steps-per-graph", type=int, default=10)
-    ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="MD steps per molecule of the CPU baseline (default 10; 1000 for the "
+                         "single molecule)")
+    ap.add_argument("--accuracy-sample", type=int, default=0,
+                    help="molecules checked against the float64 oracle (0: all)")
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=20)
@@ -492,6 +539,9 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    args.steps = args.steps if args.steps is not None else 20
+    args.warmup = args.warmup if args.warmup is not None else 5
+    args.cpu_steps = args.cpu_steps if args.cpu_steps is not None else 10
     torch.manual_seed(0)
     model = pkg.StudentForceField(hidden_dim=128, num_interactions=3, num_rbf=20, cutoff=5.0)
     model = model.to(dev)
@@ -548,9 +598,14 @@ def main():
         f"E={E_edges}, P={P_pairs})")
 
     summ = md.summary()
+    edges_mean, replay_same = window_edges(model, mols, args, md)
+    log(f"edges over the timed window: mean {edges_mean:.0f} (deterministic replay, "
+        f"bit-identical trajectory: {replay_same})")
     sd_cpu = {k: v.detach().cpu() for k, v in model.state_dict().items()}
-    df, de = accuracy(sd_cpu, md, sample=16)
-    log(f"accuracy vs FP64 oracle: max|dF|={df:.3e} eV/A, max|dE|/N={de:.3e} eV/atom")
+    acc_n = args.accuracy_sample or len(mols)
+    df, de = accuracy(sd_cpu, md, sample=acc_n)
+    log(f"accuracy vs FP64 oracle ({acc_n} molecules): max|dF|={df:.3e} eV/A, "
+        f"max|dE|/N={de:.3e} eV/atom")
 
     flops = {
         "filter_fwd": P0 * 2 * (R * H + 3 * H * H),
@@ -592,22 +647,22 @@ def main():
     if not args.skip_baselines:
         sd_dev = {k: v.detach() for k, v in model.state_dict().items()}
         sample = mols[:args.plain_sample]
-        log(f"plain eager GPU path on {len(sample)} molecules")
-        plain_rate = plain_md_rate(sd_dev, H, L, sample, synth.masses_of, 0.5 * synth.FS, 2, dev)
+        log(f"plain eager GPU path on {len(sample)} molecules x {args.plain_steps} steps")
+        plain_rate = plain_md_rate(sd_dev, H, L, sample, synth.masses_of, 0.5 * synth.FS,
+                                   args.plain_steps, dev)
         plain = {"value": plain_rate, "unit": "atom-steps/s",
-                 "sample": f"first {len(sample)} of the molecules, 2 MD steps each",
+                 "sample": f"{len(sample)} molecules, {args.plain_steps} MD steps each",
                  "speedup": value / plain_rate}
         log(f"plain: {plain_rate:.4g} atom-steps/s -> batched/plain = {value / plain_rate:.1f}x")
         csample = mols[:args.cpu_sample]
         log(f"cpu baseline on {len(csample)} molecules x {args.cpu_steps} steps")
         cpu_rate = cpu_baseline(sd_cpu, csample, synth.masses_of, 0.5 * synth.FS, args.cpu_steps)
-        cpu = {"value": cpu_rate, "unit": "atom-steps/s", "cores": torch.get_num_threads(),
-               "kind": "port",
-               "sample": f"first {len(csample)} molecules of the workload, {args.cpu_steps} MD "
-                         "steps each, float32 PyTorch-eager restatement of the reference "
-                         "(dense neighbour search, autograd forces, Velocity Verlet)",
-               "cpu_count": os.cpu_count(),
-               "affinity": len(os.sched_getaffinity(0))}
+        cpu = {"value": cpu_rate, "unit": "atom-steps/s", "kind": "port",
+               "sample": f"{len(csample)} molecules of the workload, {args.cpu_steps} MD "
+                         "steps each, one molecule at a time, float32 PyTorch-eager "
+                         "restatement of the reference (dense neighbour search, autograd "
+                         "forces, Velocity Verlet)",
+               **cpu_fields()}
         log(f"cpu: {cpu_rate:.4g} atom-steps/s on {torch.get_num_threads()} threads")
 
     drift = summ["energy_drift_per_atom"]
@@ -620,9 +675,10 @@ def main():
                                "hipGraph NVE Velocity Verlet dt 0.5 fs",
                    "n_atoms": n_atoms, "steps_per_graph": args.steps_per_graph,
                    "edges_t0": E0, "pairs_t0": P0, "edges_window_start": E_start,
-                   "edges_window_end": E_edges,
+                   "edges_window_end": E_edges, "edges_window_mean": edges_mean,
                    "note": "random-init forces are not restoring: molecules expand during "
-                           "the run and the edge count falls over the window"},
+                           "the run and the edge count falls over the window; the mean "
+                           "comes from a deterministic replay of the same steps"},
         "static_ef_t0": {"ms": ef_s * 1e3, "atom_evals_per_s": n_atoms / ef_s,
                          "streams": args.streams,
                          "one_stream_ms": ef_other * 1e3 if ef_other else None},
@@ -631,7 +687,8 @@ def main():
                                           for k, v in tuning["timings_us"].items()}}
                      if tuning else None),
         "accuracy": {"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de,
-                     "sample_molecules": 16},
+                     "sample_molecules": acc_n},
+        "replay_bit_identical": replay_same,
         "plain": plain,
         "nve": {"max_abs_drift_eV_per_atom": float(np.abs(drift).max()),
                 "steps": int(summ["n_steps"])},

@@ -16,7 +16,7 @@ class Atoms:
         self.positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
         self.cell = np.zeros((3, 3)) if cell is None else np.asarray(cell, dtype=np.float64)
         self.pbc = np.broadcast_to(np.asarray(pbc, dtype=bool), (3,)).copy()
-        self.masses = (np.array([_MASSES[int(z)] for z in self.numbers]) if masses is None
+        self.masses = (np.array([_MASSES.get(int(z), 1.0) for z in self.numbers]) if masses is None
                        else np.asarray(masses, dtype=np.float64).copy())
         self.momenta = (np.zeros((len(self.numbers), 3)) if momenta is None
                         else np.asarray(momenta, dtype=np.float64).reshape(-1, 3).copy())

@@ -244,3 +244,82 @@ def test_harness_overhead_and_trajectory_memory(gpu, ckpt):
     td = h.trajectory_data
     assert sys.getsizeof(td["positions"]) + sys.getsizeof(td["forces"]) < 1_000_000
     assert not np.any(np.isnan(h.get_trajectory_array("total_energy")))
+
+
+def test_torchscript_export_matches_oracle(gpu, ckpt, tmp_path):
+    """The exported TorchScript file's energy and forces (forces through
+    autograd of the loaded module, as export_to_torchscript.py's users take
+    them) against the FP64 oracle on a molecule of another size than the
+    tracing one."""
+    note("TorchScript export vs oracle")
+    jit = pkg_mod("jit")
+    path, sd = ckpt
+    model = pkg().StudentForceField.load(path, device="cuda")
+    out = tmp_path / "student_jit.pt"
+    jit.export_to_torchscript(model, out, num_atoms=50, device="cuda")
+    m = jit.load_torchscript(out, "cuda")
+    atoms = _mol(64, 31)
+    x32 = atoms.positions.astype(np.float32)
+    z = torch.tensor(atoms.numbers, device=gpu)
+    x = torch.tensor(x32, device=gpu, requires_grad=True)
+    E = m(z, x)
+    F = -torch.autograd.grad(E, x)[0]
+    Eref, Fref = R.energy_and_forces(sd, atoms.numbers, x32.astype(np.float64))
+    assert abs(E.item() - Eref.item()) / len(atoms) < 1e-5
+    assert np.abs(F.cpu().numpy() - Fref.numpy()).max() < 1e-4
+
+
+def test_calculate_batch_stress_matches_oracle(gpu, ckpt):
+    """calculate_batch with enable_stress on two periodic boxes of different
+    cells and a molecule: each periodic structure's stress is sym(virial)/V
+    of the FP64 oracle's virial, the molecule's is zeros (reference
+    ase_calculator.py:590-645, 810-812)."""
+    note("calculate_batch stress: two boxes + molecule")
+    path, sd = ckpt
+    synth = pkg_mod("synth")
+    b1 = synth.water_box(4, seed=3)
+    b2 = synth.water_box(4, seed=8)
+    cell2 = b2.cell.copy()
+    cell2[1, 0] += 1.5                         # skewed second box
+    atoms = [Atoms(b1.numbers, b1.positions, b1.cell, True), _mol(20, 3),
+             Atoms(b2.numbers, b2.positions, cell2, True)]
+    calc = pkg().StudentForceFieldCalculator(path, enable_stress=True)
+    res = calc.calculate_batch(atoms, properties=["energy", "forces", "stress"])
+    assert np.all(res[1]["stress"] == 0)
+    for a, r in ((atoms[0], res[0]), (atoms[2], res[2])):
+        P32 = a.positions.astype(np.float32)
+        c32 = a.cell.astype(np.float32)
+        rp, cl, sh = nl_ref.neighbor_csr(P32, np.array([0, len(P32)]), 5.0, c32[None],
+                                         np.ones((1, 3), bool))
+        ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(len(P32)), np.diff(rp))])
+        E, F, W = R.energy_and_forces(sd, a.numbers, P32.astype(np.float64), edge_index=ei,
+                                      shifts=sh, cell=c32.astype(np.float64), virial=True)
+        W = W.numpy()[0]
+        s = 0.5 * (W + W.T) / abs(np.linalg.det(a.cell))
+        want = np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
+        assert abs(r["energy"] - E.item()) / len(a) < 1e-5
+        assert np.abs(r["forces"] - F.numpy()).max() < 1e-4
+        assert np.abs(r["stress"] - want).max() < 1e-5 + 1e-3 * np.abs(want).max()
+
+
+def test_nve_harness_continues_from_the_atoms_velocities(gpu, ckpt):
+    """run_simulation(initialize_velocities=False) starts from the velocities
+    already on the atoms (as ASE's VelocityVerlet does), and after a run the
+    atoms hold the final positions and momenta: their kinetic energy is the
+    last frame's."""
+    note("NVEMDHarness: continue from the atoms' velocities")
+    path, _ = ckpt
+    calc = pkg().StudentForceFieldCalculator(path)
+    atoms = _mol(18, 6)
+    rng = np.random.default_rng(2)
+    atoms.set_velocities(rng.normal(size=(18, 3)) * 0.01)
+    v0 = atoms.get_velocities()
+    h = pkg().NVEMDHarness(atoms, calc, temperature=300.0, seed=0)
+    out = h.run_simulation(20, initialize_velocities=False)
+    td = out["trajectory_data"]
+    np.testing.assert_allclose(td["velocities"][0], v0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(h.atoms.get_positions(), td["positions"][-1], rtol=0, atol=0)
+    assert abs(h.atoms.get_kinetic_energy() - td["kinetic_energy"][-1]) < 1e-9
+    out2 = h.run_simulation(10, initialize_velocities=False)
+    td2 = out2["trajectory_data"]
+    np.testing.assert_allclose(td2["velocities"][21], td["velocities"][20], rtol=0, atol=1e-12)

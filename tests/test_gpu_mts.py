@@ -83,3 +83,76 @@ def test_train_student_on_md_frames(gpu):
     hist = mts.train_student(student, frames, steps=30, batch_molecules=8, lr=3e-3)
     note(f"loss {hist[0]:.4f} -> {np.mean(hist[-5:]):.4f}")
     assert np.mean(hist[-5:]) < 0.5 * hist[0]
+
+
+def _oracle_forces(sd, Z, x, batch):
+    """FP64 oracle forces at the float32 positions the device model sees."""
+    from oracle import painn_ref as R
+    x32 = x.astype(np.float32).astype(np.float64)
+    _, F = R.energy_and_forces(sd, Z, x32, cutoff=5.0, batch=batch, n_structures=int(batch.max()) + 1)
+    return F.numpy()
+
+
+def test_mts_matches_host_respa_on_oracle_forces(gpu):
+    """r-RESPA with a distinct compact inner model (H32/L2/R16) and the full
+    model (H128/L3/R20) outside, on the first molecules of the 512-molecule
+    seed-5 workload: 3 outer x 4 inner steps on the device equal an r-RESPA
+    written here on the host, driven by FP64 oracle forces of both models
+    (p += Dt/2 (F_full - F_fast); 4 x [p += dt/2 F_fast; x += dt p/m;
+    p += dt/2 F_fast]; p += Dt/2 (F_full - F_fast))."""
+    note("MTS vs host r-RESPA on FP64 oracle forces (seed-5 molecules)")
+    synth = pkg_mod("synth")
+    mts = pkg_mod("mts")
+    full = _model()
+    fast = _model(32, 2, 16, seed=3)
+    mols = synth.molecule_set(512, 10, 100, seed=5)[:3]
+    Z, X, mol_ptr = synth.flatten(mols)
+    batch = np.repeat(np.arange(len(mols)), np.diff(mol_ptr))
+    m = synth.masses_of(Z)[:, None]
+    rng = np.random.Generator(np.random.PCG64(9))
+    P = np.concatenate([synth.maxwell_boltzmann(s.numbers, s.positions, 300.0, rng) for s in mols])
+    a = mts.MTSNVE(full, fast, mols, momenta=P, threshold=1e9, max_outer=4)
+    a.run(3, chunk=3, graph=False)
+    sd_full = {k: v.detach().cpu() for k, v in full.state_dict().items()}
+    sd_fast = {k: v.detach().cpu() for k, v in fast.state_dict().items()}
+    dt = 0.5 * synth.FS
+    n_inner = 4
+    Dt = n_inner * dt
+    x, p = X.copy(), P.copy()
+    for _ in range(3):
+        slow = _oracle_forces(sd_full, Z, x, batch) - _oracle_forces(sd_fast, Z, x, batch)
+        p = p + 0.5 * Dt * slow
+        f = _oracle_forces(sd_fast, Z, x, batch)
+        for _ in range(n_inner):
+            p = p + 0.5 * dt * f
+            x = x + dt * p / m
+            f = _oracle_forces(sd_fast, Z, x, batch)
+            p = p + 0.5 * dt * f
+        slow = _oracle_forces(sd_full, Z, x, batch) - f
+        p = p + 0.5 * Dt * slow
+    dx = np.abs(a.x.cpu().numpy() - x).max()
+    dp = np.abs(a.p.cpu().numpy() - p).max()
+    note(f"|dx|={dx:.2e} A  |dp|={dp:.2e}")
+    assert dx <= 1e-6 and dp <= 1e-4
+    # the inner model really differs from the outer one
+    assert np.abs(_oracle_forces(sd_full, Z, X, batch) - _oracle_forces(sd_fast, Z, X, batch)).max() > 0.1
+
+
+def test_mts_full_seed5_workload_graph_equals_eager(gpu):
+    """The 512-molecule seed-5 MTS workload itself: two chunks of 5 outer
+    steps replayed from HIP graphs equal the eager loop bit for bit, and the
+    ledger stays finite."""
+    note("MTS 512 molecules seed 5: graph vs eager, 10 outer steps")
+    synth = pkg_mod("synth")
+    mts = pkg_mod("mts")
+    full = _model()
+    fast = _model(32, 2, 16, seed=3)
+    mols = synth.molecule_set(512, 10, 100, seed=5)
+    a = mts.MTSNVE(full, fast, mols, seed=5, threshold=1e9, max_outer=12)
+    b = mts.MTSNVE(full, fast, mols, seed=5, threshold=1e9, max_outer=12)
+    a.run(10, chunk=5, graph=True)
+    b.run(10, chunk=5, graph=False)
+    assert a.N > 25000
+    assert torch.equal(a.x, b.x) and torch.equal(a.p, b.p)
+    ea, ka = a.energies()
+    assert np.array_equal(ea, b.energies()[0]) and np.isfinite(ea).all() and np.isfinite(ka).all()

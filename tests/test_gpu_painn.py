@@ -653,3 +653,67 @@ def test_batch_with_an_empty_structure(gpu):
     assert tuple(E.shape) == (3,)
     assert E[1].item() == 0.0
     assert abs(E[0].item() - E0.item()) < 1e-6 and abs(E[2].item() - E2.item()) < 1e-6
+
+
+def test_periodic_water_full_model_energy_forces_virial(gpu):
+    """The periodic bulk workload's model at its own size class: H128/L3/R20
+    (seed-0 weights) on a 7^3-water box (1029 atoms, 21.8 A, cell-list
+    neighbour search): energy, forces and virial against the FP64 oracle on
+    the oracle's own minimum-image edges."""
+    note("periodic water 7^3 H128/L3/R20 E/F/virial")
+    synth = pkg_mod("synth")
+    box = synth.water_box(7, seed=3)
+    model = _model(128, 3, 20, seed=0)
+    P32 = box.positions.astype(np.float32)
+    e64, f, vir = model.energy_forces_virial(torch.tensor(box.numbers, device=gpu),
+                                             torch.tensor(P32, device=gpu),
+                                             torch.tensor(box.cell), torch.tensor(box.pbc))
+    rp, cl, sh = nl_ref.neighbor_csr(P32, np.array([0, len(P32)]), 5.0, box.cell[None],
+                                     box.pbc[None])
+    ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(len(P32)), np.diff(rp))])
+    sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    E, F, W = R.energy_and_forces(sd, box.numbers, P32.astype(np.float64), cutoff=5.0,
+                                  edge_index=ei, shifts=sh,
+                                  cell=box.cell.astype(np.float32).astype(np.float64),
+                                  virial=True)
+    n = len(P32)
+    e_err = abs(e64.item() - E.item()) / n
+    f_err = np.abs(f.cpu().numpy() - F.numpy()).max()
+    w_err = np.abs(vir.cpu().numpy()[0] - W.numpy()[0]).max()
+    note(f"N={n} E={ei.shape[1]} |dE|/N={e_err:.2e} |dF|={f_err:.2e} |dW|={w_err:.2e}")
+    assert n == 1029 and ei.shape[1] > 40 * n
+    assert e_err < E_TOL and f_err < F_TOL
+    assert w_err < 1e-3 * max(1.0, np.abs(W.numpy()).max())
+
+
+def test_forward_with_analytical_forces_matches_oracle(gpu):
+    """forward_with_analytical_forces (student_model.py:797-932): energy and
+    forces of one molecule and of a shuffled three-molecule batch against the
+    FP64 oracle, and equal to predict_energy_and_forces bit for bit."""
+    note("forward_with_analytical_forces single + batched")
+    model = _model(128, 3, 20, seed=0).eval()
+    sd = {k: v.detach().cpu() for k, v in model.state_dict().items()}
+    mols = _mols(3, 15, 60, seed=23)
+    m0 = mols[0]
+    z = torch.tensor(m0.numbers, device=gpu)
+    x32 = m0.positions.astype(np.float32)
+    x = torch.tensor(x32, device=gpu)
+    E, F = model.forward_with_analytical_forces(z, x)
+    Eref, Fref = R.energy_and_forces(sd, m0.numbers, x32.astype(np.float64))
+    assert abs(E.item() - Eref.item()) / len(z) < E_TOL
+    assert np.abs(F.cpu().numpy() - Fref.numpy()).max() < F_TOL
+    E2, F2 = model.predict_energy_and_forces(z, x)
+    assert torch.equal(F, F2) and E.item() == E2.item()
+    synth = pkg_mod("synth")
+    Z, P, mol_ptr = synth.flatten(mols)
+    b = np.repeat(np.arange(3), np.diff(mol_ptr))
+    perm = np.random.default_rng(4).permutation(len(Z))       # atoms interleaved
+    Zp, Pp, bp = Z[perm], P[perm].astype(np.float32), b[perm]
+    Eb, Fb = model.forward_with_analytical_forces(torch.tensor(Zp, device=gpu),
+                                                  torch.tensor(Pp, device=gpu),
+                                                  batch=torch.tensor(bp, device=gpu))
+    Eref, Fref = R.energy_and_forces(sd, Zp, Pp.astype(np.float64), batch=bp)
+    sizes = np.bincount(bp)
+    assert Eb.shape == (3,)
+    assert np.max(np.abs(Eb.detach().cpu().numpy() - Eref.numpy()) / sizes) < E_TOL
+    assert np.abs(Fb.cpu().numpy() - Fref.numpy()).max() < F_TOL
