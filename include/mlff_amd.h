@@ -184,6 +184,35 @@ int mlff_energy_forces(mlff_plan* plan, void* stream, const float* pos_dev,
                        double* energy_dev, float* forces_dev, double* virial_dev,
                        float* atom_energy_dev);
 
+/* ---------------------------------------------------------------------------
+ * Evaluation that allocates nothing.  No counterpart in the reference: this
+ * is for callers that cannot allocate device memory or pass a workspace --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same result as mlff_plan_create + mlff_plan_bind +
+ * mlff_energy_forces with the same config (one stream, bit-identical), but
+ * the workspace is a static device array of MLFF_STATIC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends; a batch whose
+ * workspace (mlff_plan_workspace_bytes of the same config) exceeds it is
+ * rejected with MLFF_ERR_WORKSPACE.  No hipMalloc/hipFree, no stream or event
+ * creation, no host synchronisation: everything is enqueued on `stream`.
+ *
+ * Every call on one GPU shares that GPU's scratch, which keeps the last
+ * call's contents: calls that overlap on one GPU (different streams, or
+ * different threads) race and must be ordered by the caller.  Periodic
+ * structures use the brute-force minimum-image search (cfg->cell_capacity is
+ * ignored).  counts_dev [MLFF_COUNT_SLOTS] int32 (caller-owned output) is
+ * cleared by the call and receives the edge and pair counts and the overflow
+ * flag of this evaluation.  The weights' bf16 planes are re-split from
+ * weights_dev on every call.
+ * ------------------------------------------------------------------------- */
+#define MLFF_STATIC_SCRATCH_BYTES (64ll << 20)
+int mlff_energy_forces_static(void* stream, const mlff_config* cfg, const float* weights_dev,
+                              const int64_t* numbers_dev, const int32_t* mol_ptr_dev,
+                              const int32_t* atom_mol_dev, const float* cells_dev,
+                              const uint8_t* pbc_dev, const float* pos_dev, double* energy_dev,
+                              float* forces_dev, double* virial_dev, float* atom_energy_dev,
+                              int32_t* counts_dev);
+
 /* Re-launch one stage of the last mlff_energy_forces call on its workspace
  * contents (for timing individual kernels and for the launch-shape tuner;
  * no counterpart in the reference).  Stages are idempotent: they recompute

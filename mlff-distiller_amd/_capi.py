@@ -40,8 +40,11 @@ EXPORTS = (
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
     "mlff_md_bind_frames", "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
     "mlff_launch_stage", "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
-    "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams",
+    "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams", "mlff_energy_forces_static",
 )
+
+# bytes of the static device scratch behind mlff_energy_forces_static
+STATIC_SCRATCH_BYTES = 64 << 20
 
 STAGE_NEIGHBOR, STAGE_FILTER_FWD, STAGE_MESSAGE_FWD, STAGE_UPDATE_FWD = 0, 1, 2, 3
 STAGE_READOUT, STAGE_UPDATE_BWD, STAGE_FILTER_BWD, STAGE_MESSAGE_BWD, STAGE_FORCES = 4, 5, 6, 7, 8
@@ -130,6 +133,8 @@ def _declare(lib):
     lib.mlff_mts_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.mlff_mts_graph_run.restype = ctypes.c_int
     lib.mlff_mts_graph_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.mlff_energy_forces_static.restype = ctypes.c_int
+    lib.mlff_energy_forces_static.argtypes = [vp, ctypes.POINTER(Config)] + [vp] * 12
 
 
 def lib():

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "common.hpp"
@@ -217,6 +218,24 @@ hipError_t acquire_streams(mlff_plan* pl) {
   return hipSuccess;
 }
 
+// Workgroups of the persistent pair kernels: the caller's choice, else three
+// rounds of two resident blocks per CU (cus < 0: ask the current device).
+int persistent_grid(int blocks, int cus) {
+  if (blocks <= 0) {
+    if (cus <= 0) {
+      int dev = 0;
+      cus = 256;
+      if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+          cus = prop.multiProcessorCount;
+      }
+    }
+    blocks = 6 * cus;
+  }
+  return (blocks + 7) / 8 * 8;
+}
+
 template <typename T>
 T* segp(mlff_plan* pl, void* ws, int k) {
   return pl->seg[k].bytes ? reinterpret_cast<T*>(static_cast<char*>(ws) + pl->seg[k].off) : nullptr;
@@ -293,17 +312,7 @@ int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
   pl->cfg = *cfg;
   pl->mw = mw;
   carve(pl);
-  int blocks = cfg->persistent_blocks;
-  if (blocks <= 0) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-      hipDeviceProp_t prop;
-      if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-        cus = prop.multiProcessorCount;
-    }
-    blocks = 6 * cus;   // the persistent pair kernels: three rounds of two resident blocks per CU
-  }
-  pl->buf.blocks = (blocks + 7) / 8 * 8;
+  pl->buf.blocks = persistent_grid(cfg->persistent_blocks, -1);
   pl->n_streams = 2;
   pl->stream_dev = -1;
   *out = pl;
@@ -323,26 +332,26 @@ int mlff_plan_destroy(mlff_plan* pl) {
 
 int64_t mlff_plan_workspace_bytes(const mlff_plan* pl) { return pl ? pl->ws_bytes : -1; }
 
-int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weights_dev,
-                   const int64_t* numbers_dev, const int32_t* mol_ptr_dev,
-                   const int32_t* atom_mol_dev, const float* cells_dev, const uint8_t* pbc_dev) {
-  if (!pl) { set_error("mlff_plan_bind: null plan"); return MLFF_ERR_INVALID; }
-  if (!ws || ws_bytes < pl->ws_bytes) {
-    set_error("mlff_plan_bind: workspace of %lld bytes, need %lld", (long long)ws_bytes,
-              (long long)pl->ws_bytes);
-    return MLFF_ERR_WORKSPACE;
-  }
+}  // extern "C"
+
+namespace {
+
+// Point the plan's buffers and neighbour-list arguments at the workspace
+// segments and the per-batch constants (no device work).
+int bind_buffers(mlff_plan* pl, const char* who, void* ws, const float* weights_dev,
+                 const int64_t* numbers_dev, const int32_t* mol_ptr_dev,
+                 const int32_t* atom_mol_dev, const float* cells_dev, const uint8_t* pbc_dev) {
   if (!weights_dev || !numbers_dev || !mol_ptr_dev || !atom_mol_dev) {
-    set_error("mlff_plan_bind: null pointer");
+    set_error("%s: null pointer", who);
     return MLFF_ERR_INVALID;
   }
   // the kernels read and write these with 16-byte vector accesses
   if ((reinterpret_cast<uintptr_t>(ws) | reinterpret_cast<uintptr_t>(weights_dev)) & 15) {
-    set_error("mlff_plan_bind: workspace and weights must be 16-byte aligned");
+    set_error("%s: workspace and weights must be 16-byte aligned", who);
     return MLFF_ERR_INVALID;
   }
   if (pl->cfg.any_periodic && (!cells_dev || !pbc_dev)) {
-    set_error("mlff_plan_bind: periodic plan needs cells and pbc");
+    set_error("%s: periodic plan needs cells and pbc", who);
     return MLFF_ERR_INVALID;
   }
   Buffers& b = pl->buf;
@@ -395,6 +404,26 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   a.atom_cell = segp<int32_t>(pl, ws, S_ATOMCELL);
   a.edge_tmp_shift = segp<int32_t>(pl, ws, S_TMPSHIFT);
   pl->grid_dev = segp<int4>(pl, ws, S_GRID);
+  return MLFF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weights_dev,
+                   const int64_t* numbers_dev, const int32_t* mol_ptr_dev,
+                   const int32_t* atom_mol_dev, const float* cells_dev, const uint8_t* pbc_dev) {
+  if (!pl) { set_error("mlff_plan_bind: null plan"); return MLFF_ERR_INVALID; }
+  if (!ws || ws_bytes < pl->ws_bytes) {
+    set_error("mlff_plan_bind: workspace of %lld bytes, need %lld", (long long)ws_bytes,
+              (long long)pl->ws_bytes);
+    return MLFF_ERR_WORKSPACE;
+  }
+  int rc = bind_buffers(pl, "mlff_plan_bind", ws, weights_dev, numbers_dev, mol_ptr_dev,
+                        atom_mol_dev, cells_dev, pbc_dev);
+  if (rc != MLFF_OK) return rc;
+  Buffers& b = pl->buf;
   hipError_t e = hipMemset(b.counts, 0, 4 * MLFF_COUNT_SLOTS);
   if (e != hipSuccess) return hip_fail(e, "mlff_plan_bind");
   e = acquire_streams(pl);
@@ -756,3 +785,122 @@ int mlff_mts_graph_run(mlff_plan* pl, void* stream, int outer_per_graph, int n_l
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Evaluation without device allocation: the workspace is a file-scope device
+// array, one copy per GPU, held until the process ends.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[MLFF_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+// Per device: the device address of `scratch` (the host-side name is only a
+// stand-in and must never reach a kernel) and the CU count.
+struct StaticScratch {
+  std::mutex mu;
+  void* ptr[kMaxDevices] = {};
+  int cus[kMaxDevices] = {};
+};
+StaticScratch g_static;
+
+int static_scratch(int* dev_out, void** ptr_out, int* cus_out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "mlff_energy_forces_static: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("mlff_energy_forces_static: device %d beyond the %d this library tracks", dev,
+              kMaxDevices);
+    return MLFF_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_static.mu);
+  if (!g_static.ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "mlff_energy_forces_static: hipGetSymbolAddress");
+    if (!p) {
+      set_error("mlff_energy_forces_static: no device address for the scratch array");
+      return MLFF_ERR_HIP;
+    }
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+    g_static.cus[dev] = cus;
+    g_static.ptr[dev] = p;
+  }
+  *dev_out = dev;
+  *ptr_out = g_static.ptr[dev];
+  *cus_out = g_static.cus[dev];
+  return MLFF_OK;
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+extern "C" int mlff_energy_forces_static(void* stream, const mlff_config* cfg,
+                                         const float* weights_dev, const int64_t* numbers_dev,
+                                         const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev,
+                                         const float* cells_dev, const uint8_t* pbc_dev,
+                                         const float* pos_dev, double* energy_dev,
+                                         float* forces_dev, double* virial_dev,
+                                         float* atom_energy_dev, int32_t* counts_dev) {
+  static const char* who = "mlff_energy_forces_static";
+  if (!cfg || !pos_dev || !energy_dev || !counts_dev) {
+    set_error("%s: null argument", who);
+    return MLFF_ERR_INVALID;
+  }
+  if (cfg->n_atoms < 1 || cfg->n_structures < 1 || cfg->edge_capacity < 0 ||
+      cfg->pair_capacity < 0 || !(cfg->cutoff > 0.f)) {
+    set_error("%s: invalid sizes", who);
+    return MLFF_ERR_INVALID;
+  }
+  if (reinterpret_cast<uintptr_t>(counts_dev) & 3) {
+    set_error("%s: counts must be 4-byte aligned", who);
+    return MLFF_ERR_INVALID;
+  }
+  // the plan lives on this call's stack: nothing to allocate, nothing kept
+  mlff_plan pl;
+  memset(&pl, 0, sizeof(pl));
+  pl.cfg = *cfg;
+  pl.cfg.cell_capacity = 0;       // periodic structures use the brute-force image search
+  int rc = build_layout(cfg->hidden_dim, cfg->num_interactions, cfg->num_rbf, cfg->max_z,
+                        &pl.mw, nullptr, 0, nullptr);
+  if (rc != MLFF_OK) return rc;
+  carve(&pl);
+  if (pl.ws_bytes > (int64_t)MLFF_STATIC_SCRATCH_BYTES) {
+    set_error("%s: this batch needs %lld bytes of scratch; the static scratch holds %lld "
+              "(use mlff_energy_forces with a workspace)", who, (long long)pl.ws_bytes,
+              (long long)MLFF_STATIC_SCRATCH_BYTES);
+    return MLFF_ERR_WORKSPACE;
+  }
+  int dev = 0, cus = 0;
+  void* ws = nullptr;
+  rc = static_scratch(&dev, &ws, &cus);
+  if (rc != MLFF_OK) return rc;
+  pl.buf.blocks = persistent_grid(cfg->persistent_blocks, cus);
+  pl.n_streams = 1;
+  pl.stream_dev = -1;
+  rc = bind_buffers(&pl, who, ws, weights_dev, numbers_dev, mol_ptr_dev, atom_mol_dev, cells_dev,
+                    pbc_dev);
+  if (rc != MLFF_OK) return rc;
+  // the caller's counts replace the scratch's slots, cleared by a kernel (not
+  // a memset node, so the call can be captured into a HIP graph)
+  pl.buf.counts = counts_dev;
+  pl.nl.counts = counts_dev;
+  pl.buf.aux = nullptr;
+  pl.buf.ev = pl.ev;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fill_i32_kernel<<<1, 64, 0, s>>>(counts_dev, MLFF_COUNT_SLOTS, 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, who);
+  e = split_filter_weights_async(pl.buf, s);
+  if (e != hipSuccess) return hip_fail(e, "mlff_energy_forces_static: bf16 weight planes");
+  return run_ef(&pl, s, pos_dev, energy_dev, forces_dev, virial_dev, atom_energy_dev);
+}

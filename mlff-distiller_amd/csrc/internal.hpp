@@ -85,6 +85,8 @@ __host__ __device__ inline int64_t wbf_f2w(int H, int l) { return (int64_t)l * 3
 __host__ __device__ inline int64_t wbf_elems(int H, int L) { return (int64_t)L * 3 * wbf_plane(H); }
 // Split the filter weights into their bf16 planes (once per bind; synchronous).
 hipError_t split_filter_weights(const Buffers& b);
+// The same split enqueued on stream s (no synchronisation).
+hipError_t split_filter_weights_async(const Buffers& b, hipStream_t s);
 hipError_t launch_forward(const Buffers& b, hipStream_t s);
 hipError_t launch_backward(const Buffers& b, hipStream_t s, float* forces, bool want_virial);
 // One model stage (mlff_stage 1..8) of block `layer`, on the workspace as the

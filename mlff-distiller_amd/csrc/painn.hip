@@ -1211,6 +1211,12 @@ hipError_t split_filter_weights(const Buffers& b) {
   return hipDeviceSynchronize();
 }
 
+hipError_t split_filter_weights_async(const Buffers& b, hipStream_t s) {
+  if (b.wbf == nullptr || b.L < 1) return hipSuccess;
+  split_filter_weights_kernel<<<256, 256, 0, s>>>(b.w, *b.mw, b.wbf);
+  return hipGetLastError();
+}
+
 hipError_t launch_stage(const Buffers& b, hipStream_t s, int stage, int layer) {
   if (layer < 0 || layer >= b.L) return hipErrorInvalidValue;
   switch (b.H) {

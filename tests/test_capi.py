@@ -22,6 +22,28 @@ def test_header_and_binding_agree():
     assert _declared_symbols() == set(C.EXPORTS)
 
 
+def _declared_arity():
+    """Parameter count of every entry point in the header."""
+    text = open(os.path.join(ROOT, "include", "mlff_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"(mlff_\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    """Every ctypes declaration lists as many arguments as the C prototype.
+    ctypes passes surplus arguments with default int conversion, which
+    truncates a 64-bit device pointer: a missing argtype is a wrong address."""
+    lib = C.lib()
+    arity = _declared_arity()
+    assert set(arity) == set(C.EXPORTS)
+    for name, n in arity.items():
+        assert len(getattr(lib, name).argtypes) == n, (name, n)
+
+
 def test_library_exports_every_symbol():
     lib = C.lib()
     for name in C.EXPORTS:
@@ -110,3 +132,25 @@ def test_weight_packing_places_tensors():
             view = blob[s.offset:s.offset + s.ld]
             assert torch.equal(view[:8], sd["energy_head.4.weight"][0])
             assert torch.all(view[8:] == 0)
+
+
+def test_static_entry_rejects_before_touching_the_device():
+    """mlff_energy_forces_static checks its arguments and the scratch bound
+    on the host: null arguments and a batch whose workspace exceeds the
+    static scratch return errors without any device call (the non-null
+    pointers below are never dereferenced)."""
+    lib = C.lib()
+    fake = ctypes.c_void_p(1 << 20)
+    cfg = C.Config(128, 3, 20, 118, 5.0, 1000, 10, 400000, 200000, 0, 0, 0)
+    assert lib.mlff_energy_forces_static(None, None, *([fake] * 12)) == -1
+    rc = lib.mlff_energy_forces_static(None, ctypes.byref(cfg), *([fake] * 12))
+    assert rc == -4
+    msg = lib.mlff_last_error().decode()
+    assert "static scratch holds %d" % C.STATIC_SCRATCH_BYTES in msg
+    h = C.vp()
+    C.check("create", lib.mlff_plan_create(ctypes.byref(h), ctypes.byref(cfg)))
+    need = lib.mlff_plan_workspace_bytes(h)
+    lib.mlff_plan_destroy(h)
+    assert "needs %d bytes" % need in msg and need > C.STATIC_SCRATCH_BYTES
+    bad = C.Config(128, 3, 20, 118, 5.0, 0, 1, 10, 10, 0, 0, 0)
+    assert lib.mlff_energy_forces_static(None, ctypes.byref(bad), *([fake] * 12)) == -1
