@@ -350,6 +350,7 @@ def config3(args, pkg, synth, mdmod, dev):
                    "seed 0, cell-free minimum image, hipGraph NVE dt 0.5 fs",
                    edges_start=int(c[0]),
                    autotune={"persistent_blocks": int(tuning["blocks"]),
+                             "launch_shapes": list(tuning["tiles"]),
                              "evaluation_us": {str(k): round(v, 1)
                                                for k, v in tuning["timings_us"].items()}},
                    accuracy={"t0": acc0, "final": acc1},
@@ -557,6 +558,8 @@ def main():
     if tuning:
         log(f"autotune: persistent grid {tuning['blocks']} workgroups "
             f"{ {k: round(v, 1) for k, v in tuning['timings_us'].items()} }")
+        log(f"autotune: launch shapes {tuning['tiles']} "
+            f"{ {k: round(v, 1) for k, v in tuning['shape_timings_us'].items()} }")
     # E+F alone on the starting (compact) structures, and per-kernel timings there
     c0 = md.plan.read_counts()
     E0, P0 = int(c0[0]), int(c0[1])
@@ -683,8 +686,13 @@ def main():
                          "streams": args.streams,
                          "one_stream_ms": ef_other * 1e3 if ef_other else None},
         "autotune": ({"persistent_blocks": tuning["blocks"],
+                      "launch_shapes": dict(zip(("filter_fwd_pairs", "filter_bwd_pairs",
+                                                 "message_fwd_unroll", "message_bwd_unroll",
+                                                 "filter_bwd_prefetch"), tuning["tiles"])),
                       "evaluation_us": {str(k): round(v, 1)
-                                          for k, v in tuning["timings_us"].items()}}
+                                          for k, v in tuning["timings_us"].items()},
+                      "shape_evaluation_us": {k: round(v, 1)
+                                              for k, v in tuning["shape_timings_us"].items()}}
                      if tuning else None),
         "accuracy": {"max_abs_dF_eV_per_A": df, "max_abs_dE_per_atom_eV": de,
                      "sample_molecules": acc_n},

@@ -234,6 +234,21 @@ int mlff_launch_stage(mlff_plan* plan, void* stream, int stage, int layer);
 /* Set the persistent grid (workgroups) used by the pair kernels. */
 int mlff_plan_set_blocks(mlff_plan* plan, int blocks);
 
+/* Launch shapes of the pair and message kernels (for the launch-shape
+ * tuner; no counterpart in the reference, whose Triton kernels fix their
+ * block sizes, kernels/fused_rbf_cutoff.py:121-132): pairs per filter-forward
+ * tile, pairs per filter-adjoint tile, and edges in flight per row in the
+ * message forward and adjoint.  0 keeps the default; each kind has one
+ * pre-instantiated alternative (forward tile 64 or 32 for H >= 64, 128 or 64
+ * for H = 32; adjoint tile 32 or 64 for H >= 64, 64 or 32 for H = 32;
+ * message 5 or 3 and 4 or 2 edges), anything else is MLFF_ERR_UNSUPPORTED.
+ * bwd_prefetch = 1 (default adjoint tile only) makes the filter adjoint load
+ * the first gather item of its next tile before the current tile's matrix
+ * products.  Results are bit-identical across shapes (every output element
+ * sums in the same order).  Drops any captured graph of the plan. */
+int mlff_plan_set_tiles(mlff_plan* plan, int fwd_pairs, int bwd_pairs, int msg_unroll_fwd,
+                        int msg_unroll_bwd, int bwd_prefetch);
+
 /* Number of streams one evaluation uses: 2 (default) runs the pair filter
  * kernels on a stream of the plan's own, forked from and joined back into the
  * caller's stream with events, so they overlap the atom-side kernels; 1 keeps
@@ -292,6 +307,89 @@ int mlff_mts_init_forces(mlff_plan* outer, void* stream);
 int mlff_mts_run(mlff_plan* outer, void* stream, int n_outer, int fallback);
 /* Capture outer_per_graph outer steps (no fallback) once, replay n_launches times. */
 int mlff_mts_graph_run(mlff_plan* outer, void* stream, int outer_per_graph, int n_launches);
+
+/* ---------------------------------------------------------------------------
+ * Force-matching training pass.  The reference differentiates the loss
+ * wE MSE(E) + wF MSE(F) by autograd double backward
+ * (models/distillation_wrapper.py:86-91, training/trainer.py:325-328,
+ * scripts/train_compact_models.py:229-372); the package's train.py forms the
+ * same gradient explicitly (a reverse pass and its tangent along dL/dF) and
+ * runs its edge- and pair-sized parts here.  All buffers are float32 device
+ * memory (features [N][H] and [N][3][H], 16-byte aligned), graphs are a
+ * symmetric receiver CSR (row_ptr [N+1], col [E] ascending within a row)
+ * with edge_pair [E] the unordered pair (lo < hi) of each directed edge.
+ * Every sum runs in a fixed order (bit-reproducible, no atomics).
+ * hidden_dim is 32, 64 or 128.
+ * ------------------------------------------------------------------------- */
+
+/* Message block of PaiNNMessage.forward (student_model.py:357-387) on
+ * given filters W [P][3H] = (Ws, Wv1, Wv2) and pair unit vectors u [P][3]
+ * (from lo to hi):
+ *   s_out[i] = s[i] + sum_e s[j] Ws[p],
+ *   v_out[i] = v[i] + sum_e (v[j] Wv1[p] + sg u[p] (x) Wv2[p]),
+ * sg = +1 when i is the pair's lower atom, -1 otherwise.  With ds_dev non-NULL
+ * also the tangent along (ds, dv, dW, du) into ds_out, dv_out. */
+int mlff_train_message(void* stream, int hidden_dim, int n_atoms, const int32_t* row_ptr_dev,
+                       const int32_t* col_dev, const int32_t* edge_pair_dev, const float* s_dev,
+                       const float* v_dev, const float* w_dev, const float* u_dev,
+                       const float* ds_dev, const float* dv_dev, const float* dw_dev,
+                       const float* du_dev, float* s_out_dev, float* v_out_dev,
+                       float* ds_out_dev, float* dv_out_dev);
+
+/* Adjoint of the message gathers (the backward of student_model.py:359-385,
+ * index_add_ transposed): gs_out[i] = gs[i] + sum_e gs[j] Ws[p],
+ * gv_out[i] = gv[i] + sum_e gv[j] Wv1[p]; with dgs_dev non-NULL also its
+ * tangent along (dgs, dgv, dW). */
+int mlff_train_message_adjoint(void* stream, int hidden_dim, int n_atoms,
+                               const int32_t* row_ptr_dev, const int32_t* col_dev,
+                               const int32_t* edge_pair_dev, const float* gs_dev,
+                               const float* gv_dev, const float* w_dev, const float* dgs_dev,
+                               const float* dgv_dev, const float* dw_dev, float* gs_out_dev,
+                               float* gv_out_dev, float* dgs_out_dev, float* dgv_out_dev);
+
+/* Gradient w.r.t. each pair's filter output W_p (the backward of
+ * student_model.py:351-385 into rbf_to_scalar's output): for p = (a < b),
+ *   gW[p] = (gs_a s_b + gs_b s_a,  sum_x gv_a,x v_b,x + gv_b,x v_a,x,
+ *            sum_x u_x (gv_a,x - gv_b,x)),
+ * and with dgw_dev non-NULL its tangent along (du, ds, dv, dgs, dgv). */
+int mlff_train_pair_grad(void* stream, int hidden_dim, int n_pairs, const int32_t* pair_lo_dev,
+                         const int32_t* pair_hi_dev, const float* u_dev, const float* du_dev,
+                         const float* s_dev, const float* v_dev, const float* ds_dev,
+                         const float* dv_dev, const float* gs_dev, const float* gv_dev,
+                         const float* dgs_dev, const float* dgv_dev, float* gw_dev,
+                         float* dgw_dev);
+
+/* PaiNNMessage.forward on a caller-given directed edge list
+ * (student_model.py:346-387, the interaction submodules' own forward):
+ * each edge e (src -> dst) has its own filter row w[e] [3H] and edge vector
+ * ev[e] [3].  Edges are grouped by receiver: row_ptr [N+1] over dst, col [E]
+ * the sources, edge_id [E] each edge's index into w and ev; within a row they
+ * are added in that order:
+ *   s_out[i] = s[i] + sum_e s[src] Ws[e],
+ *   v_out[i] = v[i] + sum_e (v[src] Wv1[e] + ev[e] (x) Wv2[e]). */
+int mlff_edge_message(void* stream, int hidden_dim, int n_atoms, const int32_t* row_ptr_dev,
+                      const int32_t* col_dev, const int32_t* edge_id_dev, const float* s_dev,
+                      const float* v_dev, const float* w_dev, const float* ev_dev,
+                      float* s_out_dev, float* v_out_dev);
+
+/* Per-edge gradients of mlff_edge_message given the gradients (gs, gv) of
+ * its outputs: gw[e] = (gs[dst] s[src], sum_x gv[dst]_x v[src]_x,
+ * sum_x ev_x gv[dst]_x) [E][3H] and gev[e]_x = sum_h gv[dst]_x,h Wv2[e]_h
+ * [E][3].  (The gradients w.r.t. s and v are mlff_train_message_adjoint on
+ * the edges grouped by source.) */
+int mlff_edge_grad(void* stream, int hidden_dim, int n_edges, const int32_t* src_dev,
+                   const int32_t* dst_dev, const float* s_dev, const float* v_dev,
+                   const float* w_dev, const float* ev_dev, const float* gs_dev,
+                   const float* gv_dev, float* gw_dev, float* gev_dev);
+
+/* Fixed-order segment sums, replacing index_add_ reductions (per-structure
+ * energies, student_model.py:739-755; the embedding gradient):
+ * out[k][c] = sum over r in [seg_ptr[k], seg_ptr[k+1]) of
+ * src[perm ? perm[r] : r][c], added in r order, into exactly one of
+ * out64_dev (float64) or out32_dev (float32), [n_segments][width]. */
+int mlff_segment_sum(void* stream, int n_segments, int width, const int32_t* seg_ptr_dev,
+                     const int32_t* perm_dev, const float* src_dev, double* out64_dev,
+                     float* out32_dev);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,8 @@ EXPORTS = (
     "mlff_md_bind_frames", "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
     "mlff_launch_stage", "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
     "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams", "mlff_energy_forces_static",
+    "mlff_train_message", "mlff_train_message_adjoint", "mlff_train_pair_grad",
+    "mlff_segment_sum", "mlff_plan_set_tiles", "mlff_edge_message", "mlff_edge_grad",
 )
 
 # bytes of the static device scratch behind mlff_energy_forces_static
@@ -121,6 +123,8 @@ def _declare(lib):
     lib.mlff_launch_stage.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.mlff_plan_set_blocks.restype = ctypes.c_int
     lib.mlff_plan_set_blocks.argtypes = [vp, ctypes.c_int]
+    lib.mlff_plan_set_tiles.restype = ctypes.c_int
+    lib.mlff_plan_set_tiles.argtypes = [vp] + [ctypes.c_int] * 5
     lib.mlff_plan_set_streams.restype = ctypes.c_int
     lib.mlff_plan_set_streams.argtypes = [vp, ctypes.c_int]
     lib.mlff_plan_set_cell_grid.restype = ctypes.c_int
@@ -133,6 +137,18 @@ def _declare(lib):
     lib.mlff_mts_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.mlff_mts_graph_run.restype = ctypes.c_int
     lib.mlff_mts_graph_run.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.mlff_train_message.restype = ctypes.c_int
+    lib.mlff_train_message.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 15
+    lib.mlff_train_message_adjoint.restype = ctypes.c_int
+    lib.mlff_train_message_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 13
+    lib.mlff_train_pair_grad.restype = ctypes.c_int
+    lib.mlff_train_pair_grad.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 14
+    lib.mlff_edge_message.restype = ctypes.c_int
+    lib.mlff_edge_message.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 9
+    lib.mlff_edge_grad.restype = ctypes.c_int
+    lib.mlff_edge_grad.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 10
+    lib.mlff_segment_sum.restype = ctypes.c_int
+    lib.mlff_segment_sum.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 5
     lib.mlff_energy_forces_static.restype = ctypes.c_int
     lib.mlff_energy_forces_static.argtypes = [vp, ctypes.POINTER(Config)] + [vp] * 12
 

@@ -1,12 +1,23 @@
-"""Launch-shape autotuner for the persistent pair kernels.
+"""Launch-shape autotuner for the pair and message kernels.
 
-The pair kernels (filter forward and backward) run a persistent grid whose
-size trades load balance against L2 locality and occupancy, and, with the
-filter kernels on their own stream, against the room left for the atom-side
-kernels running beside them.  The tuner times each candidate grid on the
-plan's own workload with HIP events around whole evaluations (neighbour list,
-forward, adjoint: the unit the caller repeats), keeps the fastest, and
-remembers the choice per (device, hidden size, streams, pair-count bucket).
+Two kinds of choice, both timed on the plan's own workload with HIP events
+around whole evaluations (neighbour list, forward, adjoint: the unit the
+caller repeats):
+
+* the persistent grid of the pair kernels (filter forward and adjoint),
+  which trades load balance against L2 locality and occupancy, and, with the
+  filter kernels on their own stream, against the room left for the
+  atom-side kernels running beside them;
+* the launch shapes among the pre-instantiated variants
+  (engine.launch_shapes): pairs per filter-forward tile, pairs per
+  filter-adjoint tile, and edges in flight per message row (forward,
+  adjoint).  They trade registers and LDS (blocks per CU) against work per
+  tile; every variant gives bit-identical results.
+
+The grid is searched first with the default shapes, then each shape kind in
+turn is switched to its alternative and kept when the evaluation gets
+faster (coordinate descent).  The choice is remembered per (device, hidden
+size, streams, pair-count bucket).
 """
 
 from __future__ import annotations
@@ -66,26 +77,40 @@ def time_evaluation(plan, positions: torch.Tensor, e: torch.Tensor, f: torch.Ten
     return a.elapsed_time(b) / 1e3 / reps
 
 
-def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = None,
-         reps: int = 10, cache_file: Optional[str] = None) -> Dict[str, object]:
-    """Pick the persistent grid for ``plan`` (evaluated once at ``positions``).
+SHAPE_NAMES = ("filter_fwd_pairs", "filter_bwd_pairs", "message_fwd_unroll",
+               "message_bwd_unroll", "filter_bwd_prefetch")
 
-    Returns {"blocks": best, "timings_us": {blocks: us}}; the plan keeps the
-    best grid.  With ``cache_file`` the choice is read from / written to a
-    JSON file so later runs skip the search.
+
+def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = None,
+         reps: int = 10, cache_file: Optional[str] = None,
+         shapes: bool = True) -> Dict[str, object]:
+    """Pick the persistent grid and the launch shapes for ``plan`` (evaluated
+    at ``positions``).
+
+    Returns {"blocks": best grid, "tiles": (fwd pairs, bwd pairs, message
+    fwd unroll, message bwd unroll), "timings_us": {blocks: us},
+    "shape_timings_us": {shape label: us}}; the plan keeps the choice.  With
+    ``cache_file`` the choice is read from / written to a JSON file so later
+    runs skip the search; ``shapes=False`` tunes the grid only.
     """
+    from .engine import launch_shapes
     lib = C.lib()
     B = plan.layout.n_structures
     e = torch.empty(B, dtype=torch.float64, device=positions.device)
     f = torch.empty(plan.layout.n_atoms, 3, dtype=torch.float32, device=positions.device)
     plan.energy_forces(positions, e, f)
-    key = _key(plan)
+    key = _key(plan) + ("" if shapes else "|grid")
     if cache_file and os.path.exists(cache_file):
         with open(cache_file) as fh:
             _CACHE.update(json.load(fh))
     if key in _CACHE:
-        C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, int(_CACHE[key])))
-        return {"blocks": int(_CACHE[key]), "timings_us": {}, "cached": True}
+        hit = _CACHE[key]
+        blocks = int(hit["blocks"] if isinstance(hit, dict) else hit)
+        tiles = tuple(hit.get("tiles", plan.tiles)) if isinstance(hit, dict) else plan.tiles
+        C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, blocks))
+        plan.set_tiles(*tiles)
+        return {"blocks": blocks, "tiles": tuple(plan.tiles), "timings_us": {},
+                "shape_timings_us": {}, "cached": True}
     cands = sorted({(int(c) + 7) // 8 * 8 for c in (candidates or default_candidates())})
     timings = {}
     for blocks in cands:
@@ -93,9 +118,27 @@ def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = No
         timings[blocks] = time_evaluation(plan, positions, e, f, reps) * 1e6
     best = min(timings, key=timings.get)
     C.check("set_blocks", lib.mlff_plan_set_blocks(plan.handle, best))
-    _CACHE[key] = best
+    shape_timings = {}
+    tiles = list(plan.tiles)
+    if shapes:
+        best_us = time_evaluation(plan, positions, e, f, reps) * 1e6
+        shape_timings["default"] = best_us
+        shapes_h = launch_shapes(plan.H)
+        for k, (default, alt) in enumerate(shapes_h):
+            trial = list(tiles)
+            trial[k] = alt if tiles[k] == default else default
+            if trial[4] and trial[1] != shapes_h[1][0]:
+                continue                 # the prefetch exists for the default adjoint tile only
+            plan.set_tiles(*trial)
+            us = time_evaluation(plan, positions, e, f, reps) * 1e6
+            shape_timings[f"{SHAPE_NAMES[k]}={trial[k]}"] = us
+            if us < best_us:
+                best_us, tiles = us, trial
+        plan.set_tiles(*tiles)
+    _CACHE[key] = {"blocks": best, "tiles": list(plan.tiles)}
     if cache_file:
         os.makedirs(os.path.dirname(os.path.abspath(cache_file)), exist_ok=True)
         with open(cache_file, "w") as fh:
             json.dump(_CACHE, fh, indent=1)
-    return {"blocks": best, "timings_us": timings, "cached": False}
+    return {"blocks": best, "tiles": tuple(plan.tiles), "timings_us": timings,
+            "shape_timings_us": shape_timings, "cached": False}

@@ -236,6 +236,15 @@ int persistent_grid(int blocks, int cus) {
   return (blocks + 7) / 8 * 8;
 }
 
+void set_default_shapes(mlff_plan* pl) {
+  const int H = pl->cfg.hidden_dim;
+  pl->buf.tile_fwd = default_shape(H, 0);
+  pl->buf.tile_bwd = default_shape(H, 1);
+  pl->buf.unroll_fwd = default_shape(H, 2);
+  pl->buf.unroll_bwd = default_shape(H, 3);
+  pl->buf.bwd_prefetch = default_shape(H, 4);
+}
+
 template <typename T>
 T* segp(mlff_plan* pl, void* ws, int k) {
   return pl->seg[k].bytes ? reinterpret_cast<T*>(static_cast<char*>(ws) + pl->seg[k].off) : nullptr;
@@ -313,6 +322,7 @@ int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
   pl->mw = mw;
   carve(pl);
   pl->buf.blocks = persistent_grid(cfg->persistent_blocks, -1);
+  set_default_shapes(pl);
   pl->n_streams = 2;
   pl->stream_dev = -1;
   *out = pl;
@@ -539,6 +549,45 @@ int mlff_plan_set_blocks(mlff_plan* pl, int blocks) {
   pl->buf.blocks = blocks;
   if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
+  return MLFF_OK;
+}
+
+int mlff_plan_set_tiles(mlff_plan* pl, int fwd_pairs, int bwd_pairs, int msg_unroll_fwd,
+                        int msg_unroll_bwd, int bwd_prefetch) {
+  if (!pl) { set_error("mlff_plan_set_tiles: null plan"); return MLFF_ERR_INVALID; }
+  const int H = pl->cfg.hidden_dim;
+  const int want[4] = {fwd_pairs, bwd_pairs, msg_unroll_fwd, msg_unroll_bwd};
+  int got[4];
+  for (int k = 0; k < 4; ++k) {
+    const int d = default_shape(H, k), a = alternative_shape(H, k);
+    got[k] = want[k] == 0 ? d : want[k];
+    if (got[k] != d && got[k] != a) {
+      set_error("mlff_plan_set_tiles: shape %d of kind %d not instantiated for hidden_dim %d "
+                "(choices %d, %d)", want[k], k, H, d, a);
+      return MLFF_ERR_UNSUPPORTED;
+    }
+  }
+  if (bwd_prefetch != 0 && bwd_prefetch != 1) {
+    set_error("mlff_plan_set_tiles: bwd_prefetch must be 0 or 1");
+    return MLFF_ERR_INVALID;
+  }
+  if (bwd_prefetch && got[1] != default_shape(H, 1)) {
+    set_error("mlff_plan_set_tiles: the gather prefetch exists for the default adjoint tile "
+              "(%d pairs) only", default_shape(H, 1));
+    return MLFF_ERR_UNSUPPORTED;
+  }
+  pl->buf.tile_fwd = got[0];
+  pl->buf.tile_bwd = got[1];
+  pl->buf.unroll_fwd = got[2];
+  pl->buf.unroll_bwd = got[3];
+  pl->buf.bwd_prefetch = bwd_prefetch;
+  // captured graphs hold the old launches
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
   return MLFF_OK;
 }
 
@@ -885,6 +934,7 @@ extern "C" int mlff_energy_forces_static(void* stream, const mlff_config* cfg,
   rc = static_scratch(&dev, &ws, &cus);
   if (rc != MLFF_OK) return rc;
   pl.buf.blocks = persistent_grid(cfg->persistent_blocks, cus);
+  set_default_shapes(&pl);
   pl.n_streams = 1;
   pl.stream_dev = -1;
   rc = bind_buffers(&pl, who, ws, weights_dev, numbers_dev, mol_ptr_dev, atom_mol_dev, cells_dev,

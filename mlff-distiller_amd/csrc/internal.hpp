@@ -52,6 +52,11 @@ struct Buffers {
   float rc;
   int32_t ecap, pcap;
   int blocks;                       // persistent grid for pair kernels
+  // launch shapes: pairs per filter tile (forward, adjoint) and edges in
+  // flight per message row (forward, adjoint); default_shape() or
+  // alternative_shape() of each
+  int tile_fwd, tile_bwd, unroll_fwd, unroll_bwd;
+  int bwd_prefetch;   // filter adjoint loads its next tile's first gather item early
   const float* w;                   // weight blob
   const ModelW* mw;
   // bf16 three-plane copies of the filter's second layer, per block l:
@@ -83,6 +88,13 @@ struct Buffers {
 __host__ __device__ inline int64_t wbf_plane(int H) { return 3 * (int64_t)H * H; }
 __host__ __device__ inline int64_t wbf_f2w(int H, int l) { return (int64_t)l * 3 * wbf_plane(H); }
 __host__ __device__ inline int64_t wbf_elems(int H, int L) { return (int64_t)L * 3 * wbf_plane(H); }
+// Launch shapes (which: 0 filter-forward pairs per tile, 1 filter-adjoint
+// pairs per tile, 2 message-forward edges in flight, 3 message-adjoint edges
+// in flight, 4 filter-adjoint gather prefetch across tiles, default tile
+// only): the default for hidden size H and the one pre-instantiated
+// alternative.
+int default_shape(int H, int which);
+int alternative_shape(int H, int which);
 // Split the filter weights into their bf16 planes (once per bind; synchronous).
 hipError_t split_filter_weights(const Buffers& b);
 // The same split enqueued on stream s (no synchronisation).

@@ -43,14 +43,18 @@ template <int H>
 constexpr int kBwdPairs = H <= 32 ? 64 : 32;
 constexpr int kUpdWaves = 8;          // update kernels: waves per 32-atom tile
 constexpr int kUpdBlock = 64 * kUpdWaves;
-#ifndef MLFF_MSG_UNROLL_FWD
-#define MLFF_MSG_UNROLL_FWD 5
-#endif
-#ifndef MLFF_MSG_UNROLL_BWD
-#define MLFF_MSG_UNROLL_BWD 4
-#endif
-constexpr int kMsgUnrollFwd = MLFF_MSG_UNROLL_FWD;   // edges in flight per row (message forward)
-constexpr int kMsgUnrollBwd = MLFF_MSG_UNROLL_BWD;   // (message adjoint)
+// edges in flight per row in the message kernels (forward, adjoint)
+constexpr int kMsgUnrollFwd = 5;
+constexpr int kMsgUnrollBwd = 4;
+// The other launch shape of each kernel, pre-instantiated so the launch-shape
+// tuner can choose per workload (mlff_plan_set_tiles).  Results are
+// bit-identical across shapes: every output element sums in the same order.
+template <int H>
+constexpr int kFwdPairsAlt = H <= 32 ? 64 : 32;
+template <int H>
+constexpr int kBwdPairsAlt = H <= 32 ? 32 : 64;
+constexpr int kMsgUnrollFwdAlt = 3;
+constexpr int kMsgUnrollBwdAlt = 2;
 
 // Grid of a pair kernel: the plan's persistent grid, capped at the number of
 // tiles the pair capacity allows (a multiple of 8 for the XCD-interleaved tile
@@ -83,13 +87,13 @@ __global__ void embed_kernel(int N, const int64_t* __restrict__ Z,
 // ---------------------------------------------------------------------------
 // filter forward: W_p = W2 silu(W1 e(d_p) + b1) + b2 for kFwdPairs<H> pairs per tile
 // ---------------------------------------------------------------------------
-template <int H>
+template <int H, int TM>
 __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
     const float4* __restrict__ geom, const int32_t* __restrict__ counts, int32_t pcap,
     const float* __restrict__ w, LayerW lw, const float* __restrict__ centers,
     const float* __restrict__ widths, int R, int RP, float rc, float* __restrict__ Wout,
     const uint16_t* __restrict__ w2bf) {
-  constexpr int TM = kFwdPairs<H>, RT = TM / 16;
+  constexpr int RT = TM / 16;
   constexpr int LDE = kMaxRbf + 4;
   constexpr int LDHB = H + 8;          // bf16 row stride of the split hidden activations
   constexpr int PHB = TM * LDHB;       // plane stride
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_fwd_kernel(
 // One row per H/4 lanes, four channels (one float4) per lane; neighbours in
 // ascending j, a few edges' operands loaded before any is added.
 // ---------------------------------------------------------------------------
-template <int H, bool FWD>
+template <int H, bool FWD, int UNROLL>
 __global__ __launch_bounds__(kBlock) void message_kernel(
     int N, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const int32_t* __restrict__ edge_pair, int32_t ecap, const float4* __restrict__ geom,
@@ -264,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void message_kernel(
       a2 += q.uz * q.wv2;
     }
   };
-  constexpr int kUnroll = FWD ? kMsgUnrollFwd : kMsgUnrollBwd;
+  constexpr int kUnroll = UNROLL;
   // The row's lanes load the neighbour and pair indices of a batch of
   // kBatch edges together (one edge per lane), the next batch's while this
   // one is processed; each group of kUnroll edges takes its indices from the
@@ -714,7 +718,28 @@ __device__ __forceinline__ void gd_partial(const float* ga, int lda, const float
 // dL/du.  The running (dL/du, dL/dd) of all blocks is kept per pair; the
 // last block processed turns it into dE/dvec.
 // ---------------------------------------------------------------------------
-template <int H>
+// Pair indices of this thread's gather items of tile t (0 past the last tile
+// or the last pair: those items are switched off by p >= P in the gather).
+template <int TM, int C4, int NIT>
+__device__ __forceinline__ void load_pair_indices(int t, int t1, int P,
+                                                  const int32_t* __restrict__ pair_lo,
+                                                  const int32_t* __restrict__ pair_hi,
+                                                  int (&pa)[NIT], int (&pb)[NIT]) {
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int idx = threadIdx.x + k * kPairBlock;
+    const int p = t * TM + idx / C4;
+    const bool on = t < t1 && idx < TM * C4 && p < P;
+    pa[k] = on ? pair_lo[p] : 0;
+    pb[k] = on ? pair_hi[p] : 0;
+  }
+}
+
+// PF: the pair indices of a thread's gather items of its next tile are loaded
+// before the current tile's matrix products, so the next gather starts with
+// its operand loads instead of a dependent index load (same arithmetic,
+// bit-identical results).
+template <int H, int TM, bool PF>
 __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     const int32_t* __restrict__ counts, int32_t pcap, const int32_t* __restrict__ pair_lo,
     const int32_t* __restrict__ pair_hi, const float4* __restrict__ geom,
@@ -723,7 +748,7 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     const float* __restrict__ s_in, const float* __restrict__ v_in,
     const float* __restrict__ gs1, const float* __restrict__ gv1, float4* __restrict__ pacc,
     float4* __restrict__ gvec, int first, int last) {
-  constexpr int TM = kBwdPairs<H>, RT = TM / 16;
+  constexpr int RT = TM / 16;
   constexpr int LDE = kMaxRbf + 4, LDG = 3 * H + 4, LDA = H + 4;
   constexpr int C4 = H / 4;                    // lanes per pair in the gather (4 channels each)
   constexpr int NPART = 1;                     // one dE/du partial per pair
@@ -746,6 +771,9 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
   int tstep;
   persistent_range(ntiles, t0, t1, tstep);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  constexpr int NIT = (TM * C4 + kPairBlock - 1) / kPairBlock;   // gather items per thread
+  int pa[NIT], pb[NIT];
+  if (PF) load_pair_indices<TM, C4, NIT>(t0, t1, P, pair_lo, pair_hi, pa, pb);
   for (int tile = t0; tile < t1; tile += tstep) {
     const int pbase = tile * TM;
     if (threadIdx.x < TM) {   // the cutoff once per pair
@@ -771,12 +799,14 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
     // gather both endpoints, four channels per lane, every load of one item
     // issued up front (items one at a time: two blocks per CU fit the registers)
 #pragma unroll 1
-    for (int idx = threadIdx.x; idx < TM * C4; idx += kPairBlock) {
+    for (int k = 0; k < NIT; ++k) {
+      const int idx = threadIdx.x + k * kPairBlock;
+      if (idx >= TM * C4) break;
       const int m = idx / C4, c = 4 * (idx - m * C4), p = pbase + m;
       f4 gws = {0.f, 0.f, 0.f, 0.f}, gwv1 = gws, gwv2 = gws;
       float gu0 = 0.f, gu1 = 0.f, gu2 = 0.f;
       if (p < P) {
-        const int a = pair_lo[p], b = pair_hi[p];
+        const int a = PF ? pa[k] : pair_lo[p], b = PF ? pb[k] : pair_hi[p];
         const float4 gm = geom[p];
         const float* va = v_in + (size_t)a * 3 * H + c;
         const float* vb = v_in + (size_t)b * 3 * H + c;
@@ -823,6 +853,10 @@ __global__ __launch_bounds__(kPairBlock, 4) void filter_bwd_kernel(
       }
     }
     __syncthreads();
+    if (PF) {   // the next tile's pair indices, in flight during the products below
+      load_pair_indices<TM, C4, NIT>(tile + tstep, t1, P, pair_lo, pair_hi, pa, pb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     {
       // When the tile has at most one 16x16 output block per wave, each wave
       // takes one (row tile, column tile); otherwise every wave takes all row
@@ -1030,6 +1064,68 @@ __global__ void split_filter_weights_kernel(const float* __restrict__ w, ModelW 
 
 inline size_t nh(const Buffers& b) { return (size_t)b.N * b.H; }
 
+// Launches in the plan's chosen shapes (Buffers::tile_fwd, tile_bwd,
+// unroll_fwd, unroll_bwd; each is the default or its alternative).
+template <int H>
+void launch_filter_fwd(const Buffers& b, hipStream_t st, int l) {
+  const ModelW& mw = *b.mw;
+  const float* centers = b.w + mw.centers;
+  const float* widths = b.w + mw.widths;
+  float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+  const uint16_t* w2 = b.wbf + wbf_f2w(H, l);
+  if (b.tile_fwd == kFwdPairsAlt<H>) {
+    constexpr int TM = kFwdPairsAlt<H>;
+    filter_fwd_kernel<H, TM><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
+        b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc, Wl, w2);
+  } else {
+    constexpr int TM = kFwdPairs<H>;
+    filter_fwd_kernel<H, TM><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
+        b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc, Wl, w2);
+  }
+}
+
+template <int H>
+void launch_filter_bwd(const Buffers& b, hipStream_t st, int l, const float* gS1,
+                       const float* gV1, int first, int last) {
+  const ModelW& mw = *b.mw;
+  const float* centers = b.w + mw.centers;
+  const float* widths = b.w + mw.widths;
+  const float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
+  const float* Sl = b.S + (size_t)l * nh(b);
+  const float* Vl = b.V + (size_t)l * 3 * nh(b);
+  if (b.tile_bwd == kBwdPairsAlt<H>) {
+    constexpr int TM = kBwdPairsAlt<H>;
+    filter_bwd_kernel<H, TM, false><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
+        b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, mw.layer[l], centers, widths, b.R,
+        mw.RP, b.rc, Wl, Sl, Vl, gS1, gV1, b.pacc, b.gvec, first, last);
+  } else if (b.bwd_prefetch) {
+    constexpr int TM = kBwdPairs<H>;
+    filter_bwd_kernel<H, TM, true><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
+        b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, mw.layer[l], centers, widths, b.R,
+        mw.RP, b.rc, Wl, Sl, Vl, gS1, gV1, b.pacc, b.gvec, first, last);
+  } else {
+    constexpr int TM = kBwdPairs<H>;
+    filter_bwd_kernel<H, TM, false><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
+        b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, mw.layer[l], centers, widths, b.R,
+        mw.RP, b.rc, Wl, Sl, Vl, gS1, gV1, b.pacc, b.gvec, first, last);
+  }
+}
+
+template <int H, bool FWD>
+void launch_message(const Buffers& b, hipStream_t st, int grid, const float* Wl,
+                    const float* s_in, const float* v_in, float* s_out, float* v_out) {
+  constexpr int U0 = FWD ? kMsgUnrollFwd : kMsgUnrollBwd;
+  constexpr int U1 = FWD ? kMsgUnrollFwdAlt : kMsgUnrollBwdAlt;
+  if ((FWD ? b.unroll_fwd : b.unroll_bwd) == U1)
+    message_kernel<H, FWD, U1><<<grid, kBlock, 0, st>>>(b.N, b.row_ptr, b.col, b.edge_pair,
+                                                        b.ecap, b.geom, Wl, s_in, v_in, s_out,
+                                                        v_out);
+  else
+    message_kernel<H, FWD, U0><<<grid, kBlock, 0, st>>>(b.N, b.row_ptr, b.col, b.edge_pair,
+                                                        b.ecap, b.geom, Wl, s_in, v_in, s_out,
+                                                        v_out);
+}
+
 // Fork/join between the caller's stream and the filter stream.  Events order
 // the two streams both eagerly and under graph capture (a wait on an event
 // recorded in a capturing stream pulls the other stream into the capture).
@@ -1058,9 +1154,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
   // filter stream while the caller's stream walks the atom-side chain
   st.order(s, st.filt, 0);
   for (int l = 0; l < b.L; ++l) {
-    filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs<H>), kPairBlock, 0, st.filt>>>(
-        b.geom, b.counts, b.pcap, b.w, mw.layer[l], centers, widths, b.R, mw.RP, b.rc,
-        b.Wf + (size_t)l * b.pcap * 3 * H, b.wbf + wbf_f2w(H, l));
+    launch_filter_fwd<H>(b, st.filt, l);
     if (st.two() && st.err == hipSuccess) st.err = hipEventRecord(b.ev[1 + l], st.filt);
   }
   {
@@ -1079,8 +1173,7 @@ hipError_t forward_impl(const Buffers& b, hipStream_t s) {
     float* Vl = b.V + (size_t)l * 3 * nh(b);
     float* Sm = b.Smid + (size_t)l * nh(b);
     float* Vm = b.Vmid + (size_t)l * 3 * nh(b);
-    message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair, b.ecap,
-                                                       b.geom, Wl, Sl, Vl, Sm, Vm);
+    launch_message<H, true>(b, s, msg_grid, Wl, Sl, Vl, Sm, Vm);
     update_fwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
         N, b.w, lw, Sm, Vm, b.S + (size_t)(l + 1) * nh(b), b.V + (size_t)(l + 1) * 3 * nh(b),
         b.Aupd + (size_t)l * nh(b), b.G12 + (size_t)l * 2 * nh(b));
@@ -1116,14 +1209,9 @@ hipError_t backward_impl(const Buffers& b, hipStream_t s, float* forces, bool wa
         N, b.w, lw, gs2, gv2, b.Vmid + (size_t)l * 3 * nh(b), b.Aupd + (size_t)l * nh(b),
         b.G12 + (size_t)l * 2 * nh(b), gS1, gV1);
     st.order(s, st.filt, b.L + 1 + l);
-    filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs<H>), kPairBlock, 0, st.filt>>>(
-        b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
-        b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), gS1, gV1, b.pacc,
-        b.gvec, l == b.L - 1, l == 0);
+    launch_filter_bwd<H>(b, st.filt, l, gS1, gV1, l == b.L - 1, l == 0);
     if (l > 0) {
-      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
-                                                          b.ecap, b.geom, Wl, gS1, gV1,
-                                                          b.gS0, b.gV0);
+      launch_message<H, false>(b, s, msg_grid, Wl, gS1, gV1, b.gS0, b.gV0);
       gs2 = b.gS0;
       gv2 = b.gV0;
     }
@@ -1149,14 +1237,12 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
   float* Wl = b.Wf + (size_t)l * b.pcap * 3 * H;
   switch (stage) {
     case MLFF_STAGE_FILTER_FWD:
-      filter_fwd_kernel<H><<<pair_grid(b, kFwdPairs<H>), kPairBlock, 0, s>>>(b.geom, b.counts, b.pcap, b.w, lw, centers,
-                                                       widths, b.R, mw.RP, b.rc, Wl,
-                                                       b.wbf + wbf_f2w(H, l));
+      launch_filter_fwd<H>(b, s, l);
       break;
     case MLFF_STAGE_MESSAGE_FWD:
-      message_kernel<H, true><<<msg_grid, kBlock, 0, s>>>(
-          N, b.row_ptr, b.col, b.edge_pair, b.ecap, b.geom, Wl, b.S + (size_t)l * nh(b),
-          b.V + (size_t)l * 3 * nh(b), b.Smid + (size_t)l * nh(b), b.Vmid + (size_t)l * 3 * nh(b));
+      launch_message<H, true>(b, s, msg_grid, Wl, b.S + (size_t)l * nh(b),
+                              b.V + (size_t)l * 3 * nh(b), b.Smid + (size_t)l * nh(b),
+                              b.Vmid + (size_t)l * 3 * nh(b));
       break;
     case MLFF_STAGE_UPDATE_FWD:
       update_fwd_kernel<H><<<atom_grid, kUpdBlock, 0, s>>>(
@@ -1177,15 +1263,12 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
       break;
     case MLFF_STAGE_FILTER_BWD:
       // first=1 overwrites the pair accumulator, so repeated launches agree
-      filter_bwd_kernel<H><<<pair_grid(b, kBwdPairs<H>), kPairBlock, 0, s>>>(
-          b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, lw, centers, widths, b.R, mw.RP,
-          b.rc, Wl, b.S + (size_t)l * nh(b), b.V + (size_t)l * 3 * nh(b), b.gS1 + (size_t)l * nh(b),
-          b.gV1 + (size_t)l * 3 * nh(b), b.pacc, b.gvec, 1, 0);
+      launch_filter_bwd<H>(b, s, l, b.gS1 + (size_t)l * nh(b), b.gV1 + (size_t)l * 3 * nh(b),
+                           1, 0);
       break;
     case MLFF_STAGE_MESSAGE_BWD:
-      message_kernel<H, false><<<msg_grid, kBlock, 0, s>>>(
-          N, b.row_ptr, b.col, b.edge_pair, b.ecap, b.geom, Wl, b.gS1 + (size_t)l * nh(b),
-          b.gV1 + (size_t)l * 3 * nh(b), b.gS0, b.gV0);
+      launch_message<H, false>(b, s, msg_grid, Wl, b.gS1 + (size_t)l * nh(b),
+                               b.gV1 + (size_t)l * 3 * nh(b), b.gS0, b.gV0);
       break;
     case MLFF_STAGE_FORCES:
       force_gather_kernel<<<(N + 127) / 128, 128, 0, s>>>(N, b.row_ptr, b.col, b.edge_pair,
@@ -1199,6 +1282,28 @@ hipError_t stage_impl(const Buffers& b, hipStream_t s, int stage, int l) {
 }
 
 }  // namespace
+
+int default_shape(int H, int which) {
+  switch (which) {
+    case 0: return H <= 32 ? kFwdPairs<32> : kFwdPairs<128>;
+    case 1: return H <= 32 ? kBwdPairs<32> : kBwdPairs<128>;
+    case 2: return kMsgUnrollFwd;
+    case 3: return kMsgUnrollBwd;
+    case 4: return 0;
+    default: return 0;
+  }
+}
+
+int alternative_shape(int H, int which) {
+  switch (which) {
+    case 0: return H <= 32 ? kFwdPairsAlt<32> : kFwdPairsAlt<128>;
+    case 1: return H <= 32 ? kBwdPairsAlt<32> : kBwdPairsAlt<128>;
+    case 2: return kMsgUnrollFwdAlt;
+    case 3: return kMsgUnrollBwdAlt;
+    case 4: return 1;
+    default: return 0;
+  }
+}
 
 hipError_t split_filter_weights(const Buffers& b) {
   if (b.wbf == nullptr || b.L < 1) return hipSuccess;

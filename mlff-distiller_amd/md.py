@@ -86,6 +86,7 @@ class BatchedNVE:
         self._headroom = headroom
         self._persistent_blocks = persistent_blocks
         self._streams = streams
+        self._tiles = None
         # size the neighbour capacity from the starting configuration
         self._make_plan(self._initial_edges() * headroom)
         self.steps_done = 0
@@ -122,6 +123,7 @@ class BatchedNVE:
         energy_saved = self.energy.clone()
         self.tuning = autotune.tune(self.plan, self.pos32, **kw)
         self._persistent_blocks = int(self.tuning["blocks"])
+        self._tiles = tuple(self.tuning["tiles"])
         self.forces.copy_(forces_saved)
         self.energy.copy_(energy_saved)
         return self.tuning
@@ -139,7 +141,8 @@ class BatchedNVE:
         self.plan = engine.Plan(self.model.hidden_dim, self.model.num_interactions,
                                 self.model.num_rbf, self.model.max_z, self.model.cutoff,
                                 self.layout, self.numbers, self.weights, cap,
-                                persistent_blocks=self._persistent_blocks, streams=self._streams)
+                                persistent_blocks=self._persistent_blocks, streams=self._streams,
+                                tiles=self._tiles)
         self.lib = C.lib()
         C.check("mlff_md_bind", self.lib.mlff_md_bind(
             self.plan.handle, C.ptr(self.x), C.ptr(self.p), C.ptr(self.inv_mass),

@@ -75,15 +75,73 @@ class CosineCutoff(nn.Module):
         return f * (distances < self.cutoff).to(distances.dtype)
 
 
-class _FusedOnly:
-    def forward(self, *args, **kwargs):
-        raise NotImplementedError(
-            f"{type(self).__name__} holds parameters only; StudentForceField evaluates the "
-            "interaction blocks inside its fused HIP pipeline")
+def _edge_csr(key: torch.Tensor, other: torch.Tensor, n: int):
+    """Edges grouped by ``key`` (stable, so each group keeps the caller's
+    edge order): row_ptr [n+1], the other endpoint of each edge, its index."""
+    order = torch.sort(key, stable=True).indices
+    row_ptr = torch.zeros(n + 1, dtype=torch.int32, device=key.device)
+    row_ptr[1:] = torch.cumsum(torch.bincount(key, minlength=n), 0).to(torch.int32)
+    return row_ptr, other[order].to(torch.int32).contiguous(), order.to(torch.int32).contiguous()
 
 
-class PaiNNMessage(_FusedOnly, nn.Module):
-    """Parameters of the message block (student_model.py:297-322)."""
+def _device_f32(*ts):
+    for t in ts:
+        if not t.is_cuda or t.dtype != torch.float32:
+            raise TypeError("the interaction blocks run on the HIP kernels: float32 tensors on "
+                            "a ROCm device are required")
+
+
+class _EdgeMessage(torch.autograd.Function):
+    """The gathers of PaiNNMessage.forward (student_model.py:357-387) on a
+    caller-given directed edge list: the receiver-row HIP kernel forward; the
+    backward groups the edges by source for the feature gradients and takes
+    the filter and edge-vector gradients per edge (all on HIP kernels, fixed
+    order, no atomics)."""
+
+    @staticmethod
+    def forward(ctx, s, v, W, ev, src, dst):
+        from . import _capi as C
+        _device_f32(s, v, W, ev)
+        N, H = s.shape
+        s, v, W, ev = s.contiguous(), v.contiguous(), W.contiguous(), ev.contiguous()
+        src, dst = src.long(), dst.long()
+        if src.numel() and (int(src.min()) < 0 or int(src.max()) >= N or int(dst.min()) < 0
+                            or int(dst.max()) >= N):
+            raise ValueError("edge_index refers to atoms outside [0, num_atoms)")
+        row_ptr, col, eid = _edge_csr(dst, src, N)
+        so, vo = torch.empty_like(s), torch.empty_like(v)
+        C.check("mlff_edge_message", C.lib().mlff_edge_message(
+            C.stream_handle(), H, N, C.ptr(row_ptr), C.ptr(col), C.ptr(eid), C.ptr(s), C.ptr(v),
+            C.ptr(W), C.ptr(ev), C.ptr(so), C.ptr(vo)))
+        ctx.save_for_backward(s, v, W, ev, src, dst)
+        return so, vo
+
+    @staticmethod
+    def backward(ctx, gso, gvo):
+        from . import _capi as C
+        s, v, W, ev, src, dst = ctx.saved_tensors
+        N, H = s.shape
+        E = src.numel()
+        gso = torch.zeros_like(s) if gso is None else gso.contiguous().float()
+        gvo = torch.zeros_like(v) if gvo is None else gvo.contiguous().float()
+        row_ptr, col, eid = _edge_csr(src, dst, N)
+        gs, gv = torch.empty_like(s), torch.empty_like(v)
+        C.check("mlff_train_message_adjoint", C.lib().mlff_train_message_adjoint(
+            C.stream_handle(), H, N, C.ptr(row_ptr), C.ptr(col), C.ptr(eid), C.ptr(gso),
+            C.ptr(gvo), C.ptr(W), None, None, None, C.ptr(gs), C.ptr(gv), None, None))
+        gW = torch.empty(E, 3 * H, dtype=torch.float32, device=s.device)
+        gev = torch.empty(E, 3, dtype=torch.float32, device=s.device)
+        s32, d32 = src.to(torch.int32).contiguous(), dst.to(torch.int32).contiguous()
+        C.check("mlff_edge_grad", C.lib().mlff_edge_grad(
+            C.stream_handle(), H, E, C.ptr(s32), C.ptr(d32), C.ptr(s), C.ptr(v), C.ptr(W),
+            C.ptr(ev), C.ptr(gso), C.ptr(gvo), C.ptr(gW), C.ptr(gev)))
+        return gs, gv, gW, gev, None, None
+
+
+class PaiNNMessage(nn.Module):
+    """Message block (student_model.py:297-387).  StudentForceField evaluates
+    it inside its fused pipeline; its own forward takes a caller-given
+    directed edge list, as the reference's, and runs on the HIP kernels."""
 
     def __init__(self, hidden_dim: int, num_rbf: int):
         super().__init__()
@@ -91,9 +149,20 @@ class PaiNNMessage(_FusedOnly, nn.Module):
         self.rbf_to_scalar = nn.Sequential(nn.Linear(num_rbf, hidden_dim), nn.SiLU(),
                                            nn.Linear(hidden_dim, hidden_dim * 3))
 
+    def forward(self, scalar_features: torch.Tensor, vector_features: torch.Tensor,
+                edge_index: torch.Tensor, edge_rbf: torch.Tensor,
+                edge_vector: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        """s_i + sum_{j->i} s_j Ws, v_i + sum_{j->i} (v_j Wv1 + e_ji (x) Wv2)
+        with (Ws, Wv1, Wv2) = rbf_to_scalar(edge_rbf) per edge."""
+        filt = self.rbf_to_scalar(edge_rbf)
+        return _EdgeMessage.apply(scalar_features, vector_features, filt, edge_vector,
+                                  edge_index[0], edge_index[1])
 
-class PaiNNUpdate(_FusedOnly, nn.Module):
-    """Parameters of the update block (student_model.py:390-416)."""
+
+class PaiNNUpdate(nn.Module):
+    """Update block (student_model.py:390-470): n = |v| over xyz,
+    (ds, g1, g2) = update_mlp([s, n]), s + ds and v g1 + (M v) g2 (no
+    residual on v; the 3x3 mixing acts on the spatial axis)."""
 
     def __init__(self, hidden_dim: int):
         super().__init__()
@@ -102,14 +171,31 @@ class PaiNNUpdate(_FusedOnly, nn.Module):
                                         nn.Linear(hidden_dim, hidden_dim * 3))
         self.mixing_matrix = nn.Parameter(torch.randn(3, 3) / np.sqrt(3))
 
+    def forward(self, scalar_features: torch.Tensor,
+                vector_features: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        _device_f32(scalar_features, vector_features)
+        H = self.hidden_dim
+        norms = torch.linalg.vector_norm(vector_features, dim=1)
+        upd = self.update_mlp(torch.cat([scalar_features, norms], dim=-1))
+        ds, g1, g2 = upd[:, :H], upd[:, H:2 * H], upd[:, 2 * H:]
+        mixed = torch.matmul(self.mixing_matrix, vector_features)
+        return (scalar_features + ds,
+                vector_features * g1.unsqueeze(1) + mixed * g2.unsqueeze(1))
 
-class PaiNNInteraction(_FusedOnly, nn.Module):
-    """Message + update parameters (student_model.py:473-488)."""
+
+class PaiNNInteraction(nn.Module):
+    """Message then update (student_model.py:473-527)."""
 
     def __init__(self, hidden_dim: int, num_rbf: int):
         super().__init__()
         self.message = PaiNNMessage(hidden_dim, num_rbf)
         self.update = PaiNNUpdate(hidden_dim)
+
+    def forward(self, scalar_features: torch.Tensor, vector_features: torch.Tensor,
+                edge_index: torch.Tensor, edge_rbf: torch.Tensor,
+                edge_vector: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        s, v = self.message(scalar_features, vector_features, edge_index, edge_rbf, edge_vector)
+        return self.update(s, v)
 
 
 class _EnergyFunction(torch.autograd.Function):
@@ -174,8 +260,7 @@ class _Session:
     def param_grads(self, gE):
         from . import train
         p, g, c = self._prepared()
-        grads, _, _ = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype), want_forces=False,
-                                    rbf_grads=self._rbf_grads())
+        grads, _ = train.reverse(p, c, g, gE.reshape(-1), rbf_grads=self._rbf_grads())
         return grads
 
     def _rbf_grads(self) -> bool:
@@ -184,9 +269,8 @@ class _Session:
     def tangent_grads(self, gE, lam):
         from . import train
         p, g, c = self._prepared()
-        _, _, tg = train.reverse(p, c, g, gE.reshape(-1).to(self.pos.dtype),
-                                 lam=self._to_sorted(lam.to(self.pos.dtype)), want_forces=False,
-                                 rbf_grads=self._rbf_grads())
+        _, tg = train.reverse(p, c, g, gE.reshape(-1), lam=self._to_sorted(lam),
+                              rbf_grads=self._rbf_grads())
         return tg
 
 

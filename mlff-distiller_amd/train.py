@@ -1,4 +1,4 @@
-"""Force-matching training: hand-derived second-order pass (no autograd).
+"""Force-matching training on the GPU: hand-derived second-order pass.
 
 The reference trains with torch autograd double backward
 (models/distillation_wrapper.py:86-91, training/trainer.py:325-328, and the
@@ -9,29 +9,43 @@ respect to the weights is formed explicitly:
   dL/dtheta = sum_b dL/dE_b dE_b/dtheta  -  d/dtheta < lambda, grad_x E >,
   lambda = dL/dF.
 
-The first term is the reverse (adjoint) pass with parameter gradients.  The
-second is the directional derivative, along the position perturbation
-lambda, of the parameter gradient of the total energy: the tangent-linear
-version of the same reverse pass (forward-over-reverse).  Every line below
-is an explicit derivative of the corresponding line of
-student_model.py:634-757; the RBF expansion is recomputed from the
-distances rather than stored.
+The first term is the reverse (adjoint) pass with parameter gradients; the
+second is the tangent of that reverse pass along the position direction
+lambda (forward-over-reverse).  Both come from one reverse pass seeded with
+unit structure weights: its tangent gives the second term, and because
+structures do not interact the first term is the same pass's parameter
+terms with every atom or pair row scaled by dL/dE of its structure.
 
-The computation is written over unordered pairs (lo < hi), like the HIP
-kernels: a pair's filter is evaluated once and feeds both directed edges.
-Tensors may live on any device and in any float dtype (the tests run it in
-float64 on the CPU against the oracle's autograd); in the product it runs on
-the GPU on the neighbour list and pair geometry built by the HIP kernels.
+Where the work runs:
+* edge- and pair-sized parts on HIP kernels (mlff_train_message,
+  mlff_train_message_adjoint, mlff_train_pair_grad): the message gathers of
+  the forward pass and its tangent as receiver rows in ascending neighbour
+  order, the adjoint gathers the same way, and the filter-output gradients
+  per unordered pair from both endpoints -- no scatter-adds, no atomics;
+* per-structure energies and the embedding gradient by fixed-order segment
+  sums (mlff_segment_sum);
+* the dense products (filter and update MLPs, weight gradients) as plain
+  library GEMMs; weight gradients over many rows are split into 2048-row
+  chunks and summed, so no library split-K reduction is involved.
+Every reduction runs in a fixed order: two runs give bit-identical gradients.
+
+The forces the loss needs come from the fused HIP evaluation of the current
+weights (the trainer calls model.evaluate).  The computation is written
+over unordered pairs (lo < hi), like the HIP inference kernels: a pair's
+filter is evaluated once and feeds both directed edges.  float32 on the
+device; energies are summed in float64.
 """
 
 from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, Optional, Sequence
 
 import numpy as np
 import torch
+
+from . import _capi as C
 
 Tensor = torch.Tensor
 EPS = 1e-8   # edge_vector / (d + 1e-8), student_model.py:715
@@ -39,27 +53,148 @@ EPS = 1e-8   # edge_vector / (d + 1e-8), student_model.py:715
 
 @dataclass
 class PairGraph:
-    lo: Tensor          # [P] int64
-    hi: Tensor          # [P] int64
-    shift: Optional[Tensor]  # [P,3] float (already multiplied by the cell) or None
-    batch: Tensor       # [N] int64 structure of each atom
+    """Receiver CSR of a structure-sorted batch and its unordered pairs."""
+    row_ptr: Tensor          # [N+1] int32
+    col: Tensor              # [E] int32, ascending within a row
+    edge_pair: Tensor        # [E] int32, pair of each directed edge
+    lo: Tensor               # [P] int64 (lo < hi)
+    hi: Tensor               # [P] int64
+    lo32: Tensor             # [P] int32
+    hi32: Tensor             # [P] int32
+    shift: Optional[Tensor]  # [P,3] float shift vector (already times the cell) or None
+    batch: Tensor            # [N] int64 structure of each atom
+    mol_ptr: Tensor          # [B+1] int32
     n_structures: int
 
 
 def pair_graph_from_csr(row_ptr: Tensor, col: Tensor, batch: Tensor, n_structures: int,
                         shift_vec: Optional[Tensor] = None) -> PairGraph:
-    """Unordered pairs (row < col) from a symmetric receiver CSR."""
+    """Pairs (row < col) of a symmetric receiver CSR with rows in ascending
+    column order, and the pair of every directed edge: an upper edge
+    (row < col) numbers its pair in CSR order; a lower edge takes the
+    number of its reverse edge, found by a binary search of the sorted
+    (row, col) keys."""
+    dev = col.device
     n = row_ptr.numel() - 1
     deg = (row_ptr[1:] - row_ptr[:-1]).long()
-    row = torch.repeat_interleave(torch.arange(n, device=col.device), deg)
+    row = torch.repeat_interleave(torch.arange(n, device=dev), deg)
     colL = col.long()
-    keep = colL > row
-    sv = None if shift_vec is None else shift_vec[keep]
-    return PairGraph(row[keep], colL[keep], sv, batch.long(), n_structures)
+    up = colL > row
+    up_id = torch.cumsum(up.to(torch.int64), 0) - 1
+    key = row * n + colL
+    rev = torch.searchsorted(key, colL * n + row)
+    edge_pair = torch.where(up, up_id, up_id[rev]).to(torch.int32)
+    lo, hi = row[up], colL[up]
+    sizes = torch.bincount(batch, minlength=n_structures)
+    mol_ptr = torch.zeros(n_structures + 1, dtype=torch.int32, device=dev)
+    mol_ptr[1:] = torch.cumsum(sizes, 0).to(torch.int32)
+    sv = None if shift_vec is None else shift_vec[up].contiguous()
+    return PairGraph(row_ptr.to(torch.int32).contiguous(), col.to(torch.int32).contiguous(),
+                     edge_pair.contiguous(), lo, hi, lo.to(torch.int32).contiguous(),
+                     hi.to(torch.int32).contiguous(), sv, batch.long(), mol_ptr, n_structures)
+
+
+def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=None,
+                 pbc=None) -> PairGraph:
+    """Pair graph of a structure-sorted batch from the HIP neighbour list."""
+    from . import engine
+    periodic = pbc is not None and bool(np.asarray(pbc).any())
+    layout = engine.make_layout(sizes, x.device, cells if periodic else None,
+                                pbc if periodic else None)
+    pos32 = x.detach().to(torch.float32).contiguous()
+    row_ptr, col, shift = engine.neighbor_list(pos32, layout, rc, with_shifts=periodic)
+    batch = layout.atom_mol.long()
+    shift_vec = None
+    if periodic:
+        cell_t = torch.as_tensor(np.asarray(cells, dtype=np.float32).reshape(-1, 3, 3),
+                                 device=x.device)
+        n = row_ptr.numel() - 1
+        deg = (row_ptr[1:] - row_ptr[:-1]).long()
+        row = torch.repeat_interleave(torch.arange(n, device=x.device), deg)
+        cells_e = cell_t[batch[row]]
+        sh = shift.to(torch.float32)
+        shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
+                     sh[:, 2:3] * cells_e[:, 2])
+    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec)
 
 
 # ---------------------------------------------------------------------------
-# elementwise pieces
+# HIP launches
+# ---------------------------------------------------------------------------
+
+def _p(t: Optional[Tensor]) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def _dense(t: Tensor) -> Tensor:
+    if t.dtype != torch.float32 or not t.is_cuda:
+        raise TypeError("the training pass runs on float32 device tensors (HIP kernels)")
+    return t.contiguous()
+
+
+def _message(g: PairGraph, H: int, s, v, W, u, ds=None, dv=None, dW=None, du=None):
+    """Message outputs (s + sum s_j Ws, v + sum (v_j Wv1 + sg u Wv2)) and,
+    with the tangent inputs, their tangents (mlff_train_message)."""
+    s, v, W, u = _dense(s), _dense(v), _dense(W), _dense(u)
+    so, vo = torch.empty_like(s), torch.empty_like(v)
+    dso = dvo = None
+    if ds is not None:
+        ds, dv, dW, du = _dense(ds), _dense(dv), _dense(dW), _dense(du)
+        dso, dvo = torch.empty_like(s), torch.empty_like(v)
+    C.check("mlff_train_message", C.lib().mlff_train_message(
+        C.stream_handle(), H, s.shape[0], _p(g.row_ptr), _p(g.col), _p(g.edge_pair), _p(s),
+        _p(v), _p(W), _p(u), _p(ds), _p(dv), _p(dW), _p(du), _p(so), _p(vo), _p(dso), _p(dvo)))
+    return so, vo, dso, dvo
+
+
+def _message_adjoint(g: PairGraph, H: int, gs, gv, W, dgs=None, dgv=None, dW=None):
+    """gs + sum gs_j Ws, gv + sum gv_j Wv1 and, with the tangent inputs,
+    their tangents (mlff_train_message_adjoint)."""
+    gs, gv, W = _dense(gs), _dense(gv), _dense(W)
+    go, gvo = torch.empty_like(gs), torch.empty_like(gv)
+    dgo = dgvo = None
+    if dgs is not None:
+        dgs, dgv, dW = _dense(dgs), _dense(dgv), _dense(dW)
+        dgo, dgvo = torch.empty_like(gs), torch.empty_like(gv)
+    C.check("mlff_train_message_adjoint", C.lib().mlff_train_message_adjoint(
+        C.stream_handle(), H, gs.shape[0], _p(g.row_ptr), _p(g.col), _p(g.edge_pair), _p(gs),
+        _p(gv), _p(W), _p(dgs), _p(dgv), _p(dW), _p(go), _p(gvo), _p(dgo), _p(dgvo)))
+    return go, gvo, dgo, dgvo
+
+
+def _pair_grad(g: PairGraph, H: int, u, s, v, gs, gv, du=None, ds=None, dv=None, dgs=None,
+               dgv=None):
+    """dL/dW per pair from both endpoints and, with the tangents, its tangent
+    (mlff_train_pair_grad)."""
+    P = g.lo.numel()
+    u, s, v, gs, gv = _dense(u), _dense(s), _dense(v), _dense(gs), _dense(gv)
+    gW = torch.empty(P, 3 * H, dtype=torch.float32, device=s.device)
+    dgW = None
+    if dgs is not None:
+        du, ds, dv, dgs, dgv = _dense(du), _dense(ds), _dense(dv), _dense(dgs), _dense(dgv)
+        dgW = torch.empty_like(gW)
+    C.check("mlff_train_pair_grad", C.lib().mlff_train_pair_grad(
+        C.stream_handle(), H, P, _p(g.lo32), _p(g.hi32), _p(u), _p(du), _p(s), _p(v), _p(ds),
+        _p(dv), _p(gs), _p(gv), _p(dgs), _p(dgv), _p(gW), _p(dgW)))
+    return gW, dgW
+
+
+def _segment_sum(seg_ptr: Tensor, src: Tensor, perm: Optional[Tensor] = None,
+                 f64: bool = False) -> Tensor:
+    """out[k] = sum of the rows of segment k in row order (mlff_segment_sum)."""
+    src = _dense(src)
+    n_seg = seg_ptr.numel() - 1
+    width = src[0].numel() if src.dim() > 1 else 1
+    shape = (n_seg,) + tuple(src.shape[1:])
+    out = torch.empty(shape, dtype=torch.float64 if f64 else torch.float32, device=src.device)
+    C.check("mlff_segment_sum", C.lib().mlff_segment_sum(
+        C.stream_handle(), n_seg, width, _p(seg_ptr), _p(perm), _p(src),
+        _p(out) if f64 else 0, 0 if f64 else _p(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# elementwise pieces and dense products
 # ---------------------------------------------------------------------------
 
 def _silu(x):
@@ -77,7 +212,8 @@ def _silu2(x):
 
 
 def _edge_features(d, centers, widths, rc):
-    """e_k(d) = exp(-(d-mu_k)^2 / w_k^2) * 0.5 (cos(pi d/rc) + 1) [d < rc] and de/dd."""
+    """e_k(d) = exp(-(d-mu_k)^2 / w_k^2) * 0.5 (cos(pi d/rc) + 1) [d < rc]
+    (student_model.py:249-253, 285-290, 720-722) and de/dd."""
     diff = d.unsqueeze(-1) - centers
     gamma = 1.0 / widths ** 2
     phi = torch.exp(-gamma * diff ** 2)
@@ -88,28 +224,10 @@ def _edge_features(d, centers, widths, rc):
     return phi * fc, (-2.0 * gamma * diff * phi) * fc + phi * dfc
 
 
-def _edge_features_d2(d, centers, widths, rc):
-    """d^2 e_k / dd^2 (used by the tangent of de/dd)."""
-    diff = d.unsqueeze(-1) - centers
-    gamma = 1.0 / widths ** 2
-    phi = torch.exp(-gamma * diff ** 2)
-    dphi = -2.0 * gamma * diff * phi
-    d2phi = (-2.0 * gamma + 4.0 * gamma ** 2 * diff ** 2) * phi
-    inside = (d < rc).to(d.dtype).unsqueeze(-1)
-    arg = math.pi * d.unsqueeze(-1) / rc
-    k = math.pi / rc
-    fc = 0.5 * (torch.cos(arg) + 1.0) * inside
-    dfc = -0.5 * k * torch.sin(arg) * inside
-    d2fc = -0.5 * k * k * torch.cos(arg) * inside
-    return d2phi * fc + 2 * dphi * dfc + phi * d2fc
-
-
 def _edge_feature_param_derivs(d, centers, widths, rc):
-    """Derivatives of e_k(d) = phi_k(d) fc(d) in the basis parameters
-    (learnable_rbf, student_model.py:226-236): de_k/dmu_k, de_k/dw_k, and
-    the d-derivatives of both (for the tangent pass), each [P, R].
-    phi = exp(-(d-mu)^2 / w^2): dphi/dmu = 2 gamma (d-mu) phi,
-    dphi/dw = 2 (d-mu)^2 phi / w^3, gamma = 1/w^2."""
+    """de_k/dmu_k, de_k/dw_k and their d-derivatives, each [P, R]
+    (learnable_rbf, student_model.py:226-236): with phi = exp(-(d-mu)^2/w^2),
+    dphi/dmu = 2 gamma (d-mu) phi and dphi/dw = 2 (d-mu)^2 phi / w^3."""
     diff = d.unsqueeze(-1) - centers
     gamma = 1.0 / widths ** 2
     phi = torch.exp(-gamma * diff ** 2)
@@ -128,8 +246,8 @@ def _edge_feature_param_derivs(d, centers, widths, rc):
 def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
     """a.T @ b for tall a [K, M], b [K, N] (a weight gradient, K = pairs or
     atoms): the K reduction is split into chunks computed as one batched
-    product and then summed, because a single long reduction leaves most
-    of the GPU idle in the library's kernels."""
+    product and then summed, which keeps the GPU busy on one long
+    reduction and keeps every sum in a fixed order."""
     K = a.shape[0]
     if K < 4 * chunk:
         return a.T @ b
@@ -143,23 +261,17 @@ def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
 
 
 def _lin(x, p, name):
-    return x @ p[name + ".weight"].T + p[name + ".bias"]
+    return torch.addmm(p[name + ".bias"], x, p[name + ".weight"].T)
 
 
-def _add_pairs(n, lo, hi, to_lo, to_hi):
-    """Segment sum onto atoms of per-pair contributions to lo and to hi."""
-    out = torch.zeros((n,) + to_lo.shape[1:], dtype=to_lo.dtype, device=to_lo.device)
-    out.index_add_(0, lo, to_lo)
-    out.index_add_(0, hi, to_hi)
-    return out
+def _mix(M, v):
+    """(M v)_x = sum_z M[x][z] v_z over the xyz axis of [N,3,H]."""
+    return torch.matmul(M, v)
 
 
 def _mixT(M, v):
-    """(M^T v)_z = sum_x M[x][z] v_x over the xyz axis of [N,3,H] (three
-    weighted sums: a 3x3 contraction as a batched product would launch one
-    tiny GEMM per atom)."""
-    v0, v1, v2 = v[:, 0], v[:, 1], v[:, 2]
-    return torch.stack([M[0, z] * v0 + M[1, z] * v1 + M[2, z] * v2 for z in range(3)], dim=1)
+    """(M^T v)_z = sum_x M[x][z] v_x over the xyz axis of [N,3,H]."""
+    return torch.matmul(M.T.contiguous(), v)
 
 
 def _xyz_outer(a, b):
@@ -167,42 +279,34 @@ def _xyz_outer(a, b):
     return _tmm(a.permute(0, 2, 1).reshape(-1, 3), b.permute(0, 2, 1).reshape(-1, 3))
 
 
-def _mix(M, v):
-    """(M v)_x = sum_z M[x][z] v_z over the xyz axis of [N,3,H]."""
-    v0, v1, v2 = v[:, 0], v[:, 1], v[:, 2]
-    return torch.stack([M[x, 0] * v0 + M[x, 1] * v1 + M[x, 2] * v2 for x in range(3)], dim=1)
-
-
 # ---------------------------------------------------------------------------
-# forward with saved activations
+# forward (with saved activations) and its tangent
 # ---------------------------------------------------------------------------
 
-def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float):
-    """Primal pass; returns (E [B], cache)."""
+def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
+            lam: Optional[Tensor] = None):
+    """Primal pass (student_model.py:634-757); returns (E [B] float64, cache).
+    With ``lam`` [N,3] the tangent along lam is computed too."""
     H = p["embedding.weight"].shape[1]
     L = len({int(k.split(".")[1]) for k in p if k.startswith("interactions.")})
     n = Z.shape[0]
-    dt = x.dtype
+    x = x.to(torch.float32)
     vec = x[g.hi] - x[g.lo]
     if g.shift is not None:
-        vec = vec + g.shift.to(dt)
+        vec = vec + g.shift.to(torch.float32)
     d = torch.linalg.vector_norm(vec, dim=1)
     den = d + EPS
-    u = vec / den.unsqueeze(1)
-    e, de = _edge_features(d, p["rbf.centers"].to(dt), p["rbf.widths"].to(dt), rc)
-    s = p["embedding.weight"][Z].to(dt)
-    v = torch.zeros(n, 3, H, dtype=dt, device=x.device)
+    u = (vec / den.unsqueeze(1)).contiguous()
+    e, de = _edge_features(d, p["rbf.centers"], p["rbf.widths"], rc)
+    s = p["embedding.weight"][Z].contiguous()
+    v = torch.zeros(n, 3, H, dtype=torch.float32, device=x.device)
     layers = []
     for k in range(L):
         pre = f"interactions.{k}.message.rbf_to_scalar"
         a1 = _lin(e, p, pre + ".0")
         h = _silu(a1)
         W = _lin(h, p, pre + ".2")
-        Ws, Wv1, Wv2 = W[:, :H], W[:, H:2 * H], W[:, 2 * H:]
-        s_m = s + _add_pairs(n, g.lo, g.hi, s[g.hi] * Ws, s[g.lo] * Ws)
-        uW = u.unsqueeze(-1) * Wv2.unsqueeze(1)
-        v_m = v + _add_pairs(n, g.lo, g.hi, v[g.hi] * Wv1.unsqueeze(1) + uW,
-                             v[g.lo] * Wv1.unsqueeze(1) - uW)
+        s_m, v_m, _, _ = _message(g, H, s, v, W, u)
         upre = f"interactions.{k}.update"
         nrm = torch.linalg.vector_norm(v_m, dim=1)
         xin = torch.cat([s_m, nrm], dim=-1)
@@ -210,153 +314,148 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float)
         h2 = _silu(a2)
         y = _lin(h2, p, upre + ".update_mlp.2")
         ds, g1, g2 = y[:, :H], y[:, H:2 * H], y[:, 2 * H:]
-        M = p[upre + ".mixing_matrix"].to(dt)
+        M = p[upre + ".mixing_matrix"]
         Mv = _mix(M, v_m)
         s_new = s_m + ds
         v_new = v_m * g1.unsqueeze(1) + Mv * g2.unsqueeze(1)
         layers.append(dict(s=s, v=v, a1=a1, h=h, W=W, s_m=s_m, v_m=v_m, nrm=nrm, xin=xin,
                            a2=a2, h2=h2, g1=g1, g2=g2, Mv=Mv, M=M))
-        s, v = s_new, v_new
+        s, v = s_new.contiguous(), v_new.contiguous()
     r1 = _lin(s, p, "energy_head.0")
     h3 = _silu(r1)
     r2 = _lin(h3, p, "energy_head.2")
     h4 = _silu(r2)
     eps = _lin(h4, p, "energy_head.4")[:, 0]
-    E = torch.zeros(g.n_structures, dtype=dt, device=x.device).index_add_(0, g.batch, eps)
+    E = _segment_sum(g.mol_ptr, eps, f64=True)
     cache = dict(H=H, L=L, n=n, vec=vec, d=d, den=den, u=u, e=e, de=de, layers=layers,
-                 sL=s, r1=r1, h3=h3, r2=r2, h4=h4, Z=Z, rc=rc)
+                 sL=s, r1=r1, h3=h3, r2=r2, h4=h4, Z=Z, rc=rc, lam=None)
+    if lam is not None:
+        tangent(p, cache, g, lam)
     return E, cache
 
 
+def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
+    """Tangent of the forward pass along the position direction lam [N,3],
+    stored in the cache (the message tangents on the HIP row kernel)."""
+    H, n = c["H"], c["n"]
+    lam32 = lam.to(torch.float32)
+    vec, d, den = c["vec"], c["d"], c["den"]
+    dvec = lam32[g.hi] - lam32[g.lo]
+    dd = (vec * dvec).sum(1) / d
+    du = (dvec / den.unsqueeze(1) - vec * (dd / den ** 2).unsqueeze(1)).contiguous()
+    de_t = c["de"] * dd.unsqueeze(-1)
+    ds_ = torch.zeros(n, H, dtype=torch.float32, device=vec.device)
+    dv_ = torch.zeros(n, 3, H, dtype=torch.float32, device=vec.device)
+    for k, lay in enumerate(c["layers"]):
+        pre = f"interactions.{k}.message.rbf_to_scalar"
+        lay["ds"], lay["dv"] = ds_, dv_
+        da1 = de_t @ p[pre + ".0.weight"].T
+        dh = _silu1(lay["a1"]) * da1
+        dW = dh @ p[pre + ".2.weight"].T
+        lay["da1"], lay["dh"], lay["dW"] = da1, dh, dW
+        _, _, dsm, dvm = _message(g, H, lay["s"], lay["v"], lay["W"], c["u"], ds_, dv_, dW, du)
+        nrm, v_m = lay["nrm"], lay["v_m"]
+        safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
+        dnrm = torch.where(nrm > 0, (v_m * dvm).sum(1) / safe, torch.zeros_like(nrm))
+        dxin = torch.cat([dsm, dnrm], dim=-1)
+        upre = f"interactions.{k}.update"
+        da2 = dxin @ p[upre + ".update_mlp.0.weight"].T
+        dh2 = _silu1(lay["a2"]) * da2
+        dy = dh2 @ p[upre + ".update_mlp.2.weight"].T
+        dMv = _mix(lay["M"], dvm)
+        lay.update(dsm=dsm, dvm=dvm, dnrm=dnrm, dxin=dxin, da2=da2, dh2=dh2, dy=dy, dMv=dMv)
+        g1, g2 = lay["g1"], lay["g2"]
+        dg1, dg2 = dy[:, H:2 * H], dy[:, 2 * H:]
+        ds_ = (dsm + dy[:, :H]).contiguous()
+        dv_ = (dvm * g1.unsqueeze(1) + v_m * dg1.unsqueeze(1) + dMv * g2.unsqueeze(1)
+               + lay["Mv"] * dg2.unsqueeze(1)).contiguous()
+    dr1 = ds_ @ p["energy_head.0.weight"].T
+    dh3 = _silu1(c["r1"]) * dr1
+    dr2 = dh3 @ p["energy_head.2.weight"].T
+    dh4 = _silu1(c["r2"]) * dr2
+    c.update(lam=lam, dd=dd, du=du, de_t=de_t, dsL=ds_, dr1=dr1, dh3=dh3, dr2=dr2, dh4=dh4)
+
+
 # ---------------------------------------------------------------------------
-# reverse pass (first order), optionally with its tangent along lambda
+# reverse pass (parameter gradients), optionally with its tangent
 # ---------------------------------------------------------------------------
+
+def _z_segments(Z: Tensor, n_rows: int):
+    """Atoms ordered by element (stable sort) and the element segments, for
+    the fixed-order embedding-gradient sum."""
+    order = torch.sort(Z, stable=True).indices.to(torch.int32)
+    counts = torch.bincount(Z, minlength=n_rows)
+    ptr = torch.zeros(n_rows + 1, dtype=torch.int32, device=Z.device)
+    ptr[1:] = torch.cumsum(counts, 0).to(torch.int32)
+    return ptr, order
+
 
 def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
-            lam: Optional[Tensor] = None, want_forces: bool = True, want_grads: bool = True,
-            gw: Optional[Tensor] = None, rbf_grads: bool = False):
-    """Adjoint of E weighted by gE [B].
+            lam: Optional[Tensor] = None, gw: Optional[Tensor] = None,
+            rbf_grads: bool = False):
+    """Parameter gradients of sum_b gw_b E_b (gw defaults to gE) and -- with
+    the tangent along ``lam`` [N,3] (reused from the cache when forward()
+    already computed it for the same lam) -- the parameter gradients of
+    < lam, grad_x (sum_b gE_b E_b) >.  Returns (grads, tgrads or None).
 
-    Returns (grads, forces, tgrads): parameter gradients of sum_b gw_b E_b
-    (gw defaults to gE), forces -d(sum gE E)/dx, and -- when ``lam`` [N,3] is
-    given -- the parameter gradients of d/deps [sum_b gE_b dE_b/dx] . lam,
-    i.e. of < lam, grad_x (sum gE E) >.
-
-    The adjoint is linear in the structure weights and structures do not
-    interact, so the parameter gradients for weights gw are the terms of
-    the gE adjoint with each atom or pair row scaled by gw_b / gE_b of its
-    structure; ``gw`` is applied that way (gE must then be all ones), which
-    lets one pass give the unit-weight tangent and the gw-weighted
-    gradients together.  ``want_grads=False`` skips the parameter gradients.
-    ``rbf_grads`` adds the gradients of the basis centres and widths
-    (learnable_rbf); they are buffers otherwise and get none.
-    """
+    gw is applied as a per-row weight on the gE adjoint (structures do not
+    interact), so gE must then be all ones.  ``rbf_grads`` adds the basis
+    centres and widths (learnable_rbf)."""
     H, L, n = c["H"], c["L"], c["n"]
     T = lam is not None
-    dt = c["d"].dtype
+    if T and c.get("lam") is not lam:
+        tangent(p, c, g, lam)
     grads: Dict[str, Tensor] = {}
     tg: Dict[str, Tensor] = {}
+    dev = c["d"].device
 
     def acc(store, name, val):
         store[name] = store[name] + val if name in store else val
 
-    def accg(name, fn):
-        """parameter gradient term (evaluated only when wanted)"""
-        if want_grads:
-            acc(grads, name, fn())
-
+    gE = gE.to(torch.float32)
     if gw is not None:
+        gw = gw.to(torch.float32)
         w_atom = gw[g.batch]
-        w_pair = gw[g.batch[g.lo]]
+        w_pair = w_atom[g.lo]
         wa = lambda t: t * w_atom.view(-1, *([1] * (t.dim() - 1)))
         wp = lambda t: t * w_pair.view(-1, *([1] * (t.dim() - 1)))
     else:
         wa = wp = lambda t: t
 
-    # ---- tangents of the forward pass along lam --------------------------------
-    if T:
-        vec, d, den, u = c["vec"], c["d"], c["den"], c["u"]
-        dvec = lam[g.hi] - lam[g.lo]
-        dd = (vec * dvec).sum(1) / d
-        du = dvec / den.unsqueeze(1) - vec * (dd / den ** 2).unsqueeze(1)
-        de_t = c["de"] * dd.unsqueeze(-1)
-        ds_ = torch.zeros(n, H, dtype=dt, device=vec.device)
-        dv_ = torch.zeros(n, 3, H, dtype=dt, device=vec.device)
-        for k, lay in enumerate(c["layers"]):
-            pre = f"interactions.{k}.message.rbf_to_scalar"
-            lay["ds"], lay["dv"] = ds_, dv_
-            da1 = de_t @ p[pre + ".0.weight"].T
-            dh = _silu1(lay["a1"]) * da1
-            dW = dh @ p[pre + ".2.weight"].T
-            lay["da1"], lay["dh"], lay["dW"] = da1, dh, dW
-            W = lay["W"]
-            Ws, Wv1, Wv2 = W[:, :H], W[:, H:2 * H], W[:, 2 * H:]
-            dWs, dWv1, dWv2 = dW[:, :H], dW[:, H:2 * H], dW[:, 2 * H:]
-            s, v = lay["s"], lay["v"]
-            dsm = ds_ + _add_pairs(n, g.lo, g.hi, ds_[g.hi] * Ws + s[g.hi] * dWs,
-                                   ds_[g.lo] * Ws + s[g.lo] * dWs)
-            duW = du.unsqueeze(-1) * Wv2.unsqueeze(1) + u.unsqueeze(-1) * dWv2.unsqueeze(1)
-            dvm = dv_ + _add_pairs(
-                n, g.lo, g.hi,
-                dv_[g.hi] * Wv1.unsqueeze(1) + v[g.hi] * dWv1.unsqueeze(1) + duW,
-                dv_[g.lo] * Wv1.unsqueeze(1) + v[g.lo] * dWv1.unsqueeze(1) - duW)
-            nrm, v_m = lay["nrm"], lay["v_m"]
-            safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-            dnrm = torch.where(nrm > 0, (v_m * dvm).sum(1) / safe, torch.zeros_like(nrm))
-            dxin = torch.cat([dsm, dnrm], dim=-1)
-            upre = f"interactions.{k}.update"
-            da2 = dxin @ p[upre + ".update_mlp.0.weight"].T
-            dh2 = _silu1(lay["a2"]) * da2
-            dy = dh2 @ p[upre + ".update_mlp.2.weight"].T
-            M = lay["M"]
-            dMv = _mix(M, dvm)
-            lay.update(dsm=dsm, dvm=dvm, dnrm=dnrm, dxin=dxin, da2=da2, dh2=dh2, dy=dy, dMv=dMv)
-            g1, g2 = lay["g1"], lay["g2"]
-            dg1, dg2 = dy[:, H:2 * H], dy[:, 2 * H:]
-            ds_ = dsm + dy[:, :H]
-            dv_ = (dvm * g1.unsqueeze(1) + v_m * dg1.unsqueeze(1) + dMv * g2.unsqueeze(1)
-                   + lay["Mv"] * dg2.unsqueeze(1))
-        dsL = ds_
-        dr1 = dsL @ p["energy_head.0.weight"].T
-        dh3 = _silu1(c["r1"]) * dr1
-        dr2 = dh3 @ p["energy_head.2.weight"].T
-        dh4 = _silu1(c["r2"]) * dr2
-
     # ---- readout reverse ------------------------------------------------------
     geps = gE[g.batch]
     r1, h3, r2, h4, sL = c["r1"], c["h3"], c["r2"], c["h4"], c["sL"]
     Q3 = p["energy_head.4.weight"]
-    accg("energy_head.4.weight", lambda: (wa(geps).unsqueeze(1) * h4).sum(0, keepdim=True))
-    accg("energy_head.4.bias", lambda: wa(geps).sum().reshape(1))
+    acc(grads, "energy_head.4.weight", (wa(geps).unsqueeze(1) * h4).sum(0, keepdim=True))
+    acc(grads, "energy_head.4.bias", wa(geps).sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
     gr2 = gh4 * _silu1(r2)
-    accg("energy_head.2.weight", lambda: _tmm(wa(gr2), h3))
-    accg("energy_head.2.bias", lambda: wa(gr2).sum(0))
+    acc(grads, "energy_head.2.weight", _tmm(wa(gr2), h3))
+    acc(grads, "energy_head.2.bias", wa(gr2).sum(0))
     gh3 = gr2 @ p["energy_head.2.weight"]
     gr1 = gh3 * _silu1(r1)
-    accg("energy_head.0.weight", lambda: _tmm(wa(gr1), sL))
-    accg("energy_head.0.bias", lambda: wa(gr1).sum(0))
-    gs = gr1 @ p["energy_head.0.weight"]
-    gv = torch.zeros(n, 3, H, dtype=dt, device=gs.device)
+    acc(grads, "energy_head.0.weight", _tmm(wa(gr1), sL))
+    acc(grads, "energy_head.0.bias", wa(gr1).sum(0))
+    gs = (gr1 @ p["energy_head.0.weight"]).contiguous()
+    gv = torch.zeros(n, 3, H, dtype=torch.float32, device=dev)
     if T:
+        dr1, dh3, dr2, dh4 = c["dr1"], c["dh3"], c["dr2"], c["dh4"]
         acc(tg, "energy_head.4.weight", (geps.unsqueeze(1) * dh4).sum(0, keepdim=True))
-        acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=dt, device=gs.device))
+        acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=torch.float32, device=dev))
         dgr2 = gh4 * _silu2(r2) * dr2
         acc(tg, "energy_head.2.weight", _tmm(dgr2, h3) + _tmm(gr2, dh3))
         acc(tg, "energy_head.2.bias", dgr2.sum(0))
         dgh3 = dgr2 @ p["energy_head.2.weight"]
         dgr1 = dgh3 * _silu1(r1) + gh3 * _silu2(r1) * dr1
-        acc(tg, "energy_head.0.weight", _tmm(dgr1, sL) + _tmm(gr1, dsL))
+        acc(tg, "energy_head.0.weight", _tmm(dgr1, sL) + _tmm(gr1, c["dsL"]))
         acc(tg, "energy_head.0.bias", dgr1.sum(0))
-        dgs = dgr1 @ p["energy_head.0.weight"]
+        dgs = (dgr1 @ p["energy_head.0.weight"]).contiguous()
         dgv = torch.zeros_like(gv)
 
-    gd = torch.zeros_like(c["d"])
-    gu = torch.zeros_like(c["u"])
     u = c["u"]
     if rbf_grads:
-        e_mu, e_w, de_mu, de_w = _edge_feature_param_derivs(
-            c["d"], p["rbf.centers"].to(dt), p["rbf.widths"].to(dt), c["rc"])
+        e_mu, e_w, de_mu, de_w = _edge_feature_param_derivs(c["d"], p["rbf.centers"],
+                                                           p["rbf.widths"], c["rc"])
     for k in range(L - 1, -1, -1):
         lay = c["layers"][k]
         upre = f"interactions.{k}.update"
@@ -365,24 +464,25 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         U2 = p[upre + ".update_mlp.2.weight"]
         M = lay["M"]
         v_m, g1, g2, Mv, nrm = lay["v_m"], lay["g1"], lay["g2"], lay["Mv"], lay["nrm"]
-        # update reverse
+        # update reverse (student_model.py:434-470)
         gg1 = (gv * v_m).sum(1)
         gg2 = (gv * Mv).sum(1)
         gy = torch.cat([gs, gg1, gg2], dim=-1)
-        accg(upre + ".update_mlp.2.weight", lambda: _tmm(wa(gy), lay["h2"]))
-        accg(upre + ".update_mlp.2.bias", lambda: wa(gy).sum(0))
+        acc(grads, upre + ".update_mlp.2.weight", _tmm(wa(gy), lay["h2"]))
+        acc(grads, upre + ".update_mlp.2.bias", wa(gy).sum(0))
         gh2 = gy @ U2
         ga2 = gh2 * _silu1(lay["a2"])
-        accg(upre + ".update_mlp.0.weight", lambda: _tmm(wa(ga2), lay["xin"]))
-        accg(upre + ".update_mlp.0.bias", lambda: wa(ga2).sum(0))
+        acc(grads, upre + ".update_mlp.0.weight", _tmm(wa(ga2), lay["xin"]))
+        acc(grads, upre + ".update_mlp.0.bias", wa(ga2).sum(0))
         gx = ga2 @ U1
         gs_x, gn = gx[:, :H], gx[:, H:]
         gvg2 = gv * g2.unsqueeze(1)
-        accg(upre + ".mixing_matrix", lambda: _xyz_outer(wa(gvg2), v_m))
+        acc(grads, upre + ".mixing_matrix", _xyz_outer(wa(gvg2), v_m))
         safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-        unit_n = torch.where((nrm > 0).unsqueeze(1), v_m / safe.unsqueeze(1), torch.zeros_like(v_m))
-        gs_m = gs + gs_x
-        gv_m = gv * g1.unsqueeze(1) + _mixT(M, gvg2) + gn.unsqueeze(1) * unit_n
+        unit_n = torch.where((nrm > 0).unsqueeze(1), v_m / safe.unsqueeze(1),
+                             torch.zeros_like(v_m))
+        gs_m = (gs + gs_x).contiguous()
+        gv_m = (gv * g1.unsqueeze(1) + _mixT(M, gvg2) + gn.unsqueeze(1) * unit_n).contiguous()
         if T:
             dvm, dMv, dxin = lay["dvm"], lay["dMv"], lay["dxin"]
             dg1, dg2 = lay["dy"][:, H:2 * H], lay["dy"][:, 2 * H:]
@@ -398,109 +498,55 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             dgx = dga2 @ U1
             dgs_x, dgn = dgx[:, :H], dgx[:, H:]
             dgvg2 = dgv * g2.unsqueeze(1) + gv * dg2.unsqueeze(1)
-            acc(tg, upre + ".mixing_matrix",
-                _xyz_outer(dgvg2, v_m) + _xyz_outer(gvg2, dvm))
+            acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m) + _xyz_outer(gvg2, dvm))
             dnrm = lay["dnrm"]
             dunit = torch.where((nrm > 0).unsqueeze(1),
                                 dvm / safe.unsqueeze(1) - v_m * (dnrm / safe ** 2).unsqueeze(1),
                                 torch.zeros_like(v_m))
-            dgs_m = dgs + dgs_x
+            dgs_m = (dgs + dgs_x).contiguous()
             dgv_m = (dgv * g1.unsqueeze(1) + gv * dg1.unsqueeze(1) + _mixT(M, dgvg2)
-                     + dgn.unsqueeze(1) * unit_n + gn.unsqueeze(1) * dunit)
-        # message reverse
+                     + dgn.unsqueeze(1) * unit_n + gn.unsqueeze(1) * dunit).contiguous()
+        # message reverse (student_model.py:346-387): filter-output gradients per pair
         s, v, W = lay["s"], lay["v"], lay["W"]
-        Ws, Wv1, Wv2 = W[:, :H], W[:, H:2 * H], W[:, 2 * H:]
-        ga, gb = g.lo, g.hi
-        gWs = gs_m[ga] * s[gb] + gs_m[gb] * s[ga]
-        gWv1 = (gv_m[ga] * v[gb]).sum(1) + (gv_m[gb] * v[ga]).sum(1)
-        dgv_ab = gv_m[ga] - gv_m[gb]
-        gWv2 = (u.unsqueeze(-1) * dgv_ab).sum(1)
-        gu = gu + (dgv_ab * Wv2.unsqueeze(1)).sum(-1)
-        gW = torch.cat([gWs, gWv1, gWv2], dim=-1)
+        if T:
+            gW, dgW = _pair_grad(g, H, u, s, v, gs_m, gv_m, c["du"], lay["ds"], lay["dv"],
+                                 dgs_m, dgv_m)
+        else:
+            gW, _ = _pair_grad(g, H, u, s, v, gs_m, gv_m)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        accg(mpre + ".2.weight", lambda: _tmm(wp(gW), lay["h"]))
-        accg(mpre + ".2.bias", lambda: wp(gW).sum(0))
+        acc(grads, mpre + ".2.weight", _tmm(wp(gW), lay["h"]))
+        acc(grads, mpre + ".2.bias", wp(gW).sum(0))
         gh = gW @ W2
         ga1 = gh * _silu1(lay["a1"])
-        accg(mpre + ".0.weight", lambda: _tmm(wp(ga1), c["e"]))
-        accg(mpre + ".0.bias", lambda: wp(ga1).sum(0))
-        ge = ga1 @ W1
-        gd = gd + (ge * c["de"]).sum(-1)
+        acc(grads, mpre + ".0.weight", _tmm(wp(ga1), c["e"]))
+        acc(grads, mpre + ".0.bias", wp(ga1).sum(0))
         if rbf_grads:
-            accg("rbf.centers", lambda: (wp(ge) * e_mu).sum(0))
-            accg("rbf.widths", lambda: (wp(ge) * e_w).sum(0))
-        gs_new = gs_m + _add_pairs(n, ga, gb, gs_m[gb] * Ws, gs_m[ga] * Ws)
-        gv_new = gv_m + _add_pairs(n, ga, gb, gv_m[gb] * Wv1.unsqueeze(1),
-                                   gv_m[ga] * Wv1.unsqueeze(1))
+            ge = ga1 @ W1
+            acc(grads, "rbf.centers", (wp(ge) * e_mu).sum(0))
+            acc(grads, "rbf.widths", (wp(ge) * e_w).sum(0))
         if T:
-            ds, dv, dW, dh, da1 = lay["ds"], lay["dv"], lay["dW"], lay["dh"], lay["da1"]
-            dWs, dWv1, dWv2 = dW[:, :H], dW[:, H:2 * H], dW[:, 2 * H:]
-            dgWs = (dgs_m[ga] * s[gb] + gs_m[ga] * ds[gb] + dgs_m[gb] * s[ga] + gs_m[gb] * ds[ga])
-            dgWv1 = ((dgv_m[ga] * v[gb] + gv_m[ga] * dv[gb]).sum(1)
-                     + (dgv_m[gb] * v[ga] + gv_m[gb] * dv[ga]).sum(1))
-            ddgv_ab = dgv_m[ga] - dgv_m[gb]
-            dgWv2 = (du.unsqueeze(-1) * dgv_ab).sum(1) + (u.unsqueeze(-1) * ddgv_ab).sum(1)
-            dgW = torch.cat([dgWs, dgWv1, dgWv2], dim=-1)
+            dh, da1 = lay["dh"], lay["da1"]
             acc(tg, mpre + ".2.weight", _tmm(dgW, lay["h"]) + _tmm(gW, dh))
             acc(tg, mpre + ".2.bias", dgW.sum(0))
             dgh = dgW @ W2
             dga1 = dgh * _silu1(lay["a1"]) + gh * _silu2(lay["a1"]) * da1
-            acc(tg, mpre + ".0.weight", _tmm(dga1, c["e"]) + _tmm(ga1, de_t))
+            acc(tg, mpre + ".0.weight", _tmm(dga1, c["e"]) + _tmm(ga1, c["de_t"]))
             acc(tg, mpre + ".0.bias", dga1.sum(0))
             if rbf_grads:
                 dge = dga1 @ W1
-                acc(tg, "rbf.centers", (dge * e_mu + ge * de_mu * dd.unsqueeze(-1)).sum(0))
-                acc(tg, "rbf.widths", (dge * e_w + ge * de_w * dd.unsqueeze(-1)).sum(0))
-            dgs = dgs_m + _add_pairs(n, ga, gb, dgs_m[gb] * Ws + gs_m[gb] * dWs,
-                                     dgs_m[ga] * Ws + gs_m[ga] * dWs)
-            dgv = dgv_m + _add_pairs(n, ga, gb,
-                                     dgv_m[gb] * Wv1.unsqueeze(1) + gv_m[gb] * dWv1.unsqueeze(1),
-                                     dgv_m[ga] * Wv1.unsqueeze(1) + gv_m[ga] * dWv1.unsqueeze(1))
-        gs, gv = gs_new, gv_new
-    if want_grads:
-        emb = torch.zeros_like(p["embedding.weight"])
-        emb.index_add_(0, c["Z"], wa(gs).to(emb.dtype))
-        acc(grads, "embedding.weight", emb)
+                dd = c["dd"].unsqueeze(-1)
+                acc(tg, "rbf.centers", (dge * e_mu + ge * de_mu * dd).sum(0))
+                acc(tg, "rbf.widths", (dge * e_w + ge * de_w * dd).sum(0))
+            gs, gv, dgs, dgv = _message_adjoint(g, H, gs_m, gv_m, W, dgs_m, dgv_m, lay["dW"])
+        else:
+            gs, gv, _, _ = _message_adjoint(g, H, gs_m, gv_m, W)
+    emb_rows = p["embedding.weight"].shape[0]
+    zptr, zorder = _z_segments(c["Z"], emb_rows)
+    acc(grads, "embedding.weight", _segment_sum(zptr, wa(gs), zorder))
     if T:
-        demb = torch.zeros_like(p["embedding.weight"])
-        demb.index_add_(0, c["Z"], dgs.to(demb.dtype))
-        acc(tg, "embedding.weight", demb)
-
-    forces = None
-    if want_forces:
-        vec, d, den = c["vec"], c["d"], c["den"]
-        vdot = (vec * gu).sum(1)
-        G = (gd / d).unsqueeze(1) * vec + gu / den.unsqueeze(1) \
-            - vec * (vdot / (d * den * den)).unsqueeze(1)
-        forces = torch.zeros(n, 3, dtype=dt, device=vec.device)
-        forces.index_add_(0, g.lo, G)
-        forces.index_add_(0, g.hi, -G)
-    return grads, forces, (tg if T else None)
-
-
-def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=None,
-                 pbc=None) -> PairGraph:
-    """Pair graph of a structure-sorted batch from the HIP neighbour list."""
-    from . import engine
-    periodic = pbc is not None and bool(np.asarray(pbc).any())
-    layout = engine.make_layout(sizes, x.device, cells if periodic else None,
-                                pbc if periodic else None)
-    pos32 = x.detach().to(torch.float32).contiguous()
-    row_ptr, col, shift = engine.neighbor_list(pos32, layout, rc, with_shifts=periodic)
-    batch = layout.atom_mol.long()
-    shift_vec = None
-    if periodic:
-        cell_t = torch.as_tensor(np.asarray(cells, dtype=np.float32).reshape(-1, 3, 3),
-                                 device=x.device).to(x.dtype)
-        n = row_ptr.numel() - 1
-        deg = (row_ptr[1:] - row_ptr[:-1]).long()
-        row = torch.repeat_interleave(torch.arange(n, device=x.device), deg)
-        cells_e = cell_t[batch[row]]
-        sh = shift.to(x.dtype)
-        shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
-                     sh[:, 2:3] * cells_e[:, 2])
-    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec)
+        acc(tg, "embedding.weight", _segment_sum(zptr, dgs, zorder))
+    return grads, (tg if T else None)
 
 
 # ---------------------------------------------------------------------------
@@ -508,38 +554,32 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
 # ---------------------------------------------------------------------------
 
 def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
-                         E_label: Tensor, F_label: Tensor, w_energy: float = 1.0,
-                         w_force: float = 100.0, forces: Optional[Tensor] = None,
+                         E_label: Tensor, F_label: Tensor, forces: Tensor,
+                         w_energy: float = 1.0, w_force: float = 100.0,
                          rbf_grads: bool = False):
-    """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and
-    its exact gradient with respect to every parameter in ``p``.
-    ``forces``: the model's forces at ``x`` when already known (the fused
-    evaluation computes them far faster than the unit adjoint pass here)."""
-    E, cache = forward(p, Z, x, g, rc)
-    B = E.shape[0]
-    ones = torch.ones_like(E)
-    if forces is None:
-        _, F, _ = reverse(p, cache, g, ones, lam=None, want_forces=True, want_grads=False)
-    else:
-        F = forces.to(E.dtype)
-    eE = E - E_label
-    eF = F - F_label
-    loss = w_energy * (eE ** 2).mean() + w_force * (eF ** 2).mean()
-    dLdE = w_energy * 2.0 * eE / B
+    """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and its
+    exact gradient with respect to every parameter in ``p``.  ``forces``
+    are the model's forces at ``x`` (from the fused evaluation)."""
+    F = forces.to(torch.float32)
+    eF = F - F_label.to(torch.float32)
     dLdF = w_force * 2.0 * eF / eF.numel()
-    # one more pass gives both parts:
+    E, cache = forward(p, Z, x, g, rc, lam=dLdF)
+    B = E.shape[0]
+    eE = E - E_label.to(torch.float64)
+    loss = w_energy * (eE ** 2).mean() + w_force * (eF.double() ** 2).mean()
+    dLdE = (w_energy * 2.0 * eE / B).to(torch.float32)
+    # one reverse pass gives both parts:
     #   d/dtheta of wE MSE(E): the unit adjoint's terms weighted by dL/dE (gw)
-    #   d/dtheta of wF MSE(F) = d/dtheta < dL/dF, F > = - d/dtheta < dL/dF, grad_x sum E >
-    gE, _, tg = reverse(p, cache, g, ones, lam=dLdF, want_forces=False, gw=dLdE,
-                        rbf_grads=rbf_grads)
+    #   d/dtheta of wF MSE(F) = - d/dtheta < dL/dF, grad_x sum E >
+    gE, tg = reverse(p, cache, g, torch.ones(B, dtype=torch.float32, device=x.device),
+                     lam=dLdF, gw=dLdE, rbf_grads=rbf_grads)
     grads = {}
     for name in p:
         a = gE.get(name)
         b = tg.get(name)
         if a is None and b is None:
             continue
-        val = (a if a is not None else 0) - (b if b is not None else 0)
-        grads[name] = val
+        grads[name] = (a if a is not None else 0) - (b if b is not None else 0)
     return loss, E, F, grads
 
 
@@ -560,16 +600,14 @@ class ForceMatchingTrainer:
         self.opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=weight_decay)
 
     def loss_and_grads(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None):
+        x = x.detach().to(torch.float32).contiguous()
         g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
         p = {k: v.detach() for k, v in self.model.state_dict().items()}
-        F = None
-        if x.dtype == torch.float32 and x.device.type == "cuda":
-            # forces of the current weights from the fused HIP evaluation
-            _, F, _ = self.model.evaluate(Z, x.detach(), np.asarray(sizes), cells, pbc)
+        # forces of the current weights from the fused HIP evaluation
+        _, F, _ = self.model.evaluate(Z, x, np.asarray(sizes), cells, pbc)
         with torch.no_grad():
-            return force_matching_grads(p, Z, x.detach(), g, self.model.cutoff, E_label, F_label,
-                                        self.w_energy, self.w_force, forces=F,
-                                        rbf_grads=self.rbf_grads)
+            return force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label, F,
+                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads)
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)

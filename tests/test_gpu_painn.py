@@ -717,3 +717,52 @@ def test_forward_with_analytical_forces_matches_oracle(gpu):
     assert Eb.shape == (3,)
     assert np.max(np.abs(Eb.detach().cpu().numpy() - Eref.numpy()) / sizes) < E_TOL
     assert np.abs(Fb.cpu().numpy() - Fref.numpy()).max() < F_TOL
+
+
+@pytest.mark.parametrize("H,L,R_", [(128, 3, 20), (32, 2, 16)])
+def test_every_launch_shape_is_bit_identical(gpu, H, L, R_):
+    """The launch-shape tuner picks pairs per filter tile (forward and
+    adjoint), edges in flight per message row and the filter adjoint's
+    index prefetch among pre-instantiated variants; each output element sums
+    in the same order in all of them, so
+    every combination gives the bits of the default (energies, forces and
+    virials); a shape that was not instantiated is refused."""
+    note(f"launch shapes H{H}: every combination vs default")
+    engine = pkg_mod("engine")
+    synth = pkg_mod("synth")
+    C = pkg_mod("_capi")
+    mols = _mols(20, 10, 100, seed=57)
+    Z, P, mol_ptr = synth.flatten(mols)
+    sizes = np.diff(mol_ptr)
+    dev = torch.device(gpu)
+    lay = engine.make_layout(sizes, dev)
+    Zt = torch.tensor(Z, dtype=torch.int64, device=dev)
+    Pt = torch.tensor(P.astype(np.float32), device=dev)
+    model = _model(H, L, R_, seed=0, bias_scale=0.2)
+    w = model.packed_weights(dev)
+    cap = engine.edge_capacity_bound(sizes, False)
+    plan = engine.Plan(H, L, R_, model.max_z, 5.0, lay, Zt, w, cap)
+    shapes = engine.launch_shapes(H)
+    assert all(len(v) == 2 for v in shapes)
+
+    def run():
+        e = torch.empty(len(sizes), dtype=torch.float64, device=dev)
+        f = torch.empty(len(Z), 3, dtype=torch.float32, device=dev)
+        vir = torch.empty(len(sizes), 9, dtype=torch.float64, device=dev)
+        plan.energy_forces(Pt, e, f, vir)
+        torch.cuda.synchronize()
+        return e, f, vir
+
+    ref = run()
+    import itertools
+    for combo in itertools.product(*shapes):
+        if combo[4] and combo[1] != shapes[1][0]:
+            with pytest.raises(C.MlffError):       # prefetch exists for the default tile only
+                plan.set_tiles(*combo)
+            continue
+        plan.set_tiles(*combo)
+        got = run()
+        for a, b in zip(ref, got):
+            assert torch.equal(a, b), combo
+    with pytest.raises(C.MlffError):
+        plan.set_tiles(48, 0, 0, 0)

@@ -153,3 +153,106 @@ def test_distillation_wrapper_on_collated_dataset(gpu, tmp_path):
         ref = grads[n].reshape(p.shape)
         rel = ((p.grad - ref).norm() / (ref.norm() + 1e-30)).item()
         assert rel < 1e-4, (n, rel)
+
+
+def test_training_gradients_are_bit_reproducible(gpu):
+    """Fixed-order sums everywhere (row kernels, pair kernel, segment sums,
+    chunked weight-gradient products): two evaluations of the loss gradient
+    on the same batch are bit-identical."""
+    note("training gradients: two runs bit-identical")
+    synth = pkg_mod("synth")
+    torch.manual_seed(1)
+    teacher = pkg().StudentForceField()
+    torch.manual_seed(0)
+    student = pkg().StudentForceField().to(gpu)
+    mols = synth.molecule_set(64, 20, 60, seed=4)
+    Z, P, sizes, b, El, Fl = _labels({k: v.double() for k, v in teacher.state_dict().items()},
+                                     mols[:12])
+    tr = pkg_mod("train").ForceMatchingTrainer(student)
+    args = (torch.tensor(Z, device=gpu), torch.tensor(P, device=gpu), sizes, El.float().to(gpu),
+            Fl.float().to(gpu))
+    l1, _, _, g1 = tr.loss_and_grads(*args)
+    l2, _, _, g2 = tr.loss_and_grads(*args)
+    assert float(l1) == float(l2)
+    assert set(g1) == set(g2)
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
+
+
+def test_learnable_rbf_gradients_vs_fp64_autograd(gpu):
+    """learnable_rbf=True: the basis centres and widths train too; their
+    gradients (and every other) match FP64 autograd double backward."""
+    note("learnable RBF gradients vs fp64 autograd")
+    synth = pkg_mod("synth")
+    torch.manual_seed(1)
+    teacher = pkg().StudentForceField(hidden_dim=64, num_interactions=2)
+    torch.manual_seed(0)
+    student = pkg().StudentForceField(hidden_dim=64, num_interactions=2,
+                                      learnable_rbf=True).to(gpu)
+    with torch.no_grad():
+        g = torch.Generator().manual_seed(3)
+        student.rbf.centers.add_((0.05 * torch.randn(20, generator=g)).to(gpu))
+    mols = synth.molecule_set(5, 20, 40, seed=12)
+    Z, P, sizes, b, El, Fl = _labels({k: v.double() for k, v in teacher.state_dict().items()}, mols)
+    tr = pkg_mod("train").ForceMatchingTrainer(student)
+    _, _, _, grads = tr.loss_and_grads(torch.tensor(Z, device=gpu), torch.tensor(P, device=gpu),
+                                       sizes, El.float().to(gpu), Fl.float().to(gpu))
+    sd = {k: v.detach().cpu() for k, v in student.state_dict().items()}
+    p = {k: v.double().clone().requires_grad_(True) for k, v in sd.items()}
+    pos = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+    E = R.energy(p, torch.tensor(Z), pos, 5.0, batch=torch.tensor(b), n_structures=len(mols))
+    F = -torch.autograd.grad(E.sum(), pos, create_graph=True)[0]
+    loss = ((E - El) ** 2).mean() + 100.0 * ((F - Fl) ** 2).mean()
+    names = list(p)
+    ref = dict(zip(names, torch.autograd.grad(loss, [p[k] for k in names], allow_unused=True)))
+    for k in ("rbf.centers", "rbf.widths", "interactions.0.message.rbf_to_scalar.0.weight"):
+        r = ref[k]
+        rel = ((grads[k].double().cpu() - r).norm() / (r.norm() + 1e-30)).item()
+        note(f"{k}: rel {rel:.2e}")
+        assert rel < 2e-3, (k, rel)
+
+
+def test_periodic_training_gradients_vs_fp64_autograd(gpu):
+    """A periodic water box (minimum-image edges with lattice shifts): the
+    training gradients match FP64 autograd double backward on the oracle's
+    own edge set."""
+    note("periodic training gradients vs fp64 autograd")
+    from oracle import nl_ref
+    synth = pkg_mod("synth")
+    box = synth.water_box(4, seed=3)
+    torch.manual_seed(1)
+    teacher = pkg().StudentForceField(hidden_dim=64, num_interactions=2)
+    torch.manual_seed(0)
+    student = pkg().StudentForceField(hidden_dim=64, num_interactions=2).to(gpu)
+    P32 = box.positions.astype(np.float32)
+    n = len(P32)
+    rp, cl, sh = nl_ref.neighbor_csr(P32, np.array([0, n]), 5.0, box.cell[None], box.pbc[None])
+    ei = np.stack([cl.astype(np.int64), np.repeat(np.arange(n), np.diff(rp))])
+    cell = box.cell.astype(np.float32).astype(np.float64)
+    sdt = {k: v.double() for k, v in teacher.state_dict().items()}
+    El, Fl = R.energy_and_forces(sdt, box.numbers, P32.astype(np.float64), edge_index=ei,
+                                 shifts=sh, cell=cell)
+    El = El.reshape(1)
+    tr = pkg_mod("train").ForceMatchingTrainer(student)
+    _, _, _, grads = tr.loss_and_grads(torch.tensor(box.numbers, device=gpu),
+                                       torch.tensor(P32, device=gpu), np.array([n]),
+                                       El.float().to(gpu), Fl.float().to(gpu),
+                                       cells=box.cell[None], pbc=box.pbc[None])
+    sd = {k: v.detach().cpu() for k, v in student.state_dict().items()}
+    p = {k: v.double().clone().requires_grad_(True) for k, v in sd.items()}
+    pos = torch.tensor(P32, dtype=torch.float64, requires_grad=True)
+    sv = torch.tensor(sh, dtype=torch.float64) @ torch.tensor(cell)
+    E = R.energy(p, torch.tensor(box.numbers), pos, 5.0, edge_index=torch.tensor(ei),
+                 shift_vec=sv).reshape(1)
+    F = -torch.autograd.grad(E.sum(), pos, create_graph=True)[0]
+    loss = ((E - El) ** 2).mean() + 100.0 * ((F - Fl) ** 2).mean()
+    names = [k for k in p if not k.startswith("rbf.")]
+    ref = dict(zip(names, torch.autograd.grad(loss, [p[k] for k in names], allow_unused=True)))
+    worst = 0.0
+    for k, r in ref.items():
+        if r is None:
+            continue
+        rel = ((grads[k].double().cpu() - r).norm() / (r.norm() + 1e-30)).item()
+        worst = max(worst, rel)
+        assert rel < 2e-3, (k, rel)
+    note(f"worst relative gradient error {worst:.2e}")

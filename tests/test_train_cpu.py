@@ -1,5 +1,6 @@
-"""Hand-derived training gradients against the oracle's autograd double
-backward and central finite differences, in float64 on the CPU."""
+"""The hand-derived training gradients (oracle/train_ref.py, the checker of
+the HIP training pass) against the oracle's autograd double backward and
+central finite differences, in float64 on the CPU."""
 
 import numpy as np
 import pytest
@@ -7,6 +8,7 @@ import torch
 
 from conftest import pkg_mod
 from oracle import painn_ref as R
+from oracle import train_ref
 
 
 def _setup(H=16, L=2, Rb=6, rc=3.5, n_mol=3, seed=0, bias=0.2):
@@ -34,7 +36,7 @@ def _setup(H=16, L=2, Rb=6, rc=3.5, n_mol=3, seed=0, bias=0.2):
 
 
 def _graph(x, batch, rc, n_mol):
-    train = pkg_mod("train")
+    train = train_ref
     ei = R.radius_graph_dense(x, rc, batch)
     keep = ei[1] > ei[0]                       # src > dst: one entry per unordered pair
     lo, hi = ei[1][keep], ei[0][keep]
@@ -42,7 +44,7 @@ def _graph(x, batch, rc, n_mol):
 
 
 def test_forward_and_forces_match_oracle():
-    train = pkg_mod("train")
+    train = train_ref
     sd, Z, x, batch, rc = _setup()
     g = _graph(x, batch, rc, 3)
     E, cache = train.forward(sd, Z, x, g, rc)
@@ -65,7 +67,7 @@ def _autograd_loss_grads(sd, Z, x, batch, rc, El, Fl, wE, wF, rbf=False):
 
 
 def test_loss_gradient_matches_autograd_double_backward():
-    train = pkg_mod("train")
+    train = train_ref
     sd, Z, x, batch, rc = _setup()
     g = _graph(x, batch, rc, 3)
     gen = torch.Generator().manual_seed(5)
@@ -85,7 +87,7 @@ def test_loss_gradient_matches_autograd_double_backward():
 
 def test_loss_gradient_central_differences():
     """Pins the derivation independently of autograd."""
-    train = pkg_mod("train")
+    train = train_ref
     sd, Z, x, batch, rc = _setup(H=8, L=2, Rb=4, n_mol=2, seed=3)
     g = _graph(x, batch, rc, 2)
     El = torch.tensor([0.3, -0.2], dtype=torch.float64)
@@ -110,8 +112,8 @@ def test_loss_gradient_central_differences():
 
 
 def test_chunked_weight_gradient_product():
-    """The split-K weight-gradient product equals a.T @ b, including a
-    ragged tail and the short-K path."""
+    """The product's split-K weight-gradient product equals a.T @ b,
+    including a ragged tail and the short-K path."""
     train = pkg_mod("train")
     g = torch.Generator().manual_seed(3)
     for K in (100, 8192, 10000, 20480 + 7):
@@ -127,7 +129,7 @@ def test_weighted_gradients_equal_weighted_adjoint():
     """Structures do not interact, so the parameter gradients of sum_b w_b E_b
     from the unit adjoint with rows weighted by w (gw) equal those of the
     adjoint seeded with w; want_grads=False returns no gradients."""
-    train = pkg_mod("train")
+    train = train_ref
     sd, Z, x, batch, rc = _setup(seed=2)
     g = _graph(x, batch, rc, 3)
     E, cache = train.forward(sd, Z, x, g, rc)
@@ -146,7 +148,7 @@ def test_learnable_rbf_gradients_match_autograd_double_backward():
     student_model.py:226-236): their loss gradients, first and second order,
     equal FP64 autograd's; the centres and widths are moved off their
     defaults so every term of the derivative is exercised."""
-    train = pkg_mod("train")
+    train = train_ref
     sd, Z, x, batch, rc = _setup(seed=4)
     gen = torch.Generator().manual_seed(11)
     sd["rbf.centers"] = sd["rbf.centers"] + 0.1 * torch.randn(sd["rbf.centers"].shape,
