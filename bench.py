@@ -688,7 +688,7 @@ def main():
         "autotune": ({"persistent_blocks": tuning["blocks"],
                       "launch_shapes": dict(zip(("filter_fwd_pairs", "filter_bwd_pairs",
                                                  "message_fwd_unroll", "message_bwd_unroll",
-                                                 "filter_bwd_prefetch"), tuning["tiles"])),
+                                                 "filter_bwd_variant"), tuning["tiles"])),
                       "evaluation_us": {str(k): round(v, 1)
                                           for k, v in tuning["timings_us"].items()},
                       "shape_evaluation_us": {k: round(v, 1)

@@ -242,12 +242,13 @@ int mlff_plan_set_blocks(mlff_plan* plan, int blocks);
  * pre-instantiated alternative (forward tile 64 or 32 for H >= 64, 128 or 64
  * for H = 32; adjoint tile 32 or 64 for H >= 64, 64 or 32 for H = 32;
  * message 5 or 3 and 4 or 2 edges), anything else is MLFF_ERR_UNSUPPORTED.
- * bwd_prefetch = 1 (default adjoint tile only) makes the filter adjoint load
- * the first gather item of its next tile before the current tile's matrix
- * products.  Results are bit-identical across shapes (every output element
- * sums in the same order).  Drops any captured graph of the plan. */
+ * bwd_variant (default adjoint tile only) picks the filter adjoint's form:
+ * 0 as described above; 1 loads the pair indices of the next tile's gather
+ * before the current tile's matrix products.
+ * Results are bit-identical across shapes (every output element sums in the
+ * same order).  Drops any captured graph of the plan. */
 int mlff_plan_set_tiles(mlff_plan* plan, int fwd_pairs, int bwd_pairs, int msg_unroll_fwd,
-                        int msg_unroll_bwd, int bwd_prefetch);
+                        int msg_unroll_bwd, int bwd_variant);
 
 /* Number of streams one evaluation uses: 2 (default) runs the pair filter
  * kernels on a stream of the plan's own, forked from and joined back into the

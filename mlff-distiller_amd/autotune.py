@@ -78,7 +78,7 @@ def time_evaluation(plan, positions: torch.Tensor, e: torch.Tensor, f: torch.Ten
 
 
 SHAPE_NAMES = ("filter_fwd_pairs", "filter_bwd_pairs", "message_fwd_unroll",
-               "message_bwd_unroll", "filter_bwd_prefetch")
+               "message_bwd_unroll", "filter_bwd_variant")
 
 
 def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = None,
@@ -124,16 +124,21 @@ def tune(plan, positions: torch.Tensor, candidates: Optional[Iterable[int]] = No
         best_us = time_evaluation(plan, positions, e, f, reps) * 1e6
         shape_timings["default"] = best_us
         shapes_h = launch_shapes(plan.H)
-        for k, (default, alt) in enumerate(shapes_h):
-            trial = list(tiles)
-            trial[k] = alt if tiles[k] == default else default
-            if trial[4] and trial[1] != shapes_h[1][0]:
-                continue                 # the prefetch exists for the default adjoint tile only
-            plan.set_tiles(*trial)
-            us = time_evaluation(plan, positions, e, f, reps) * 1e6
-            shape_timings[f"{SHAPE_NAMES[k]}={trial[k]}"] = us
-            if us < best_us:
-                best_us, tiles = us, trial
+        for k, values in enumerate(shapes_h):
+            best_k = list(tiles)
+            for v in values:
+                if v == tiles[k]:
+                    continue
+                trial = list(tiles)
+                trial[k] = v
+                if trial[4] and trial[1] != shapes_h[1][0]:
+                    continue             # adjoint variants exist for the default tile only
+                plan.set_tiles(*trial)
+                us = time_evaluation(plan, positions, e, f, reps) * 1e6
+                shape_timings[f"{SHAPE_NAMES[k]}={v}"] = us
+                if us < best_us:
+                    best_us, best_k = us, trial
+            tiles = best_k
         plan.set_tiles(*tiles)
     _CACHE[key] = {"blocks": best, "tiles": list(plan.tiles)}
     if cache_file:

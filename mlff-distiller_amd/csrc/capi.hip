@@ -242,7 +242,7 @@ void set_default_shapes(mlff_plan* pl) {
   pl->buf.tile_bwd = default_shape(H, 1);
   pl->buf.unroll_fwd = default_shape(H, 2);
   pl->buf.unroll_bwd = default_shape(H, 3);
-  pl->buf.bwd_prefetch = default_shape(H, 4);
+  pl->buf.bwd_variant = default_shape(H, 4);
 }
 
 template <typename T>
@@ -555,7 +555,7 @@ int mlff_plan_set_blocks(mlff_plan* pl, int blocks) {
 }
 
 int mlff_plan_set_tiles(mlff_plan* pl, int fwd_pairs, int bwd_pairs, int msg_unroll_fwd,
-                        int msg_unroll_bwd, int bwd_prefetch) {
+                        int msg_unroll_bwd, int bwd_variant) {
   if (!pl) { set_error("mlff_plan_set_tiles: null plan"); return MLFF_ERR_INVALID; }
   const int H = pl->cfg.hidden_dim;
   const int want[4] = {fwd_pairs, bwd_pairs, msg_unroll_fwd, msg_unroll_bwd};
@@ -569,12 +569,12 @@ int mlff_plan_set_tiles(mlff_plan* pl, int fwd_pairs, int bwd_pairs, int msg_unr
       return MLFF_ERR_UNSUPPORTED;
     }
   }
-  if (bwd_prefetch != 0 && bwd_prefetch != 1) {
-    set_error("mlff_plan_set_tiles: bwd_prefetch must be 0 or 1");
+  if (bwd_variant < 0 || bwd_variant > alternative_shape(H, 4)) {
+    set_error("mlff_plan_set_tiles: bwd_variant must be 0..%d", alternative_shape(H, 4));
     return MLFF_ERR_INVALID;
   }
-  if (bwd_prefetch && got[1] != default_shape(H, 1)) {
-    set_error("mlff_plan_set_tiles: the gather prefetch exists for the default adjoint tile "
+  if (bwd_variant && got[1] != default_shape(H, 1)) {
+    set_error("mlff_plan_set_tiles: filter-adjoint variants exist for the default adjoint tile "
               "(%d pairs) only", default_shape(H, 1));
     return MLFF_ERR_UNSUPPORTED;
   }
@@ -582,7 +582,7 @@ int mlff_plan_set_tiles(mlff_plan* pl, int fwd_pairs, int bwd_pairs, int msg_unr
   pl->buf.tile_bwd = got[1];
   pl->buf.unroll_fwd = got[2];
   pl->buf.unroll_bwd = got[3];
-  pl->buf.bwd_prefetch = bwd_prefetch;
+  pl->buf.bwd_variant = bwd_variant;
   // captured graphs hold the old launches
   if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }

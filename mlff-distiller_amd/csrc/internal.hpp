@@ -56,7 +56,7 @@ struct Buffers {
   // flight per message row (forward, adjoint); default_shape() or
   // alternative_shape() of each
   int tile_fwd, tile_bwd, unroll_fwd, unroll_bwd;
-  int bwd_prefetch;   // filter adjoint loads its next tile's first gather item early
+  int bwd_variant;    // filter adjoint: 0 plain, 1 next-tile pair-index prefetch
   const float* w;                   // weight blob
   const ModelW* mw;
   // bf16 three-plane copies of the filter's second layer, per block l:
@@ -90,9 +90,9 @@ __host__ __device__ inline int64_t wbf_f2w(int H, int l) { return (int64_t)l * 3
 __host__ __device__ inline int64_t wbf_elems(int H, int L) { return (int64_t)L * 3 * wbf_plane(H); }
 // Launch shapes (which: 0 filter-forward pairs per tile, 1 filter-adjoint
 // pairs per tile, 2 message-forward edges in flight, 3 message-adjoint edges
-// in flight, 4 filter-adjoint gather prefetch across tiles, default tile
-// only): the default for hidden size H and the one pre-instantiated
-// alternative.
+// in flight, 4 filter-adjoint variant, default tile only): the default for
+// hidden size H and the pre-instantiated alternative (kind 4: the highest
+// variant number; every value up to it exists).
 int default_shape(int H, int which);
 int alternative_shape(int H, int which);
 // Split the filter weights into their bf16 planes (once per bind; synchronous).

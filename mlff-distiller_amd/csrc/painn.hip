@@ -1098,7 +1098,7 @@ void launch_filter_bwd(const Buffers& b, hipStream_t st, int l, const float* gS1
     filter_bwd_kernel<H, TM, false><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, mw.layer[l], centers, widths, b.R,
         mw.RP, b.rc, Wl, Sl, Vl, gS1, gV1, b.pacc, b.gvec, first, last);
-  } else if (b.bwd_prefetch) {
+  } else if (b.bwd_variant == 1) {
     constexpr int TM = kBwdPairs<H>;
     filter_bwd_kernel<H, TM, true><<<pair_grid(b, TM), kPairBlock, 0, st>>>(
         b.counts, b.pcap, b.pair_lo, b.pair_hi, b.geom, b.w, mw.layer[l], centers, widths, b.R,

@@ -310,20 +310,36 @@ __global__ __launch_bounds__(kTrainBlock) void edge_grad_kernel(
 }
 
 // out[k][c] = sum over rows r in [seg_ptr[k], seg_ptr[k+1]) of
-// src[perm ? perm[r] : r][c], added in row order (float64 or float32 sums).
+// src[perm ? perm[r] : r][c].  One workgroup per (segment, chunk of CW
+// columns): its 256 threads are CW columns x G = 256/CW row groups; group g
+// adds rows g, g + G, ... of the segment in order, then the G partials of a
+// column are added in group order (a fixed partition and order, so the sum
+// is bit-reproducible; float64 or float32 accumulation).
 template <typename T>
 __global__ __launch_bounds__(kTrainBlock) void segment_sum_kernel(
-    int n_seg, int width, const int32_t* __restrict__ seg_ptr, const int32_t* __restrict__ perm,
+    int width, int cw, const int32_t* __restrict__ seg_ptr, const int32_t* __restrict__ perm,
     const float* __restrict__ src, T* __restrict__ out) {
-  const int64_t idx = (int64_t)blockIdx.x * kTrainBlock + threadIdx.x;
-  if (idx >= (int64_t)n_seg * width) return;
-  const int k = (int)(idx / width), c = (int)(idx - (int64_t)k * width);
+  __shared__ T part[kTrainBlock];
+  const int k = blockIdx.x;
+  const int G = kTrainBlock / cw;
+  const int col = threadIdx.x % cw, grp = threadIdx.x / cw;
+  const int c = blockIdx.y * cw + col;
   T acc = 0;
-  for (int r = seg_ptr[k]; r < seg_ptr[k + 1]; ++r) {
-    const int row = perm ? perm[r] : r;
-    acc += (T)src[(size_t)row * width + c];
+  if (c < width) {
+    const int r1 = seg_ptr[k + 1];
+#pragma unroll 4
+    for (int r = seg_ptr[k] + grp; r < r1; r += G) {
+      const int row = perm ? perm[r] : r;
+      acc += (T)src[(size_t)row * width + c];
+    }
   }
-  out[idx] = acc;
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (grp == 0 && c < width) {
+    T sum = 0;
+    for (int q = 0; q < G; ++q) sum += part[q * cw + col];
+    out[(size_t)k * width + c] = sum;
+  }
 }
 
 template <int H>
@@ -602,14 +618,17 @@ int mlff_segment_sum(void* stream, int n_segments, int width, const int32_t* seg
     return MLFF_ERR_INVALID;
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int64_t total = (int64_t)n_segments * width;
-  const int grid = (int)((total + kTrainBlock - 1) / kTrainBlock);
+  // narrow column slabs: a long segment (one species' atoms) is then read by
+  // many row groups at once rather than by a few long dependent chains
+  int cw = 1;
+  while (cw < width && cw < 8) cw *= 2;
+  const dim3 grid((unsigned)n_segments, (unsigned)((width + cw - 1) / cw));
   if (out64_dev)
-    segment_sum_kernel<double><<<grid, kTrainBlock, 0, st>>>(n_segments, width, seg_ptr_dev,
-                                                             perm_dev, src_dev, out64_dev);
+    segment_sum_kernel<double><<<grid, kTrainBlock, 0, st>>>(width, cw, seg_ptr_dev, perm_dev,
+                                                             src_dev, out64_dev);
   else
-    segment_sum_kernel<float><<<grid, kTrainBlock, 0, st>>>(n_segments, width, seg_ptr_dev,
-                                                            perm_dev, src_dev, out32_dev);
+    segment_sum_kernel<float><<<grid, kTrainBlock, 0, st>>>(width, cw, seg_ptr_dev, perm_dev,
+                                                            src_dev, out32_dev);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("mlff_segment_sum: %s", hipGetErrorString(e));

@@ -204,11 +204,16 @@ def launch_shapes(H: int):
     """The pre-instantiated launch shapes of hidden size H, default first:
     pairs per filter-forward tile, pairs per filter-adjoint tile, edges in
     flight per message-forward row, per message-adjoint row, and the filter
-    adjoint's cross-tile gather prefetch (0/1; default adjoint tile only)
-    (mlff_plan_set_tiles)."""
+    adjoint's variant (0 plain, 1 next-tile index prefetch; default adjoint
+    tile only) (mlff_plan_set_tiles)."""
     small = H <= 32
     return ((128, 64) if small else (64, 32), (64, 32) if small else (32, 64), (5, 3), (4, 2),
             (0, 1))
+
+
+def default_tiles(H: int) -> Tuple[int, ...]:
+    """The default of every launch-shape kind of hidden size H."""
+    return tuple(values[0] for values in launch_shapes(H))
 
 
 class Plan:
@@ -217,7 +222,7 @@ class Plan:
     def __init__(self, H, L, R, max_z, cutoff, layout: BatchLayout, numbers: torch.Tensor,
                  weights: torch.Tensor, edge_capacity: int, pair_capacity: Optional[int] = None,
                  persistent_blocks: int = 0, streams: Optional[int] = None,
-                 tiles: Optional[Tuple[int, int, int, int]] = None):
+                 tiles: Optional[Tuple[int, ...]] = None):
         self.lib = C.lib()
         self.layout = layout
         self.H, self.L, self.R, self.max_z, self.cutoff = H, L, R, max_z, float(cutoff)
@@ -248,20 +253,19 @@ class Plan:
         self.streams = int(streams if streams is not None else DEFAULT_STREAMS)
         if self.streams != 2:
             self.set_streams(self.streams)
-        self.tiles = tuple(d for d, _ in launch_shapes(H))
+        self.tiles = default_tiles(H)
         if tiles is not None:
             self.set_tiles(*tiles)
 
     def set_tiles(self, fwd_pairs: int = 0, bwd_pairs: int = 0, msg_unroll_fwd: int = 0,
-                  msg_unroll_bwd: int = 0, bwd_prefetch: int = 0):
+                  msg_unroll_bwd: int = 0, bwd_variant: int = 0):
         """Launch shapes of the pair and message kernels (0: default); each
         must be one of launch_shapes(H).  Bit-identical results either way."""
         C.check("mlff_plan_set_tiles", self.lib.mlff_plan_set_tiles(
             self.handle, int(fwd_pairs), int(bwd_pairs), int(msg_unroll_fwd),
-            int(msg_unroll_bwd), int(bwd_prefetch)))
-        want = (fwd_pairs, bwd_pairs, msg_unroll_fwd, msg_unroll_bwd)
-        self.tiles = tuple(int(w) if w else d[0] for w, d in zip(want, launch_shapes(self.H))) \
-            + (int(bwd_prefetch),)
+            int(msg_unroll_bwd), int(bwd_variant)))
+        want = (fwd_pairs, bwd_pairs, msg_unroll_fwd, msg_unroll_bwd, bwd_variant)
+        self.tiles = tuple(int(w) if w else d for w, d in zip(want, default_tiles(self.H)))
 
     def _apply_cell_grid(self):
         if int(self.cell_grid[:, 0].max(initial=0)) > 0:

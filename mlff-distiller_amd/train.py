@@ -253,11 +253,25 @@ def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
         return a.T @ b
     nc = K // chunk
     head = nc * chunk
-    at = a[:head].reshape(nc, chunk, a.shape[1]).transpose(1, 2).contiguous()
-    out = torch.bmm(at, b[:head].reshape(nc, chunk, b.shape[1]).contiguous()).sum(0)
+    a3 = a[:head].reshape(nc, chunk, a.shape[1])
+    b3 = b[:head].reshape(nc, chunk, b.shape[1]).contiguous()
+    out = torch.bmm(a3.transpose(1, 2), b3).sum(0)
     if head < K:
         out = out + a[head:].T @ b[head:]
     return out
+
+
+def _wsum(w: Optional[Tensor], t: Tensor) -> Tensor:
+    """sum over rows of t weighted by w (plain sum when w is None)."""
+    if w is None:
+        return t.sum(0)
+    return w @ t.reshape(t.shape[0], -1)
+
+
+def _split3(t: Tensor, H: int):
+    """The three H-wide column blocks of [N, 3H] as contiguous tensors (the
+    elementwise kernels that use them then run vectorised)."""
+    return [x.contiguous() for x in t.split(H, dim=1)]
 
 
 def _lin(x, p, name):
@@ -275,8 +289,12 @@ def _mixT(M, v):
 
 
 def _xyz_outer(a, b):
-    """sum_{n,h} a[n,x,h] b[n,z,h] -> [3,3] (gradient of a 3x3 mixing matrix)."""
-    return _tmm(a.permute(0, 2, 1).reshape(-1, 3), b.permute(0, 2, 1).reshape(-1, 3))
+    """sum_{n,h} a[n,x,h] b[n,z,h] -> [3,3] (gradient of a 3x3 mixing matrix):
+    the [3H, 3H] product of the rows [N, 3H] (contiguous views, no copy),
+    then the sum of the diagonal of each H x H block."""
+    n, _, H = a.shape
+    full = _tmm(a.reshape(n, 3 * H), b.reshape(n, 3 * H))
+    return full.view(3, H, 3, H).diagonal(dim1=1, dim2=3).sum(-1)
 
 
 # ---------------------------------------------------------------------------
@@ -313,7 +331,7 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
         a2 = _lin(xin, p, upre + ".update_mlp.0")
         h2 = _silu(a2)
         y = _lin(h2, p, upre + ".update_mlp.2")
-        ds, g1, g2 = y[:, :H], y[:, H:2 * H], y[:, 2 * H:]
+        ds, g1, g2 = _split3(y, H)
         M = p[upre + ".mixing_matrix"]
         Mv = _mix(M, v_m)
         s_new = s_m + ds
@@ -365,8 +383,9 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
         dMv = _mix(lay["M"], dvm)
         lay.update(dsm=dsm, dvm=dvm, dnrm=dnrm, dxin=dxin, da2=da2, dh2=dh2, dy=dy, dMv=dMv)
         g1, g2 = lay["g1"], lay["g2"]
-        dg1, dg2 = dy[:, H:2 * H], dy[:, 2 * H:]
-        ds_ = (dsm + dy[:, :H]).contiguous()
+        dds, dg1, dg2 = _split3(dy, H)
+        lay["dg1"], lay["dg2"] = dg1, dg2
+        ds_ = dsm + dds
         dv_ = (dvm * g1.unsqueeze(1) + v_m * dg1.unsqueeze(1) + dMv * g2.unsqueeze(1)
                + lay["Mv"] * dg2.unsqueeze(1)).contiguous()
     dr1 = ds_ @ p["energy_head.0.weight"].T
@@ -413,6 +432,7 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         store[name] = store[name] + val if name in store else val
 
     gE = gE.to(torch.float32)
+    w_atom = w_pair = None
     if gw is not None:
         gw = gw.to(torch.float32)
         w_atom = gw[g.batch]
@@ -426,21 +446,22 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     geps = gE[g.batch]
     r1, h3, r2, h4, sL = c["r1"], c["h3"], c["r2"], c["h4"], c["sL"]
     Q3 = p["energy_head.4.weight"]
-    acc(grads, "energy_head.4.weight", (wa(geps).unsqueeze(1) * h4).sum(0, keepdim=True))
-    acc(grads, "energy_head.4.bias", wa(geps).sum().reshape(1))
+    wgeps = wa(geps)
+    acc(grads, "energy_head.4.weight", (wgeps @ h4).reshape(1, -1))
+    acc(grads, "energy_head.4.bias", wgeps.sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
     gr2 = gh4 * _silu1(r2)
     acc(grads, "energy_head.2.weight", _tmm(wa(gr2), h3))
-    acc(grads, "energy_head.2.bias", wa(gr2).sum(0))
+    acc(grads, "energy_head.2.bias", _wsum(w_atom, gr2))
     gh3 = gr2 @ p["energy_head.2.weight"]
     gr1 = gh3 * _silu1(r1)
     acc(grads, "energy_head.0.weight", _tmm(wa(gr1), sL))
-    acc(grads, "energy_head.0.bias", wa(gr1).sum(0))
+    acc(grads, "energy_head.0.bias", _wsum(w_atom, gr1))
     gs = (gr1 @ p["energy_head.0.weight"]).contiguous()
     gv = torch.zeros(n, 3, H, dtype=torch.float32, device=dev)
     if T:
         dr1, dh3, dr2, dh4 = c["dr1"], c["dh3"], c["dr2"], c["dh4"]
-        acc(tg, "energy_head.4.weight", (geps.unsqueeze(1) * dh4).sum(0, keepdim=True))
+        acc(tg, "energy_head.4.weight", (geps @ dh4).reshape(1, -1))
         acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=torch.float32, device=dev))
         dgr2 = gh4 * _silu2(r2) * dr2
         acc(tg, "energy_head.2.weight", _tmm(dgr2, h3) + _tmm(gr2, dh3))
@@ -468,14 +489,14 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gg1 = (gv * v_m).sum(1)
         gg2 = (gv * Mv).sum(1)
         gy = torch.cat([gs, gg1, gg2], dim=-1)
-        acc(grads, upre + ".update_mlp.2.weight", _tmm(wa(gy), lay["h2"]))
-        acc(grads, upre + ".update_mlp.2.bias", wa(gy).sum(0))
+        acc(grads, upre + ".update_mlp.2.weight", _tmm(gy, wa(lay["h2"])))
+        acc(grads, upre + ".update_mlp.2.bias", _wsum(w_atom, gy))
         gh2 = gy @ U2
         ga2 = gh2 * _silu1(lay["a2"])
         acc(grads, upre + ".update_mlp.0.weight", _tmm(wa(ga2), lay["xin"]))
-        acc(grads, upre + ".update_mlp.0.bias", wa(ga2).sum(0))
+        acc(grads, upre + ".update_mlp.0.bias", _wsum(w_atom, ga2))
         gx = ga2 @ U1
-        gs_x, gn = gx[:, :H], gx[:, H:]
+        gs_x, gn = [t.contiguous() for t in gx.split(H, dim=1)]
         gvg2 = gv * g2.unsqueeze(1)
         acc(grads, upre + ".mixing_matrix", _xyz_outer(wa(gvg2), v_m))
         safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
@@ -485,7 +506,7 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         gv_m = (gv * g1.unsqueeze(1) + _mixT(M, gvg2) + gn.unsqueeze(1) * unit_n).contiguous()
         if T:
             dvm, dMv, dxin = lay["dvm"], lay["dMv"], lay["dxin"]
-            dg1, dg2 = lay["dy"][:, H:2 * H], lay["dy"][:, 2 * H:]
+            dg1, dg2 = lay["dg1"], lay["dg2"]
             dgg1 = (dgv * v_m + gv * dvm).sum(1)
             dgg2 = (dgv * Mv + gv * dMv).sum(1)
             dgy = torch.cat([dgs, dgg1, dgg2], dim=-1)
@@ -496,7 +517,7 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             acc(tg, upre + ".update_mlp.0.weight", _tmm(dga2, lay["xin"]) + _tmm(ga2, dxin))
             acc(tg, upre + ".update_mlp.0.bias", dga2.sum(0))
             dgx = dga2 @ U1
-            dgs_x, dgn = dgx[:, :H], dgx[:, H:]
+            dgs_x, dgn = [t.contiguous() for t in dgx.split(H, dim=1)]
             dgvg2 = dgv * g2.unsqueeze(1) + gv * dg2.unsqueeze(1)
             acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m) + _xyz_outer(gvg2, dvm))
             dnrm = lay["dnrm"]
@@ -515,12 +536,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             gW, _ = _pair_grad(g, H, u, s, v, gs_m, gv_m)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        acc(grads, mpre + ".2.weight", _tmm(wp(gW), lay["h"]))
-        acc(grads, mpre + ".2.bias", wp(gW).sum(0))
+        acc(grads, mpre + ".2.weight", _tmm(gW, wp(lay["h"])))
+        acc(grads, mpre + ".2.bias", _wsum(w_pair, gW))
         gh = gW @ W2
         ga1 = gh * _silu1(lay["a1"])
-        acc(grads, mpre + ".0.weight", _tmm(wp(ga1), c["e"]))
-        acc(grads, mpre + ".0.bias", wp(ga1).sum(0))
+        acc(grads, mpre + ".0.weight", _tmm(ga1, wp(c["e"])))
+        acc(grads, mpre + ".0.bias", _wsum(w_pair, ga1))
         if rbf_grads:
             ge = ga1 @ W1
             acc(grads, "rbf.centers", (wp(ge) * e_mu).sum(0))

@@ -154,3 +154,15 @@ def test_static_entry_rejects_before_touching_the_device():
     assert "needs %d bytes" % need in msg and need > C.STATIC_SCRATCH_BYTES
     bad = C.Config(128, 3, 20, 118, 5.0, 0, 1, 10, 10, 0, 0, 0)
     assert lib.mlff_energy_forces_static(None, ctypes.byref(bad), *([fake] * 12)) == -1
+
+
+@pytest.mark.parametrize("H", [32, 64, 128])
+def test_launch_shape_defaults_are_the_first_values(H):
+    """Each launch-shape kind lists its default first; a plan starts on the
+    defaults (five kinds: filter pairs forward and adjoint, message edges in
+    flight forward and adjoint, filter adjoint variant 0)."""
+    engine = pkg_mod("engine")
+    shapes = engine.launch_shapes(H)
+    assert len(shapes) == 5 and all(len(v) >= 2 for v in shapes)
+    tiles = engine.default_tiles(H)
+    assert tiles == tuple(v[0] for v in shapes) and tiles[4] == 0

@@ -723,7 +723,7 @@ def test_forward_with_analytical_forces_matches_oracle(gpu):
 def test_every_launch_shape_is_bit_identical(gpu, H, L, R_):
     """The launch-shape tuner picks pairs per filter tile (forward and
     adjoint), edges in flight per message row and the filter adjoint's
-    index prefetch among pre-instantiated variants; each output element sums
+    variant among pre-instantiated kernels; each output element sums
     in the same order in all of them, so
     every combination gives the bits of the default (energies, forces and
     virials); a shape that was not instantiated is refused."""
@@ -743,7 +743,7 @@ def test_every_launch_shape_is_bit_identical(gpu, H, L, R_):
     cap = engine.edge_capacity_bound(sizes, False)
     plan = engine.Plan(H, L, R_, model.max_z, 5.0, lay, Zt, w, cap)
     shapes = engine.launch_shapes(H)
-    assert all(len(v) == 2 for v in shapes)
+    assert all(len(v) >= 2 for v in shapes)
 
     def run():
         e = torch.empty(len(sizes), dtype=torch.float64, device=dev)
@@ -757,7 +757,7 @@ def test_every_launch_shape_is_bit_identical(gpu, H, L, R_):
     import itertools
     for combo in itertools.product(*shapes):
         if combo[4] and combo[1] != shapes[1][0]:
-            with pytest.raises(C.MlffError):       # prefetch exists for the default tile only
+            with pytest.raises(C.MlffError):       # variants exist for the default tile only
                 plan.set_tiles(*combo)
             continue
         plan.set_tiles(*combo)
