@@ -386,11 +386,81 @@ int mlff_edge_grad(void* stream, int hidden_dim, int n_edges, const int32_t* src
 /* Fixed-order segment sums, replacing index_add_ reductions (per-structure
  * energies, student_model.py:739-755; the embedding gradient):
  * out[k][c] = sum over r in [seg_ptr[k], seg_ptr[k+1]) of
- * src[perm ? perm[r] : r][c], added in r order, into exactly one of
- * out64_dev (float64) or out32_dev (float32), [n_segments][width]. */
+ * src[perm ? perm[r] : r][c], into exactly one of out64_dev (float64) or
+ * out32_dev (float32), [n_segments][width].  The rows are added in a fixed
+ * partition and order (strided row groups, then the groups in order). */
 int mlff_segment_sum(void* stream, int n_segments, int width, const int32_t* seg_ptr_dev,
                      const int32_t* perm_dev, const float* src_dev, double* out64_dev,
                      float* out32_dev);
+
+/* Weight and bias gradients of a Linear layer over K rows (pairs or atoms),
+ * the parameter-gradient products of the reference's autograd backward
+ * through nn.Linear (student_model.py:318 and 350 rbf_to_scalar, 407 and 444
+ * update_mlp, 599 and 736 energy_head; the 3x3 mixing_matrix, 414 and 460, as
+ * the block-diagonal sums of one [3H][3H] product):
+ *   out[m][n] = sum_k w_k (a0[k][m] b0[k][n] + a1[k][m] b1[k][n])   [M][N]
+ *   bias[m]   = sum_k w_k a0[k][m]                                  [M]
+ * a0, a1 [K][lda], b0, b1 [K][ldb] row-major float32; a1/b1 NULL for one
+ * term, row_w_dev [K] NULL for w = 1, bias_out_dev NULL to skip the bias
+ * (K = 0 writes zeros; a0 and b0 may then be NULL).
+ * M and N multiples of 4, strides multiples of 4, buffers 16-byte aligned.
+ * The K rows are split at a partition that depends on (K, M, N) only and the
+ * partial sums are added in a fixed order (bit-reproducible).  ws_dev is
+ * device scratch of at least mlff_weight_grad_workspace_bytes(K, M, N). */
+size_t mlff_weight_grad_workspace_bytes(int K, int M, int N);
+int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
+                     const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
+                     const float* b1_dev, int ldb1, const float* row_w_dev, void* ws_dev,
+                     size_t ws_bytes, float* out_dev, float* bias_out_dev);
+
+/* SiLU of the MLPs (student_model.py:318-322, 407-411, 599-605) and its
+ * derivatives, elementwise over n float32 values (n a multiple of 4, buffers
+ * 16-byte aligned), each output skipped when its pointer is NULL:
+ *   h = silu(a),  dh = silu'(a) da,  ga = g silu'(a),
+ *   dga = dg silu'(a) + g silu''(a) da   (dg NULL: zero). */
+int mlff_silu_chain(void* stream, int64_t n, const float* a_dev, const float* da_dev,
+                    const float* g_dev, const float* dg_dev, float* h_dev, float* dh_dev,
+                    float* ga_dev, float* dga_dev);
+
+/* Elementwise parts of PaiNNUpdate.forward (student_model.py:430-470) in the
+ * training pass, around the update MLP's two products.  s [N][H],
+ * v [N][3][H], xin [N][2H], y = MLP(xin) = (ds, g1, g2) [N][3H], M the 3x3
+ * mixing matrix (row-major, device); hidden a multiple of 4, buffers
+ * 16-byte aligned float32.
+ *   update_norm:          xin = (s, |v|_xyz)
+ *   update_apply:         s' = s + ds, Mv = M v, v' = v g1 + Mv g2
+ *   update_norm_tangent:  dxin = (ds, dnrm), dnrm = sum_x v dv / |v| (0 at |v| = 0)
+ *   update_apply_tangent: ds' = ds + dds, dMv = M dv,
+ *                         dv' = dv g1 + v dg1 + dMv g2 + Mv dg2   (dy = (dds, dg1, dg2))
+ *   update_gate_adjoint:  gy = (gs, sum_x gv v, sum_x gv Mv), and with dgy_dev
+ *                         non-NULL its tangent along (dgs, dgv, dv, dMv)
+ *   update_input_adjoint: with gx = (gs_x, gn) = dL/dxin: gvg2 = gv g2,
+ *                         gs_m = gs + gs_x, gv_m = gv g1 + M^T gvg2 + gn v/|v|,
+ *                         and with dgs_out_dev non-NULL their tangents along
+ *                         (dgx, dgs, dgv, dy, dv, dxin). */
+int mlff_update_norm(void* stream, int n_atoms, int hidden, const float* s_dev,
+                     const float* v_dev, float* xin_dev);
+int mlff_update_apply(void* stream, int n_atoms, int hidden, const float* m_dev,
+                      const float* y_dev, const float* s_dev, const float* v_dev,
+                      float* s_out_dev, float* v_out_dev, float* mv_out_dev);
+int mlff_update_norm_tangent(void* stream, int n_atoms, int hidden, const float* v_dev,
+                             const float* xin_dev, const float* ds_dev, const float* dv_dev,
+                             float* dxin_dev);
+int mlff_update_apply_tangent(void* stream, int n_atoms, int hidden, const float* m_dev,
+                              const float* y_dev, const float* dy_dev, const float* ds_dev,
+                              const float* v_dev, const float* dv_dev, const float* mv_dev,
+                              float* ds_out_dev, float* dv_out_dev, float* dmv_out_dev);
+int mlff_update_gate_adjoint(void* stream, int n_atoms, int hidden, const float* gs_dev,
+                             const float* gv_dev, const float* v_dev, const float* mv_dev,
+                             const float* dgs_dev, const float* dgv_dev, const float* dv_dev,
+                             const float* dmv_dev, float* gy_dev, float* dgy_dev);
+int mlff_update_input_adjoint(void* stream, int n_atoms, int hidden, const float* m_dev,
+                              const float* gx_dev, const float* gs_dev, const float* gv_dev,
+                              const float* y_dev, const float* v_dev, const float* xin_dev,
+                              const float* dgx_dev, const float* dgs_dev, const float* dgv_dev,
+                              const float* dy_dev, const float* dv_dev, const float* dxin_dev,
+                              float* gs_out_dev, float* gv_out_dev, float* gvg2_out_dev,
+                              float* dgs_out_dev, float* dgv_out_dev, float* dgvg2_out_dev);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,9 @@ EXPORTS = (
     "mlff_mts_run", "mlff_mts_graph_run", "mlff_plan_set_streams", "mlff_energy_forces_static",
     "mlff_train_message", "mlff_train_message_adjoint", "mlff_train_pair_grad",
     "mlff_segment_sum", "mlff_plan_set_tiles", "mlff_edge_message", "mlff_edge_grad",
+    "mlff_weight_grad_workspace_bytes", "mlff_weight_grad", "mlff_silu_chain",
+    "mlff_update_norm", "mlff_update_apply", "mlff_update_norm_tangent",
+    "mlff_update_apply_tangent", "mlff_update_gate_adjoint", "mlff_update_input_adjoint",
 )
 
 # bytes of the static device scratch behind mlff_energy_forces_static
@@ -149,6 +152,19 @@ def _declare(lib):
     lib.mlff_edge_grad.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 10
     lib.mlff_segment_sum.restype = ctypes.c_int
     lib.mlff_segment_sum.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 5
+    lib.mlff_weight_grad_workspace_bytes.restype = ctypes.c_size_t
+    lib.mlff_weight_grad_workspace_bytes.argtypes = [ctypes.c_int] * 3
+    lib.mlff_weight_grad.restype = ctypes.c_int
+    lib.mlff_weight_grad.argtypes = ([vp] + [ctypes.c_int] * 3 + [vp, ctypes.c_int] * 4
+                                     + [vp, vp, ctypes.c_size_t, vp, vp])
+    lib.mlff_silu_chain.restype = ctypes.c_int
+    lib.mlff_silu_chain.argtypes = [vp, ctypes.c_int64] + [vp] * 8
+    for name, n_ptrs in (("mlff_update_norm", 3), ("mlff_update_apply", 7),
+                         ("mlff_update_norm_tangent", 5), ("mlff_update_apply_tangent", 10),
+                         ("mlff_update_gate_adjoint", 10), ("mlff_update_input_adjoint", 19)):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * n_ptrs
     lib.mlff_energy_forces_static.restype = ctypes.c_int
     lib.mlff_energy_forces_static.argtypes = [vp, ctypes.POINTER(Config)] + [vp] * 12
 

@@ -7,8 +7,9 @@
 // PaiNNMessage.forward (student_model.py:346-387).
 //
 // Every sum runs in a fixed order -- a receiver row adds its edges in
-// ascending neighbour order, a segment adds its rows in index order -- so
-// results are bit-reproducible; there are no atomics.
+// ascending neighbour order, segment and weight-gradient sums use a fixed
+// partition of their rows added in a fixed order -- so results are
+// bit-reproducible; there are no atomics.
 //
 // Row kernels: a group of H/4 lanes owns one receiver row, each lane four
 // channels (one f4) of every [.][H] plane; a 256-thread block holds 1024/H
@@ -342,6 +343,273 @@ __global__ __launch_bounds__(kTrainBlock) void segment_sum_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradients over K rows (pairs or atoms):
+//   out[m][n] = sum_k w_k (a0[k][m] b0[k][n] + a1[k][m] b1[k][n]),
+//   bias[m]   = sum_k w_k a0[k][m].
+//
+// One wave owns a 64 x 64 output tile over RW consecutive rows; the four waves
+// of a block take four consecutive row ranges of the same tile, and waves 1-3
+// hand their tiles to wave 0 through LDS, which adds them in wave order.
+// Block (tile, s) writes partial s; weight_grad_reduce_kernel adds the
+// partials in a fixed order.  The split depends on (K, M, N) only, so the
+// sums are bit-reproducible.  Blocks are numbered so that the tiles of one
+// row range run on one XCD: the rows they share are fetched into that XCD's
+// L2 once.
+//
+// Operands (v_mfma_f32_16x16x4f32): a k-step is four rows, lane group
+// g = lane/16 feeding row k0 + g.  Lane r = lane%16 loads the 16-byte runs
+// a[k][m0 + 4r .. 4r+3] and b[k][n0 + 4r .. 4r+3]; component q of the A run
+// is row i = r of 16x16 tile q (tile q holds columns m0 + 4i + q), and the
+// same for B.  A k-step is then two coalesced 16-byte loads per term and
+// 4 x 4 MFMAs, and
+//   acc[q][t][j] = out[m0 + 4 (4g + j) + q][n0 + 4r + t].
+// The operands of the next kWgUnroll k-steps are loaded before the MFMAs of
+// the current ones.
+// ---------------------------------------------------------------------------
+constexpr int kWgWaves = 4;
+constexpr int kWgMaxSplits = 256;
+// k-steps per operand batch (two terms: half as many, for two waves per SIMD)
+template <bool TWO> constexpr int kWgUnroll = TWO ? 2 : 4;
+constexpr int kWgRedCols = 8;  // float4 columns per block of the partial reduction
+
+__device__ __forceinline__ void wg_mfma(f4 (&acc)[4][4], const f4& a, const f4& b) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[t], acc[q][t], 0, 0, 0);
+  }
+}
+
+template <bool TWO, bool ROWW>
+struct WgBatch {
+  // raw operands of rows min(k0 + 4u + g, k_end - 1); the masks for rows at
+  // or past k_end and columns past M or N are applied when they are used.
+  // The loads are unconditional (column 0 where masked), so the compiler
+  // keeps count of them and the next batch stays in flight behind the MFMAs.
+  static constexpr int U = kWgUnroll<TWO>;
+  f4 a[U], b[U], a2[TWO ? U : 1], b2[TWO ? U : 1];
+  float w[ROWW ? U : 1];
+
+  __device__ __forceinline__ void load(int k0, int k_end, int g, int ma, int nb,
+                                       const float* __restrict__ a0, int lda0,
+                                       const float* __restrict__ b0, int ldb0,
+                                       const float* __restrict__ a1, int lda1,
+                                       const float* __restrict__ b1, int ldb1,
+                                       const float* __restrict__ roww) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t kc = (size_t)min(k0 + 4 * u + g, k_end - 1);
+      a[u] = ld4(a0 + kc * lda0 + ma);
+      b[u] = ld4(b0 + kc * ldb0 + nb);
+      if (TWO) {
+        a2[u] = ld4(a1 + kc * lda1 + ma);
+        b2[u] = ld4(b1 + kc * ldb1 + nb);
+      }
+      if (ROWW) w[u] = roww[kc];
+    }
+  }
+};
+
+template <bool TWO, bool ROWW, bool BIAS>
+__global__ __launch_bounds__(64 * kWgWaves) void weight_grad_kernel(
+    int K, int M, int N, int RW, int tiles_m, int tiles_n, int S,
+    const float* __restrict__ a0, int lda0, const float* __restrict__ b0, int ldb0,
+    const float* __restrict__ a1, int lda1, const float* __restrict__ b1, int ldb1,
+    const float* __restrict__ roww, float* __restrict__ part, float* __restrict__ part_bias) {
+  __shared__ __align__(16) f4 red[kWgWaves - 1][16 * 64];
+  __shared__ f4 bred[kWgWaves][64];
+  const int tiles = tiles_m * tiles_n;
+  const int logical = xcd_block();          // consecutive: the tiles of one row range
+  if (logical >= tiles * S) return;         // grid padding (whole block)
+  const int s = logical / tiles, tile = logical - s * tiles;
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  const int m0 = tm * 64, n0 = tn * 64;
+  const int k_begin = (s * kWgWaves + wave) * RW;
+  const int k_end = min(k_begin + RW, K);
+  const int mc = m0 + 4 * r, nc = n0 + 4 * r;
+  const bool mok = mc < M, nok = nc < N;
+  const int ma = mok ? mc : 0, nb = nok ? nc : 0;
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  f4 acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[q][t] = zero;
+  }
+  f4 cs = zero;   // column sums of w a0 (bias)
+  if (k_begin < k_end) {
+    // the last batch's successor is loaded too (clamped rows, never used)
+    WgBatch<TWO, ROWW> cur, nxt;
+    cur.load(k_begin, k_end, g, ma, nb, a0, lda0, b0, ldb0, a1, lda1, b1, ldb1, roww);
+    constexpr int U = kWgUnroll<TWO>;
+    for (int k0 = k_begin; k0 < k_end; k0 += 4 * U) {
+      nxt.load(k0 + 4 * U, k_end, g, ma, nb, a0, lda0, b0, ldb0, a1, lda1, b1, ldb1,
+               roww);
+      __builtin_amdgcn_sched_barrier(0);   // the loads above stay ahead of the MFMAs
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool kok = k0 + 4 * u + g < k_end;
+        f4 a = (kok && mok) ? cur.a[u] : zero;
+        const f4 b = (kok && nok) ? cur.b[u] : zero;
+        f4 a2 = zero, b2 = zero;
+        if (TWO) {
+          a2 = (kok && mok) ? cur.a2[u] : zero;
+          b2 = (kok && nok) ? cur.b2[u] : zero;
+        }
+        if (ROWW) {
+          const float w = kok ? cur.w[u] : 0.f;
+          a = a * w;
+          if (TWO) a2 = a2 * w;
+        }
+        if (BIAS) cs += a;
+        wg_mfma(acc, a, b);
+        if (TWO) wg_mfma(acc, a2, b2);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      cur = nxt;
+    }
+  }
+  // waves 1.. hand their tiles to wave 0 (f4 (q, j) of lane: columns t = 0..3)
+  if (wave > 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const f4 v = {acc[q][0][j], acc[q][1][j], acc[q][2][j], acc[q][3][j]};
+        red[wave - 1][(q * 4 + j) * 64 + lane] = v;
+      }
+    }
+  }
+  if (BIAS) bred[wave][lane] = cs;
+  __syncthreads();
+  if (wave != 0) return;
+  float* dst = part + (size_t)s * M * N;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      f4 v = {acc[q][0][j], acc[q][1][j], acc[q][2][j], acc[q][3][j]};
+#pragma unroll
+      for (int w = 0; w < kWgWaves - 1; ++w) v += red[w][(q * 4 + j) * 64 + lane];
+      const int m = m0 + 4 * (4 * g + j) + q;
+      if (m < M && nok) st4(dst + (size_t)m * N + nc, v);
+    }
+  }
+  if (BIAS && tn == 0) {
+    // column m0 + lane: component q = lane % 4 of lane group r = lane / 4,
+    // added over waves then lane groups g in order
+    const int rr = lane >> 2, q = lane & 3;
+    float sum = 0.f;
+#pragma unroll
+    for (int w = 0; w < kWgWaves; ++w) {
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg) sum += bred[w][gg * 16 + rr][q];
+    }
+    if (m0 + lane < M) part_bias[(size_t)s * M + m0 + lane] = sum;
+  }
+}
+
+// out[i] = sum_s part[s][i] over S partials of n4 float4 elements.  A block
+// takes kWgRedCols elements; its G = 256 / kWgRedCols thread groups add the
+// partials s = q, q + G, ... in order, then the G group sums are added in
+// group order (a fixed partition and order).
+__global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_kernel(
+    int S, int n4, const f4* __restrict__ part, f4* __restrict__ out) {
+  constexpr int G = kTrainBlock / kWgRedCols;
+  __shared__ f4 grp[kTrainBlock];
+  const int col = threadIdx.x % kWgRedCols, q = threadIdx.x / kWgRedCols;
+  const int i = blockIdx.x * kWgRedCols + col;
+  f4 sum = {0.f, 0.f, 0.f, 0.f};
+  if (i < n4) {
+#pragma unroll 4
+    for (int s = q; s < S; s += G) sum += part[(size_t)s * n4 + i];
+  }
+  grp[threadIdx.x] = sum;
+  __syncthreads();
+  if (q == 0 && i < n4) {
+    f4 tot = grp[col];
+    for (int k = 1; k < G; ++k) tot += grp[k * kWgRedCols + col];
+    out[i] = tot;
+  }
+}
+
+// Number of partials S and rows per wave RW (a multiple of 4: whole k-steps):
+// at most kWgSlots blocks, the ones resident at once on MI355X (256 CUs, two
+// blocks of this kernel each), so the grid runs as a single round instead of
+// leaving a few blocks for a second one; at most kWgMaxSplits partials.
+constexpr long kWgSlots = 512;
+void weight_grad_split(int K, int M, int N, int& RW, int& S) {
+  const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
+  long s = kWgSlots / tiles;
+  if (s > kWgMaxSplits) s = kWgMaxSplits;
+  if (s < 1) s = 1;
+  long rw = ((long)K + 4 * s - 1) / (4 * s);
+  rw = (rw + 3) / 4 * 4;
+  if (rw < 4) rw = 4;
+  RW = (int)rw;
+  S = K > 0 ? (int)(((long)K + 4 * rw - 1) / (4 * rw)) : 1;
+}
+
+template <bool TWO, bool ROWW>
+void launch_weight_grad(hipStream_t st, bool bias, int blocks, int K, int M, int N, int RW,
+                        int tiles_m, int tiles_n, int S, const float* a0, int lda0,
+                        const float* b0, int ldb0, const float* a1, int lda1, const float* b1,
+                        int ldb1, const float* roww, float* part, float* part_bias) {
+  if (bias)
+    weight_grad_kernel<TWO, ROWW, true><<<blocks, 64 * kWgWaves, 0, st>>>(
+        K, M, N, RW, tiles_m, tiles_n, S, a0, lda0, b0, ldb0, a1, lda1, b1, ldb1, roww, part,
+        part_bias);
+  else
+    weight_grad_kernel<TWO, ROWW, false><<<blocks, 64 * kWgWaves, 0, st>>>(
+        K, M, N, RW, tiles_m, tiles_n, S, a0, lda0, b0, ldb0, a1, lda1, b1, ldb1, roww, part,
+        part_bias);
+}
+
+// SiLU and its derivatives along a forward tangent and a reverse adjoint
+// (student_model.py:318-322, 407-411, 599-605: the SiLU of every MLP),
+// elementwise over n values, four per thread:
+//   h = a s,  dh = silu'(a) da,  ga = g silu'(a),
+//   dga = dg silu'(a) + g silu''(a) da,
+// s = 1 / (1 + exp(-a)), silu' = s (1 + a (1 - s)),
+// silu'' = s (1 - s) (2 + a (1 - 2 s)).  1 - s is evaluated as
+// 1 / (1 + exp(a)), not by subtraction, which would lose its low bits for
+// large positive a.  Outputs with a NULL pointer are skipped; a NULL dg is
+// zero.
+__global__ __launch_bounds__(kTrainBlock) void silu_chain_kernel(
+    int n4, const f4* __restrict__ a, const f4* __restrict__ da, const f4* __restrict__ g,
+    const f4* __restrict__ dg, f4* __restrict__ h, f4* __restrict__ dh, f4* __restrict__ ga,
+    f4* __restrict__ dga) {
+  const int i = blockIdx.x * kTrainBlock + threadIdx.x;
+  if (i >= n4) return;
+  const f4 x = a[i];
+  f4 sg, sm, d1;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    sg[c] = 1.0f / (1.0f + expf(-x[c]));
+    sm[c] = 1.0f / (1.0f + expf(x[c]));
+    d1[c] = sg[c] * (1.0f + x[c] * sm[c]);
+  }
+  if (h) h[i] = x * sg;
+  const f4 dx = da ? da[i] : f4{0.f, 0.f, 0.f, 0.f};
+  if (dh) dh[i] = d1 * dx;
+  if (ga || dga) {
+    const f4 gx = g[i];
+    if (ga) ga[i] = gx * d1;
+    if (dga) {
+      f4 d2;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        d2[c] = sg[c] * sm[c] * (2.0f + x[c] * (sm[c] - sg[c]));
+      dga[i] = dg ? dg[i] * d1 + gx * d2 * dx : gx * d2 * dx;
+    }
+  }
+}
+
 template <int H>
 hipError_t message_impl(hipStream_t st, int N, const int32_t* row_ptr, const int32_t* col,
                         const int32_t* edge_pair, const float* s, const float* v, const float* W,
@@ -632,6 +900,116 @@ int mlff_segment_sum(void* stream, int n_segments, int width, const int32_t* seg
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("mlff_segment_sum: %s", hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+size_t mlff_weight_grad_workspace_bytes(int K, int M, int N) {
+  if (K < 0 || M < 1 || N < 1) return 0;
+  int RW, S;
+  weight_grad_split(K, M, N, RW, S);
+  return (size_t)S * ((size_t)M * N + M) * sizeof(float);
+}
+
+int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
+                     const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
+                     const float* b1_dev, int ldb1, const float* row_w_dev, void* ws_dev,
+                     size_t ws_bytes, float* out_dev, float* bias_out_dev) {
+  if (K < 0 || M < 4 || N < 4 || M % 4 || N % 4) {
+    set_error("mlff_weight_grad: need K >= 0 and M, N positive multiples of 4 (K=%d M=%d N=%d)",
+              K, M, N);
+    return MLFF_ERR_INVALID;
+  }
+  const bool two = a1_dev != nullptr;
+  if ((K > 0 && (!a0_dev || !b0_dev)) || !out_dev || !ws_dev || two != (b1_dev != nullptr)) {
+    set_error("mlff_weight_grad: need a0, b0 (K > 0), out, workspace, and a1 and b1 together");
+    return MLFF_ERR_INVALID;
+  }
+  if (lda0 < M || ldb0 < N || lda0 % 4 || ldb0 % 4 ||
+      (two && (lda1 < M || ldb1 < N || lda1 % 4 || ldb1 % 4))) {
+    set_error("mlff_weight_grad: row strides must be multiples of 4 and at least M, N");
+    return MLFF_ERR_INVALID;
+  }
+  if (!aligned16(a0_dev) || !aligned16(b0_dev) || !aligned16(out_dev) || !aligned16(ws_dev) ||
+      (two && (!aligned16(a1_dev) || !aligned16(b1_dev))) ||
+      (bias_out_dev && !aligned16(bias_out_dev))) {
+    set_error("mlff_weight_grad: buffers must be 16-byte aligned");
+    return MLFF_ERR_INVALID;
+  }
+  const size_t need = mlff_weight_grad_workspace_bytes(K, M, N);
+  if (ws_bytes < need) {
+    set_error("mlff_weight_grad: workspace of %zu bytes, need %zu", ws_bytes, need);
+    return MLFF_ERR_INVALID;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int RW, S;
+  weight_grad_split(K, M, N, RW, S);
+  const int tiles_m = (M + 63) / 64, tiles_n = (N + 63) / 64;
+  // a multiple of 8 blocks, so that xcd_block() groups them by XCD
+  const int blocks = (tiles_m * tiles_n * S + 7) / 8 * 8;
+  float* part = static_cast<float*>(ws_dev);
+  float* part_bias = part + (size_t)S * M * N;
+  const bool bias = bias_out_dev != nullptr;
+  if (two && row_w_dev)
+    launch_weight_grad<true, true>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S, a0_dev,
+                                   lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev,
+                                   part, part_bias);
+  else if (two)
+    launch_weight_grad<true, false>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S, a0_dev,
+                                    lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev,
+                                    part, part_bias);
+  else if (row_w_dev)
+    launch_weight_grad<false, true>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S, a0_dev,
+                                    lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev,
+                                    part, part_bias);
+  else
+    launch_weight_grad<false, false>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S,
+                                     a0_dev, lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1,
+                                     row_w_dev, part, part_bias);
+  const int n4 = M * N / 4;
+  weight_grad_reduce_kernel<<<(n4 + kWgRedCols - 1) / kWgRedCols, kTrainBlock, 0, st>>>(
+      S, n4, reinterpret_cast<const f4*>(part), reinterpret_cast<f4*>(out_dev));
+  if (bias)
+    weight_grad_reduce_kernel<<<(M / 4 + kWgRedCols - 1) / kWgRedCols, kTrainBlock, 0, st>>>(
+        S, M / 4, reinterpret_cast<const f4*>(part_bias), reinterpret_cast<f4*>(bias_out_dev));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("mlff_weight_grad: %s", hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+int mlff_silu_chain(void* stream, int64_t n, const float* a_dev, const float* da_dev,
+                    const float* g_dev, const float* dg_dev, float* h_dev, float* dh_dev,
+                    float* ga_dev, float* dga_dev) {
+  if (n < 0 || n % 4 || n / 4 > INT32_MAX) {
+    set_error("mlff_silu_chain: n must be a non-negative multiple of 4");
+    return MLFF_ERR_INVALID;
+  }
+  if (!a_dev || (dh_dev && !da_dev) || (ga_dev && !g_dev) ||
+      (dga_dev && (!g_dev || !da_dev))) {
+    set_error("mlff_silu_chain: a missing input for a requested output");
+    return MLFF_ERR_INVALID;
+  }
+  const void* bufs[8] = {a_dev, da_dev, g_dev, dg_dev, h_dev, dh_dev, ga_dev, dga_dev};
+  for (const void* b : bufs)
+    if (!aligned16(b)) {
+      set_error("mlff_silu_chain: buffers must be 16-byte aligned");
+      return MLFF_ERR_INVALID;
+    }
+  if (n == 0) return MLFF_OK;
+  const int n4 = (int)(n / 4);
+  silu_chain_kernel<<<(n4 + kTrainBlock - 1) / kTrainBlock, kTrainBlock, 0,
+                      static_cast<hipStream_t>(stream)>>>(
+      n4, reinterpret_cast<const f4*>(a_dev), reinterpret_cast<const f4*>(da_dev),
+      reinterpret_cast<const f4*>(g_dev), reinterpret_cast<const f4*>(dg_dev),
+      reinterpret_cast<f4*>(h_dev), reinterpret_cast<f4*>(dh_dev), reinterpret_cast<f4*>(ga_dev),
+      reinterpret_cast<f4*>(dga_dev));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("mlff_silu_chain: %s", hipGetErrorString(e));
     return MLFF_ERR_HIP;
   }
   return MLFF_OK;

@@ -127,9 +127,14 @@ def _p(t: Optional[Tensor]) -> int:
 
 
 def _dense(t: Tensor) -> Tensor:
+    """t as a contiguous, 16-byte aligned float32 device buffer."""
     if t.dtype != torch.float32 or not t.is_cuda:
         raise TypeError("the training pass runs on float32 device tensors (HIP kernels)")
-    return t.contiguous()
+    if not t.is_contiguous():
+        t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()
+    return t
 
 
 def _message(g: PairGraph, H: int, s, v, W, u, ds=None, dv=None, dW=None, du=None):
@@ -197,18 +202,44 @@ def _segment_sum(seg_ptr: Tensor, src: Tensor, perm: Optional[Tensor] = None,
 # elementwise pieces and dense products
 # ---------------------------------------------------------------------------
 
+def _glue(name: str, *args):
+    """Call one of the update-block kernels (mlff_update_*): leading ints,
+    then tensors (made dense float32) or None."""
+    conv = []
+    for a in args:
+        if isinstance(a, Tensor):
+            a = _p(_dense(a))
+        elif a is None:
+            a = None
+        conv.append(a)
+    C.check(name, getattr(C.lib(), name)(C.stream_handle(), *conv))
+
+
+def _silu_call(a, da=None, g=None, dg=None, h=False, dh=False, dga=False):
+    a = _dense(a)
+    opt = lambda t: None if t is None else _dense(t)
+    da, g, dg = opt(da), opt(g), opt(dg)
+    outs = [torch.empty_like(a) if want else None for want in (h, dh, dga)]
+    C.check("mlff_silu_chain", C.lib().mlff_silu_chain(
+        C.stream_handle(), a.numel(), C.ptr(a), C.ptr(da), C.ptr(g), C.ptr(dg),
+        C.ptr(outs[0]), C.ptr(outs[1]), None, C.ptr(outs[2])))
+    return [o for o in outs if o is not None][0]
+
+
 def _silu(x):
-    return x * torch.sigmoid(x)
+    """silu(x) = x sigmoid(x) (mlff_silu_chain)."""
+    return _silu_call(x, h=True)
 
 
-def _silu1(x):
-    s = torch.sigmoid(x)
-    return s * (1 + x * (1 - s))
+def _silu_d(x, t):
+    """silu'(x) t: a tangent along t, or the adjoint of a gradient t."""
+    return _silu_call(x, da=t, dh=True)
 
 
-def _silu2(x):
-    s = torch.sigmoid(x)
-    return s * (1 - s) * (2 + x * (1 - 2 * s))
+def _silu_dd(x, g, dx, dg=None):
+    """dg silu'(x) + g silu''(x) dx: the tangent of the adjoint g silu'(x)
+    along (dx, dg); dg None is zero."""
+    return _silu_call(x, da=dx, g=g, dg=dg, dga=True)
 
 
 def _edge_features(d, centers, widths, rc):
@@ -243,57 +274,61 @@ def _edge_feature_param_derivs(d, centers, widths, rc):
     return fc * A, fc * Bw, dfc * A + fc * dA, dfc * Bw + fc * dBw
 
 
-def _tmm(a: Tensor, b: Tensor, chunk: int = 2048) -> Tensor:
-    """a.T @ b for tall a [K, M], b [K, N] (a weight gradient, K = pairs or
-    atoms): the K reduction is split into chunks computed as one batched
-    product and then summed, which keeps the GPU busy on one long
-    reduction and keeps every sum in a fixed order."""
-    K = a.shape[0]
-    if K < 4 * chunk:
-        return a.T @ b
-    nc = K // chunk
-    head = nc * chunk
-    a3 = a[:head].reshape(nc, chunk, a.shape[1])
-    b3 = b[:head].reshape(nc, chunk, b.shape[1]).contiguous()
-    out = torch.bmm(a3.transpose(1, 2), b3).sum(0)
-    if head < K:
-        out = out + a[head:].T @ b[head:]
-    return out
+def _wg_operand(t: Tensor) -> Tensor:
+    """A [K, C] float32 operand of mlff_weight_grad: unit column stride, row
+    stride and C multiples of 4 (zero columns appended), 16-byte aligned."""
+    t = t.reshape(t.shape[0], math.prod(t.shape[1:]))
+    if t.shape[1] % 4:
+        t = torch.nn.functional.pad(t, (0, 4 - t.shape[1] % 4))
+    if t.stride(1) != 1 or t.stride(0) % 4 or t.data_ptr() % 16:
+        t = t.contiguous()
+    return t
 
 
-def _wsum(w: Optional[Tensor], t: Tensor) -> Tensor:
-    """sum over rows of t weighted by w (plain sum when w is None)."""
-    if w is None:
-        return t.sum(0)
-    return w @ t.reshape(t.shape[0], -1)
-
-
-def _split3(t: Tensor, H: int):
-    """The three H-wide column blocks of [N, 3H] as contiguous tensors (the
-    elementwise kernels that use them then run vectorised)."""
-    return [x.contiguous() for x in t.split(H, dim=1)]
+def _wgrad(a0: Tensor, b0: Tensor, a1: Optional[Tensor] = None, b1: Optional[Tensor] = None,
+           w: Optional[Tensor] = None, bias: bool = False):
+    """sum_k w_k (a0[k] (x) b0[k] + a1[k] (x) b1[k]) [M, N] and, with
+    ``bias``, sum_k w_k a0[k] [M] over the K rows (pairs or atoms): a Linear
+    layer's weight and bias gradients, on the MFMA kernel with a fixed split
+    and order of the row sum (mlff_weight_grad).  Returns (weight, bias or
+    None)."""
+    M, N = a0.shape[1], b0.shape[1]
+    K = a0.shape[0]
+    A0, B0 = _wg_operand(a0), _wg_operand(b0)
+    A1 = B1 = None
+    if a1 is not None:
+        A1, B1 = _wg_operand(a1), _wg_operand(b1)
+    M4, N4 = A0.shape[1], B0.shape[1]
+    dev = a0.device
+    lib = C.lib()
+    nbytes = int(lib.mlff_weight_grad_workspace_bytes(K, M4, N4))
+    ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=dev)
+    out = torch.empty(M4, N4, dtype=torch.float32, device=dev)
+    bout = torch.empty(M4, dtype=torch.float32, device=dev) if bias else None
+    wv = None if w is None else w.to(torch.float32).contiguous()
+    C.check("mlff_weight_grad", lib.mlff_weight_grad(
+        C.stream_handle(), K, M4, N4, C.ptr(A0), A0.stride(0), C.ptr(B0), B0.stride(0),
+        C.ptr(A1), A1.stride(0) if A1 is not None else 0,
+        C.ptr(B1), B1.stride(0) if B1 is not None else 0,
+        C.ptr(wv), C.ptr(ws), ws.numel(), C.ptr(out), C.ptr(bout)))
+    if M4 != M or N4 != N:
+        out = out[:M, :N]
+    if bout is not None and M4 != M:
+        bout = bout[:M]
+    return out, bout
 
 
 def _lin(x, p, name):
     return torch.addmm(p[name + ".bias"], x, p[name + ".weight"].T)
 
 
-def _mix(M, v):
-    """(M v)_x = sum_z M[x][z] v_z over the xyz axis of [N,3,H]."""
-    return torch.matmul(M, v)
-
-
-def _mixT(M, v):
-    """(M^T v)_z = sum_x M[x][z] v_x over the xyz axis of [N,3,H]."""
-    return torch.matmul(M.T.contiguous(), v)
-
-
-def _xyz_outer(a, b):
-    """sum_{n,h} a[n,x,h] b[n,z,h] -> [3,3] (gradient of a 3x3 mixing matrix):
-    the [3H, 3H] product of the rows [N, 3H] (contiguous views, no copy),
-    then the sum of the diagonal of each H x H block."""
+def _xyz_outer(a, b, a1=None, b1=None, w=None):
+    """sum_n w_n sum_h (a[n,x,h] b[n,z,h] + a1[n,x,h] b1[n,z,h]) -> [3,3]
+    (gradient of a 3x3 mixing matrix): the [3H, 3H] weight-gradient product
+    of the rows [N, 3H], then the sum of the diagonal of each H x H block."""
     n, _, H = a.shape
-    full = _tmm(a.reshape(n, 3 * H), b.reshape(n, 3 * H))
+    flat = lambda t: None if t is None else t.reshape(n, 3 * H)
+    full, _ = _wgrad(flat(a), flat(b), flat(a1), flat(b1), w=w)
     return full.view(3, H, 3, H).diagonal(dim1=1, dim2=3).sum(-1)
 
 
@@ -326,19 +361,20 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
         W = _lin(h, p, pre + ".2")
         s_m, v_m, _, _ = _message(g, H, s, v, W, u)
         upre = f"interactions.{k}.update"
-        nrm = torch.linalg.vector_norm(v_m, dim=1)
-        xin = torch.cat([s_m, nrm], dim=-1)
+        # PaiNNUpdate.forward (student_model.py:430-470)
+        xin = torch.empty(n, 2 * H, dtype=torch.float32, device=x.device)
+        _glue("mlff_update_norm", n, H, s_m, v_m, xin)
         a2 = _lin(xin, p, upre + ".update_mlp.0")
         h2 = _silu(a2)
         y = _lin(h2, p, upre + ".update_mlp.2")
-        ds, g1, g2 = _split3(y, H)
-        M = p[upre + ".mixing_matrix"]
-        Mv = _mix(M, v_m)
-        s_new = s_m + ds
-        v_new = v_m * g1.unsqueeze(1) + Mv * g2.unsqueeze(1)
-        layers.append(dict(s=s, v=v, a1=a1, h=h, W=W, s_m=s_m, v_m=v_m, nrm=nrm, xin=xin,
-                           a2=a2, h2=h2, g1=g1, g2=g2, Mv=Mv, M=M))
-        s, v = s_new.contiguous(), v_new.contiguous()
+        M = _dense(p[upre + ".mixing_matrix"])
+        s_new = torch.empty_like(s_m)
+        v_new = torch.empty_like(v_m)
+        Mv = torch.empty_like(v_m)
+        _glue("mlff_update_apply", n, H, M, y, s_m, v_m, s_new, v_new, Mv)
+        layers.append(dict(s=s, v=v, a1=a1, h=h, W=W, s_m=s_m, v_m=v_m, xin=xin, a2=a2, h2=h2,
+                           y=y, Mv=Mv, M=M))
+        s, v = s_new, v_new
     r1 = _lin(s, p, "energy_head.0")
     h3 = _silu(r1)
     r2 = _lin(h3, p, "energy_head.2")
@@ -368,30 +404,27 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
         pre = f"interactions.{k}.message.rbf_to_scalar"
         lay["ds"], lay["dv"] = ds_, dv_
         da1 = de_t @ p[pre + ".0.weight"].T
-        dh = _silu1(lay["a1"]) * da1
+        dh = _silu_d(lay["a1"], da1)
         dW = dh @ p[pre + ".2.weight"].T
         lay["da1"], lay["dh"], lay["dW"] = da1, dh, dW
         _, _, dsm, dvm = _message(g, H, lay["s"], lay["v"], lay["W"], c["u"], ds_, dv_, dW, du)
-        nrm, v_m = lay["nrm"], lay["v_m"]
-        safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-        dnrm = torch.where(nrm > 0, (v_m * dvm).sum(1) / safe, torch.zeros_like(nrm))
-        dxin = torch.cat([dsm, dnrm], dim=-1)
+        v_m = lay["v_m"]
+        dxin = torch.empty(n, 2 * H, dtype=torch.float32, device=vec.device)
+        _glue("mlff_update_norm_tangent", n, H, v_m, lay["xin"], dsm, dvm, dxin)
         upre = f"interactions.{k}.update"
         da2 = dxin @ p[upre + ".update_mlp.0.weight"].T
-        dh2 = _silu1(lay["a2"]) * da2
+        dh2 = _silu_d(lay["a2"], da2)
         dy = dh2 @ p[upre + ".update_mlp.2.weight"].T
-        dMv = _mix(lay["M"], dvm)
-        lay.update(dsm=dsm, dvm=dvm, dnrm=dnrm, dxin=dxin, da2=da2, dh2=dh2, dy=dy, dMv=dMv)
-        g1, g2 = lay["g1"], lay["g2"]
-        dds, dg1, dg2 = _split3(dy, H)
-        lay["dg1"], lay["dg2"] = dg1, dg2
-        ds_ = dsm + dds
-        dv_ = (dvm * g1.unsqueeze(1) + v_m * dg1.unsqueeze(1) + dMv * g2.unsqueeze(1)
-               + lay["Mv"] * dg2.unsqueeze(1)).contiguous()
+        ds_ = torch.empty_like(dsm)
+        dv_ = torch.empty_like(dvm)
+        dMv = torch.empty_like(dvm)
+        _glue("mlff_update_apply_tangent", n, H, lay["M"], lay["y"], dy, dsm, v_m, dvm,
+              lay["Mv"], ds_, dv_, dMv)
+        lay.update(dsm=dsm, dvm=dvm, dxin=dxin, da2=da2, dh2=dh2, dy=dy, dMv=dMv)
     dr1 = ds_ @ p["energy_head.0.weight"].T
-    dh3 = _silu1(c["r1"]) * dr1
+    dh3 = _silu_d(c["r1"], dr1)
     dr2 = dh3 @ p["energy_head.2.weight"].T
-    dh4 = _silu1(c["r2"]) * dr2
+    dh4 = _silu_d(c["r2"], dr2)
     c.update(lam=lam, dd=dd, du=du, de_t=de_t, dsL=ds_, dr1=dr1, dh3=dh3, dr2=dr2, dh4=dh4)
 
 
@@ -450,26 +483,30 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     acc(grads, "energy_head.4.weight", (wgeps @ h4).reshape(1, -1))
     acc(grads, "energy_head.4.bias", wgeps.sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
-    gr2 = gh4 * _silu1(r2)
-    acc(grads, "energy_head.2.weight", _tmm(wa(gr2), h3))
-    acc(grads, "energy_head.2.bias", _wsum(w_atom, gr2))
+    gr2 = _silu_d(r2, gh4)
+    gw_, gb_ = _wgrad(gr2, h3, w=w_atom, bias=True)
+    acc(grads, "energy_head.2.weight", gw_)
+    acc(grads, "energy_head.2.bias", gb_)
     gh3 = gr2 @ p["energy_head.2.weight"]
-    gr1 = gh3 * _silu1(r1)
-    acc(grads, "energy_head.0.weight", _tmm(wa(gr1), sL))
-    acc(grads, "energy_head.0.bias", _wsum(w_atom, gr1))
+    gr1 = _silu_d(r1, gh3)
+    gw_, gb_ = _wgrad(gr1, sL, w=w_atom, bias=True)
+    acc(grads, "energy_head.0.weight", gw_)
+    acc(grads, "energy_head.0.bias", gb_)
     gs = (gr1 @ p["energy_head.0.weight"]).contiguous()
     gv = torch.zeros(n, 3, H, dtype=torch.float32, device=dev)
     if T:
         dr1, dh3, dr2, dh4 = c["dr1"], c["dh3"], c["dr2"], c["dh4"]
         acc(tg, "energy_head.4.weight", (geps @ dh4).reshape(1, -1))
         acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=torch.float32, device=dev))
-        dgr2 = gh4 * _silu2(r2) * dr2
-        acc(tg, "energy_head.2.weight", _tmm(dgr2, h3) + _tmm(gr2, dh3))
-        acc(tg, "energy_head.2.bias", dgr2.sum(0))
+        dgr2 = _silu_dd(r2, gh4, dr2)
+        gw_, gb_ = _wgrad(dgr2, h3, gr2, dh3, bias=True)
+        acc(tg, "energy_head.2.weight", gw_)
+        acc(tg, "energy_head.2.bias", gb_)
         dgh3 = dgr2 @ p["energy_head.2.weight"]
-        dgr1 = dgh3 * _silu1(r1) + gh3 * _silu2(r1) * dr1
-        acc(tg, "energy_head.0.weight", _tmm(dgr1, sL) + _tmm(gr1, c["dsL"]))
-        acc(tg, "energy_head.0.bias", dgr1.sum(0))
+        dgr1 = _silu_dd(r1, gh3, dr1, dgh3)
+        gw_, gb_ = _wgrad(dgr1, sL, gr1, c["dsL"], bias=True)
+        acc(tg, "energy_head.0.weight", gw_)
+        acc(tg, "energy_head.0.bias", gb_)
         dgs = (dgr1 @ p["energy_head.0.weight"]).contiguous()
         dgv = torch.zeros_like(gv)
 
@@ -483,50 +520,45 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         mpre = f"interactions.{k}.message.rbf_to_scalar"
         U1 = p[upre + ".update_mlp.0.weight"]
         U2 = p[upre + ".update_mlp.2.weight"]
-        M = lay["M"]
-        v_m, g1, g2, Mv, nrm = lay["v_m"], lay["g1"], lay["g2"], lay["Mv"], lay["nrm"]
-        # update reverse (student_model.py:434-470)
-        gg1 = (gv * v_m).sum(1)
-        gg2 = (gv * Mv).sum(1)
-        gy = torch.cat([gs, gg1, gg2], dim=-1)
-        acc(grads, upre + ".update_mlp.2.weight", _tmm(gy, wa(lay["h2"])))
-        acc(grads, upre + ".update_mlp.2.bias", _wsum(w_atom, gy))
+        M, v_m, Mv, y, xin = lay["M"], lay["v_m"], lay["Mv"], lay["y"], lay["xin"]
+        # update reverse (student_model.py:434-470), gates and their tangent
+        gy = torch.empty(n, 3 * H, dtype=torch.float32, device=dev)
+        dgy = torch.empty_like(gy) if T else None
+        dvm, dMv, dxin = (lay["dvm"], lay["dMv"], lay["dxin"]) if T else (None, None, None)
+        _glue("mlff_update_gate_adjoint", n, H, gs, gv, v_m, Mv, dgs if T else None,
+              dgv if T else None, dvm, dMv, gy, dgy)
+        gw_, gb_ = _wgrad(gy, lay["h2"], w=w_atom, bias=True)
+        acc(grads, upre + ".update_mlp.2.weight", gw_)
+        acc(grads, upre + ".update_mlp.2.bias", gb_)
         gh2 = gy @ U2
-        ga2 = gh2 * _silu1(lay["a2"])
-        acc(grads, upre + ".update_mlp.0.weight", _tmm(wa(ga2), lay["xin"]))
-        acc(grads, upre + ".update_mlp.0.bias", _wsum(w_atom, ga2))
+        ga2 = _silu_d(lay["a2"], gh2)
+        gw_, gb_ = _wgrad(ga2, xin, w=w_atom, bias=True)
+        acc(grads, upre + ".update_mlp.0.weight", gw_)
+        acc(grads, upre + ".update_mlp.0.bias", gb_)
         gx = ga2 @ U1
-        gs_x, gn = [t.contiguous() for t in gx.split(H, dim=1)]
-        gvg2 = gv * g2.unsqueeze(1)
-        acc(grads, upre + ".mixing_matrix", _xyz_outer(wa(gvg2), v_m))
-        safe = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-        unit_n = torch.where((nrm > 0).unsqueeze(1), v_m / safe.unsqueeze(1),
-                             torch.zeros_like(v_m))
-        gs_m = (gs + gs_x).contiguous()
-        gv_m = (gv * g1.unsqueeze(1) + _mixT(M, gvg2) + gn.unsqueeze(1) * unit_n).contiguous()
+        dgx = None
         if T:
-            dvm, dMv, dxin = lay["dvm"], lay["dMv"], lay["dxin"]
-            dg1, dg2 = lay["dg1"], lay["dg2"]
-            dgg1 = (dgv * v_m + gv * dvm).sum(1)
-            dgg2 = (dgv * Mv + gv * dMv).sum(1)
-            dgy = torch.cat([dgs, dgg1, dgg2], dim=-1)
-            acc(tg, upre + ".update_mlp.2.weight", _tmm(dgy, lay["h2"]) + _tmm(gy, lay["dh2"]))
-            acc(tg, upre + ".update_mlp.2.bias", dgy.sum(0))
+            gw_, gb_ = _wgrad(dgy, lay["h2"], gy, lay["dh2"], bias=True)
+            acc(tg, upre + ".update_mlp.2.weight", gw_)
+            acc(tg, upre + ".update_mlp.2.bias", gb_)
             dgh2 = dgy @ U2
-            dga2 = dgh2 * _silu1(lay["a2"]) + gh2 * _silu2(lay["a2"]) * lay["da2"]
-            acc(tg, upre + ".update_mlp.0.weight", _tmm(dga2, lay["xin"]) + _tmm(ga2, dxin))
-            acc(tg, upre + ".update_mlp.0.bias", dga2.sum(0))
+            dga2 = _silu_dd(lay["a2"], gh2, lay["da2"], dgh2)
+            gw_, gb_ = _wgrad(dga2, xin, ga2, dxin, bias=True)
+            acc(tg, upre + ".update_mlp.0.weight", gw_)
+            acc(tg, upre + ".update_mlp.0.bias", gb_)
             dgx = dga2 @ U1
-            dgs_x, dgn = [t.contiguous() for t in dgx.split(H, dim=1)]
-            dgvg2 = dgv * g2.unsqueeze(1) + gv * dg2.unsqueeze(1)
-            acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m) + _xyz_outer(gvg2, dvm))
-            dnrm = lay["dnrm"]
-            dunit = torch.where((nrm > 0).unsqueeze(1),
-                                dvm / safe.unsqueeze(1) - v_m * (dnrm / safe ** 2).unsqueeze(1),
-                                torch.zeros_like(v_m))
-            dgs_m = (dgs + dgs_x).contiguous()
-            dgv_m = (dgv * g1.unsqueeze(1) + gv * dg1.unsqueeze(1) + _mixT(M, dgvg2)
-                     + dgn.unsqueeze(1) * unit_n + gn.unsqueeze(1) * dunit).contiguous()
+        gs_m = torch.empty(n, H, dtype=torch.float32, device=dev)
+        gv_m = torch.empty(n, 3, H, dtype=torch.float32, device=dev)
+        gvg2 = torch.empty_like(gv_m)
+        dgs_m = torch.empty_like(gs_m) if T else None
+        dgv_m = torch.empty_like(gv_m) if T else None
+        dgvg2 = torch.empty_like(gv_m) if T else None
+        _glue("mlff_update_input_adjoint", n, H, M, gx, gs, gv, y, v_m, xin, dgx,
+              dgs if T else None, dgv if T else None, lay["dy"] if T else None, dvm, dxin,
+              gs_m, gv_m, gvg2, dgs_m, dgv_m, dgvg2)
+        acc(grads, upre + ".mixing_matrix", _xyz_outer(gvg2, v_m, w=w_atom))
+        if T:
+            acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m, gvg2, dvm))
         # message reverse (student_model.py:346-387): filter-output gradients per pair
         s, v, W = lay["s"], lay["v"], lay["W"]
         if T:
@@ -536,24 +568,28 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             gW, _ = _pair_grad(g, H, u, s, v, gs_m, gv_m)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        acc(grads, mpre + ".2.weight", _tmm(gW, wp(lay["h"])))
-        acc(grads, mpre + ".2.bias", _wsum(w_pair, gW))
+        gw_, gb_ = _wgrad(gW, lay["h"], w=w_pair, bias=True)
+        acc(grads, mpre + ".2.weight", gw_)
+        acc(grads, mpre + ".2.bias", gb_)
         gh = gW @ W2
-        ga1 = gh * _silu1(lay["a1"])
-        acc(grads, mpre + ".0.weight", _tmm(ga1, wp(c["e"])))
-        acc(grads, mpre + ".0.bias", _wsum(w_pair, ga1))
+        ga1 = _silu_d(lay["a1"], gh)
+        gw_, gb_ = _wgrad(ga1, c["e"], w=w_pair, bias=True)
+        acc(grads, mpre + ".0.weight", gw_)
+        acc(grads, mpre + ".0.bias", gb_)
         if rbf_grads:
             ge = ga1 @ W1
             acc(grads, "rbf.centers", (wp(ge) * e_mu).sum(0))
             acc(grads, "rbf.widths", (wp(ge) * e_w).sum(0))
         if T:
             dh, da1 = lay["dh"], lay["da1"]
-            acc(tg, mpre + ".2.weight", _tmm(dgW, lay["h"]) + _tmm(gW, dh))
-            acc(tg, mpre + ".2.bias", dgW.sum(0))
+            gw_, gb_ = _wgrad(dgW, lay["h"], gW, dh, bias=True)
+            acc(tg, mpre + ".2.weight", gw_)
+            acc(tg, mpre + ".2.bias", gb_)
             dgh = dgW @ W2
-            dga1 = dgh * _silu1(lay["a1"]) + gh * _silu2(lay["a1"]) * da1
-            acc(tg, mpre + ".0.weight", _tmm(dga1, c["e"]) + _tmm(ga1, c["de_t"]))
-            acc(tg, mpre + ".0.bias", dga1.sum(0))
+            dga1 = _silu_dd(lay["a1"], gh, da1, dgh)
+            gw_, gb_ = _wgrad(dga1, c["e"], ga1, c["de_t"], bias=True)
+            acc(tg, mpre + ".0.weight", gw_)
+            acc(tg, mpre + ".0.bias", gb_)
             if rbf_grads:
                 dge = dga1 @ W1
                 dd = c["dd"].unsqueeze(-1)

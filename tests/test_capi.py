@@ -166,3 +166,66 @@ def test_launch_shape_defaults_are_the_first_values(H):
     assert len(shapes) == 5 and all(len(v) >= 2 for v in shapes)
     tiles = engine.default_tiles(H)
     assert tiles == tuple(v[0] for v in shapes) and tiles[4] == 0
+
+
+def test_weight_grad_checks_arguments_on_the_host():
+    """mlff_weight_grad validates shapes, strides, alignment and the
+    workspace size before any device call (the fake pointers below are never
+    dereferenced); the workspace size follows the row split."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    big = 1 << 30
+
+    def call(K, M, N, a0=f, lda0=None, b0=f, ldb0=None, a1=None, b1=None, ws=f, nbytes=big,
+             out=f):
+        lda0 = M if lda0 is None else lda0
+        ldb0 = N if ldb0 is None else ldb0
+        return lib.mlff_weight_grad(None, K, M, N, a0, lda0, b0, ldb0, a1, M, b1, N, None, ws,
+                                    nbytes, out, None)
+
+    assert call(100, 6, 8) == -1          # M not a multiple of 4
+    assert call(100, 8, 10) == -1         # N not a multiple of 4
+    assert call(-1, 8, 8) == -1
+    assert call(100, 8, 8, lda0=6) == -1  # stride below M
+    assert call(100, 8, 8, a1=f) == -1    # a1 without b1
+    assert call(100, 8, 8, b0=odd) == -1  # misaligned
+    assert call(100, 8, 8, out=None) == -1
+    need = lib.mlff_weight_grad_workspace_bytes(100, 8, 8)
+    assert need > 0
+    assert call(100, 8, 8, nbytes=need - 4) == -1
+    assert "workspace" in lib.mlff_last_error().decode()
+    # the partial count grows with K up to its cap, then the rows per wave grow
+    small = lib.mlff_weight_grad_workspace_bytes(1000, 384, 128)
+    large = lib.mlff_weight_grad_workspace_bytes(1_000_000, 384, 128)
+    assert small < large == lib.mlff_weight_grad_workspace_bytes(2_000_000, 384, 128)
+    assert lib.mlff_weight_grad_workspace_bytes(0, 8, 8) == (8 * 8 + 8) * 4
+
+
+def test_silu_chain_checks_arguments_on_the_host():
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 8)
+    assert lib.mlff_silu_chain(None, 6, f, None, None, None, f, None, None, None) == -1
+    assert lib.mlff_silu_chain(None, 8, None, None, None, None, f, None, None, None) == -1
+    assert lib.mlff_silu_chain(None, 8, f, None, None, None, None, f, None, None) == -1  # dh, no da
+    assert lib.mlff_silu_chain(None, 8, f, f, None, None, None, None, None, f) == -1     # dga, no g
+    assert lib.mlff_silu_chain(None, 8, odd, None, None, None, f, None, None, None) == -1
+    assert lib.mlff_silu_chain(None, 0, f, None, None, None, f, None, None, None) == 0
+
+
+def test_update_glue_checks_arguments_on_the_host():
+    """The update-block kernels refuse bad sizes, null and misaligned
+    buffers, and a tangent without its inputs, before any device call."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    assert lib.mlff_update_norm(None, 10, 6, f, f, f) == -1
+    assert lib.mlff_update_norm(None, 10, 8, f, None, f) == -1
+    assert lib.mlff_update_norm(None, 10, 8, f, odd, f) == -1
+    assert lib.mlff_update_norm(None, 0, 8, f, f, f) == 0
+    assert lib.mlff_update_apply(None, 10, 8, None, f, f, f, f, f, f) == -1
+    assert lib.mlff_update_gate_adjoint(None, 10, 8, f, f, f, f, None, None, None, None, f,
+                                        f) == -1        # tangent output without inputs
+    assert lib.mlff_update_input_adjoint(None, 10, 8, f, f, f, f, f, f, f, None, None, None,
+                                         None, None, None, f, f, f, f, None, None) == -1

@@ -111,18 +111,24 @@ def test_loss_gradient_central_differences():
                                                                             grads[name][idx].item())
 
 
-def test_chunked_weight_gradient_product():
-    """The product's split-K weight-gradient product equals a.T @ b,
-    including a ragged tail and the short-K path."""
+def test_weight_gradient_operands():
+    """Operands of mlff_weight_grad: widths padded with zero columns to a
+    multiple of 4, views with a usable row stride passed as they are, other
+    layouts copied (host logic; the kernel is tested on the GPU)."""
     train = pkg_mod("train")
-    g = torch.Generator().manual_seed(3)
-    for K in (100, 8192, 10000, 20480 + 7):
-        a = torch.randn(K, 12, dtype=torch.float64, generator=g)
-        b = torch.randn(K, 5, dtype=torch.float64, generator=g)
-        ref = a.T @ b
-        got = train._tmm(a, b)
-        assert got.shape == ref.shape
-        assert torch.allclose(got, ref, rtol=1e-12, atol=1e-10)
+    a = torch.randn(10, 20)
+    assert train._wg_operand(a).data_ptr() == a.data_ptr()
+    p = train._wg_operand(torch.randn(10, 5))
+    assert p.shape == (10, 8) and torch.equal(p[:, 5:], torch.zeros(10, 3))
+    wide = torch.randn(10, 3 * 16)
+    view = wide[:, 16:32]
+    assert train._wg_operand(view).data_ptr() == view.data_ptr()
+    t = torch.randn(20, 10).T                      # column stride 10
+    q = train._wg_operand(t)
+    assert q.stride(1) == 1 and torch.equal(q[:, :10], t.contiguous()[:, :10])
+    r = train._wg_operand(torch.randn(6, 3, 8))    # [n, 3, H] rows flattened
+    assert r.shape == (6, 24)
+    assert train._wg_operand(torch.randn(0, 3, 5)).shape == (0, 16)
 
 
 def test_weighted_gradients_equal_weighted_adjoint():
