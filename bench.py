@@ -15,7 +15,8 @@ steps, plus
   * accuracy: max |dF| and max |dE|/N against the float64 oracle at the GPU
     trajectory's final positions (a sample of molecules);
   * roofline: the dominant kernel's algorithmic FLOP per launch over its
-    event-timed average duration;
+    event-timed average duration, against the published peak and against the
+    peak measured on the device (register-only MFMA probe, device copy);
   * cpu_baseline: the oracle (float32 PyTorch-eager restatement of the
     reference, dense neighbour search, autograd forces, VV) on the host cores.
 """
@@ -233,6 +234,48 @@ def stage_times(md, reps=20):
         out[name] = {"avg_s": tot / len(layers), "launches_per_step": len(layers),
                      "per_step_s": tot}
     return out
+
+
+def measured_peaks(dev, iters=4000, copy_mib=1024):
+    """Sustained peaks on this device: dense FP32 (16x16x4) and bf16
+    (16x16x32) MFMA from a register-only probe (four waves per SIMD, eight
+    independent chains each), and HBM bandwidth from a device-to-device copy
+    (bytes read + written per second)."""
+    C = importlib.import_module("mlff-distiller_amd._capi")
+    lib = C.lib()
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    blocks = 4 * cus
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for kind, name in ((0, "fp32_mfma_tflops"), (1, "bf16_mfma_tflops")):
+        C.check("probe", lib.mlff_probe_mfma(stream.cuda_stream, kind, blocks, iters, C.ptr(out)))
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(3):
+            C.check("probe", lib.mlff_probe_mfma(stream.cuda_stream, kind, blocks, iters,
+                                                 C.ptr(out)))
+        b.record(stream)
+        b.synchronize()
+        flops = 3 * lib.mlff_probe_mfma_flops(kind, blocks, iters)
+        res[name] = flops / (a.elapsed_time(b) / 1e3) / 1e12
+    n = copy_mib << 18
+    x = torch.empty(n, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(5):
+        y.copy_(x)
+    b.record(stream)
+    b.synchronize()
+    res["hbm_copy_gbps"] = 5 * 2 * 4 * n / (a.elapsed_time(b) / 1e3) / 1e9
+    res["method"] = (f"MFMA: {blocks} workgroups x 4 waves, {iters} x 8 independent MFMAs per "
+                     f"wave, 3 launches; copy: {copy_mib} MiB device to device, 5 times")
+    del x, y
+    return res
 
 
 # ---------------------------------------------------------------------------
@@ -629,6 +672,14 @@ def main():
                 "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": None,
                 "algorithmic_per_launch": nbytes, "avg_launch_us": st[dom]["avg_s"] * 1e6}
 
+    peaks = measured_peaks(dev)
+    log(f"measured peaks: {peaks}")
+    if roof["bound"] == "mfma":
+        roof["peak_measured"] = peaks["fp32_mfma_tflops"]
+    else:
+        roof["peak_measured"] = peaks["hbm_copy_gbps"]
+    roof["frac_of_measured"] = roof["achieved"] / roof["peak_measured"]
+
     traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
@@ -702,6 +753,7 @@ def main():
                 "steps": int(summ["n_steps"])},
         "stages_us": {k: round(v["avg_s"] * 1e6, 2) for k, v in st.items()},
         "roofline": roof,
+        "peaks_measured": peaks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

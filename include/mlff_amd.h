@@ -462,6 +462,17 @@ int mlff_update_input_adjoint(void* stream, int n_atoms, int hidden, const float
                               float* gs_out_dev, float* gv_out_dev, float* gvg2_out_dev,
                               float* dgs_out_dev, float* dgv_out_dev, float* dgvg2_out_dev);
 
+/* ---------------------------------------------------------------------------
+ * Matrix-core throughput probe, for the measured peak next to the published
+ * one in roofline figures (no counterpart in the reference, which has no GPU
+ * roofline).  blocks x 256 threads; each wave issues iters x 8 independent
+ * register-only MFMAs of one shape (kind 0: f32 16x16x4, kind 1: bf16
+ * 16x16x32) and writes one float per thread to out_dev [blocks*256].
+ * mlff_probe_mfma_flops returns the launch's FLOP count (-1 on bad input).
+ * ------------------------------------------------------------------------- */
+int64_t mlff_probe_mfma_flops(int kind, int blocks, int iters);
+int mlff_probe_mfma(void* stream, int kind, int blocks, int iters, float* out_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,3 +229,16 @@ def test_update_glue_checks_arguments_on_the_host():
                                         f) == -1        # tangent output without inputs
     assert lib.mlff_update_input_adjoint(None, 10, 8, f, f, f, f, f, f, f, None, None, None,
                                          None, None, None, f, f, f, f, None, None) == -1
+
+
+def test_probe_flop_count_and_argument_checks():
+    """The matrix-core probe's FLOP count (blocks x 4 waves x iters x 8
+    MFMAs x FLOP per MFMA) and its host-side argument checks."""
+    lib = C.lib()
+    assert lib.mlff_probe_mfma_flops(0, 2, 10) == 2 * 4 * 10 * 8 * (16 * 16 * 4 * 2)
+    assert lib.mlff_probe_mfma_flops(1, 1, 1) == 4 * 8 * (16 * 16 * 32 * 2)
+    assert lib.mlff_probe_mfma_flops(2, 1, 1) == -1
+    f = ctypes.c_void_p(1 << 20)
+    assert lib.mlff_probe_mfma(None, 2, 1, 1, f) == -1
+    assert lib.mlff_probe_mfma(None, 0, 0, 1, f) == -1
+    assert lib.mlff_probe_mfma(None, 0, 1, 1, None) == -1
