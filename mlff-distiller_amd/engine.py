@@ -62,19 +62,29 @@ def hyper_from_state(sd: Dict[str, torch.Tensor]) -> Tuple[int, int, int, int]:
     return H, L, R, max_z
 
 
+_LAYOUTS: Dict[tuple, tuple] = {}
+
+
 def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
     """Pack a StudentForceField state dict into the library's float32 blob
     (on the device directly when the parameters already live there, so a
-    training step's weight update costs device copies only)."""
+    training step's weight update costs device copies only: one fused
+    multi-tensor copy)."""
     H, L, R, max_z = hyper_from_state(sd)
-    slots, total = C.weights_layout(H, L, R, max_z)
+    key = (H, L, R, max_z)
+    if key not in _LAYOUTS:
+        _LAYOUTS[key] = C.weights_layout(H, L, R, max_z)
+    slots, total = _LAYOUTS[key]
     device = device_of(device)
     on_device = all(v.device == device for v in sd.values())
     where = device if on_device else torch.device("cpu")
     host = torch.zeros(total, dtype=torch.float32, device=where)
+    dsts, srcs = [], []
     for s in slots:
         name = _NAMES[s.tensor].format(l=s.layer)
-        t = sd[name].detach().to(where, torch.float32)
+        t = sd[name].detach()
+        if t.device != where or t.dtype != torch.float32:
+            t = t.to(where, torch.float32)
         if t.dim() == 1:
             t = t.unsqueeze(0)
         if s.kind == 1:
@@ -83,7 +93,9 @@ def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
         if rows > s.rows or cols > s.ld:
             raise ValueError(f"slot for {name} too small ({rows}x{cols} into {s.rows}x{s.ld})")
         view = host[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)
-        view[:rows, :cols] = t
+        dsts.append(view[:rows, :cols])
+        srcs.append(t)
+    torch._foreach_copy_(dsts, srcs)
     return host if on_device else host.to(device)
 
 

@@ -388,9 +388,12 @@ class StudentForceField(nn.Module):
     # ------------------------------------------------------------------
     def packed_weights(self, device) -> torch.Tensor:
         device = engine.device_of(device)
-        sig = tuple((t.data_ptr(), t._version) for t in self.state_dict().values())
+        sd = self.state_dict()
+        sig = tuple((t.data_ptr(), t._version) for t in sd.values())
         if self._packed is None or self._packed_sig != sig or self._packed.device != device:
-            self._packed = engine.pack_weights(self.state_dict(), device)
+            # a fresh blob: plans bound elsewhere (an MD run) keep the weights they
+            # were bound with, consistent with their split bf16 copies
+            self._packed = engine.pack_weights(sd, device)
             self._packed_sig = sig
             for entry in self._plans.values():
                 entry["plan"].set_weights(self._packed)

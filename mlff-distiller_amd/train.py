@@ -280,9 +280,14 @@ def _wg_operand(t: Tensor) -> Tensor:
     t = t.reshape(t.shape[0], math.prod(t.shape[1:]))
     if t.shape[1] % 4:
         t = torch.nn.functional.pad(t, (0, 4 - t.shape[1] % 4))
-    if t.stride(1) != 1 or t.stride(0) % 4 or t.data_ptr() % 16:
+    if t.stride(1) != 1 or t.stride(0) % 4:
         t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()
     return t
+
+
+_WG_WS: Dict[tuple, int] = {}   # workspace floats of mlff_weight_grad by (K, M, N)
 
 
 def _wgrad(a0: Tensor, b0: Tensor, a1: Optional[Tensor] = None, b1: Optional[Tensor] = None,
@@ -299,18 +304,24 @@ def _wgrad(a0: Tensor, b0: Tensor, a1: Optional[Tensor] = None, b1: Optional[Ten
     if a1 is not None:
         A1, B1 = _wg_operand(a1), _wg_operand(b1)
     M4, N4 = A0.shape[1], B0.shape[1]
-    dev = a0.device
     lib = C.lib()
-    nbytes = int(lib.mlff_weight_grad_workspace_bytes(K, M4, N4))
-    ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=dev)
-    out = torch.empty(M4, N4, dtype=torch.float32, device=dev)
-    bout = torch.empty(M4, dtype=torch.float32, device=dev) if bias else None
-    wv = None if w is None else w.to(torch.float32).contiguous()
+    key = (K, M4, N4)
+    ws_floats = _WG_WS.get(key)
+    if ws_floats is None:
+        ws_floats = (int(lib.mlff_weight_grad_workspace_bytes(K, M4, N4)) + 15) // 16 * 4
+        _WG_WS[key] = ws_floats
+    # one allocation: workspace, then the weight gradient, then the bias
+    buf = torch.empty(ws_floats + M4 * N4 + M4, dtype=torch.float32, device=a0.device)
+    out = buf[ws_floats:ws_floats + M4 * N4].view(M4, N4)
+    bout = buf[ws_floats + M4 * N4:] if bias else None
+    wv = None
+    if w is not None:
+        wv = w if (w.dtype == torch.float32 and w.is_contiguous()) else w.to(torch.float32).contiguous()
     C.check("mlff_weight_grad", lib.mlff_weight_grad(
-        C.stream_handle(), K, M4, N4, C.ptr(A0), A0.stride(0), C.ptr(B0), B0.stride(0),
+        C.stream_handle(), K, M4, N4, A0.data_ptr(), A0.stride(0), B0.data_ptr(), B0.stride(0),
         C.ptr(A1), A1.stride(0) if A1 is not None else 0,
         C.ptr(B1), B1.stride(0) if B1 is not None else 0,
-        C.ptr(wv), C.ptr(ws), ws.numel(), C.ptr(out), C.ptr(bout)))
+        C.ptr(wv), buf.data_ptr(), ws_floats * 4, out.data_ptr(), C.ptr(bout)))
     if M4 != M or N4 != N:
         out = out[:M, :N]
     if bout is not None and M4 != M:
@@ -435,11 +446,10 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
 def _z_segments(Z: Tensor, n_rows: int):
     """Atoms ordered by element (stable sort) and the element segments, for
     the fixed-order embedding-gradient sum."""
-    order = torch.sort(Z, stable=True).indices.to(torch.int32)
-    counts = torch.bincount(Z, minlength=n_rows)
-    ptr = torch.zeros(n_rows + 1, dtype=torch.int32, device=Z.device)
-    ptr[1:] = torch.cumsum(counts, 0).to(torch.int32)
-    return ptr, order
+    zs, order = torch.sort(Z, stable=True)
+    # first position of each element in the sorted list (no host sync)
+    ptr = torch.searchsorted(zs, torch.arange(n_rows + 1, dtype=zs.dtype, device=Z.device))
+    return ptr.to(torch.int32), order.to(torch.int32)
 
 
 def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
@@ -655,11 +665,24 @@ class ForceMatchingTrainer:
         self.rbf_grads = any(n.startswith("rbf.") for n, _ in model.named_parameters())
         self.w_energy, self.w_force, self.grad_clip = w_energy, w_force, grad_clip
         self.opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=weight_decay)
+        self._p = None
+        self._p_sig = None
+
+    def _params(self) -> Dict[str, Tensor]:
+        """Detached views of the model's state (the optimizer updates the
+        tensors in place, so the views stay current); rebuilt when a tensor
+        has been replaced (model.to(), load_state_dict(assign=True))."""
+        sig = tuple(t.data_ptr() for t in self.model.parameters()) + tuple(
+            t.data_ptr() for t in self.model.buffers())
+        if self._p is None or sig != self._p_sig:
+            self._p = {k: v.detach() for k, v in self.model.state_dict().items()}
+            self._p_sig = sig
+        return self._p
 
     def loss_and_grads(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None):
         x = x.detach().to(torch.float32).contiguous()
         g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
-        p = {k: v.detach() for k, v in self.model.state_dict().items()}
+        p = self._params()
         # forces of the current weights from the fused HIP evaluation
         _, F, _ = self.model.evaluate(Z, x, np.asarray(sizes), cells, pbc)
         with torch.no_grad():

@@ -175,3 +175,23 @@ def test_learnable_rbf_gradients_match_autograd_double_backward():
         assert err < 1e-9 * scale, (k, err, scale)
     _, _, _, plain = train.force_matching_grads(sd, Z, x, g, rc, El, Fl, 1.0, 100.0)
     assert "rbf.centers" not in plain and "rbf.widths" not in plain
+
+
+def test_trainer_parameter_views_follow_the_model():
+    """The trainer's cached parameter views see in-place optimizer updates
+    and are rebuilt when the model's tensors are replaced."""
+    model_mod = pkg_mod("model")
+    train = pkg_mod("train")
+    torch.manual_seed(0)
+    m = model_mod.StudentForceField(hidden_dim=32, num_interactions=1, num_rbf=8)
+    tr = train.ForceMatchingTrainer(m, lr=1e-3)
+    p1 = tr._params()
+    w = m.energy_head[0].weight
+    with torch.no_grad():
+        w.add_(1.0)
+    assert torch.equal(tr._params()["energy_head.0.weight"], w)
+    assert tr._params() is p1
+    m.load_state_dict({k: v.clone() for k, v in m.state_dict().items()}, assign=True)
+    p2 = tr._params()
+    assert p2 is not p1
+    assert p2["energy_head.0.weight"].data_ptr() == m.energy_head[0].weight.data_ptr()
