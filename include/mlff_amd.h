@@ -171,6 +171,15 @@ int mlff_plan_bind(mlff_plan* plan, void* workspace_dev, int64_t workspace_bytes
  * force one.  Synchronous (copies the grid to the device); not for capture. */
 int mlff_plan_set_cell_grid(mlff_plan* plan, const int32_t* grid_host, int n_structures);
 
+/* Rebind a bound plan to a new weight blob of the same layout (a training
+ * step's updated weights) without the host synchronisation of
+ * mlff_plan_bind: the bf16 planes of the filter weights are re-split on
+ * `stream`, ordered after the blob's writes when those are on the same
+ * stream.  Evaluations on another stream must be ordered after it by the
+ * caller.  Drops the plan's captured graphs.  (The reference's modules read
+ * their parameters in place, student_model.py:585-605.) */
+int mlff_plan_set_weights(mlff_plan* plan, void* stream, const float* weights_dev);
+
 /* Device pointer to the plan's int32 counts[MLFF_COUNT_SLOTS]. */
 int32_t* mlff_plan_counts(mlff_plan* plan);
 

@@ -46,7 +46,7 @@ EXPORTS = (
     "mlff_weight_grad_workspace_bytes", "mlff_weight_grad", "mlff_silu_chain",
     "mlff_update_norm", "mlff_update_apply", "mlff_update_norm_tangent",
     "mlff_update_apply_tangent", "mlff_update_gate_adjoint", "mlff_update_input_adjoint",
-    "mlff_probe_mfma_flops", "mlff_probe_mfma",
+    "mlff_probe_mfma_flops", "mlff_probe_mfma", "mlff_plan_set_weights",
 )
 
 # bytes of the static device scratch behind mlff_energy_forces_static
@@ -166,6 +166,8 @@ def _declare(lib):
         fn = getattr(lib, name)
         fn.restype = ctypes.c_int
         fn.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * n_ptrs
+    lib.mlff_plan_set_weights.restype = ctypes.c_int
+    lib.mlff_plan_set_weights.argtypes = [vp, vp, vp]
     lib.mlff_probe_mfma_flops.restype = ctypes.c_int64
     lib.mlff_probe_mfma_flops.argtypes = [ctypes.c_int] * 3
     lib.mlff_probe_mfma.restype = ctypes.c_int

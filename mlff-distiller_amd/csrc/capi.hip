@@ -450,6 +450,24 @@ int mlff_plan_bind(mlff_plan* pl, void* ws, int64_t ws_bytes, const float* weigh
   return MLFF_OK;
 }
 
+int mlff_plan_set_weights(mlff_plan* pl, void* stream, const float* weights_dev) {
+  if (!pl || !pl->bound) { set_error("mlff_plan_set_weights: plan not bound"); return MLFF_ERR_STATE; }
+  if (!weights_dev || (reinterpret_cast<uintptr_t>(weights_dev) & 15)) {
+    set_error("mlff_plan_set_weights: weights must be non-null and 16-byte aligned");
+    return MLFF_ERR_INVALID;
+  }
+  pl->buf.w = weights_dev;
+  // the bf16 planes follow the new weights in stream order on the caller's stream
+  hipError_t e = split_filter_weights_async(pl->buf, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "mlff_plan_set_weights: bf16 weight planes");
+  // captured graphs hold the old weights
+  if (pl->mts_exec) { (void)hipGraphExecDestroy(pl->mts_exec); pl->mts_exec = nullptr; }
+  if (pl->mts_graph) { (void)hipGraphDestroy(pl->mts_graph); pl->mts_graph = nullptr; }
+  if (pl->graph_exec) { (void)hipGraphExecDestroy(pl->graph_exec); pl->graph_exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return MLFF_OK;
+}
+
 int32_t* mlff_plan_counts(mlff_plan* pl) { return (pl && pl->bound) ? pl->buf.counts : nullptr; }
 
 static int run_ef(mlff_plan* pl, hipStream_t s, const float* pos, double* energy, float* forces,

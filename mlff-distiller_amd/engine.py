@@ -285,15 +285,12 @@ class Plan:
             C.check("mlff_plan_set_cell_grid", self.lib.mlff_plan_set_cell_grid(
                 self.handle, g.ctypes.data, self.layout.n_structures))
 
-    def set_weights(self, weights: torch.Tensor):
-        """Rebind with a new weight blob (same layout)."""
+    def set_weights(self, weights: torch.Tensor, stream=None):
+        """Rebind with a new weight blob (same layout), in stream order on
+        ``stream`` (default: the current stream) without a host sync."""
         self.weights = weights
-        lay = self.layout
-        C.check("mlff_plan_bind", self.lib.mlff_plan_bind(
-            self.handle, C.ptr(self.workspace), self.workspace.numel(), C.ptr(weights),
-            C.ptr(self.numbers), C.ptr(lay.mol_ptr), C.ptr(lay.atom_mol), C.ptr(lay.cells),
-            C.ptr(lay.pbc)))
-        self._apply_cell_grid()
+        C.check("mlff_plan_set_weights", self.lib.mlff_plan_set_weights(
+            self.handle, C.stream_handle(stream), C.ptr(weights)))
 
     def energy_forces(self, positions: torch.Tensor, energy: torch.Tensor,
                       forces: Optional[torch.Tensor], virial: Optional[torch.Tensor] = None,

@@ -365,6 +365,8 @@ class StudentForceField(nn.Module):
         self._initialize_parameters()
         self._packed = None
         self._packed_sig = None
+        self._z_checked = None
+        self._z_checked_version = -1
         self._plans: "OrderedDict[tuple, dict]" = OrderedDict()
         self.last_energy_f64: Optional[torch.Tensor] = None
         logger.info("Initialized StudentForceField: %s parameters, %dD hidden, %d interactions, "
@@ -455,6 +457,53 @@ class StudentForceField(nn.Module):
         self._plans[key] = key_entry
         return key_entry
 
+    def _check_numbers(self, atomic_numbers: torch.Tensor):
+        """Element numbers must index the embedding table: one host read of
+        their range, skipped for the tensor checked last while it has not
+        been modified (same object, same version counter)."""
+        if (atomic_numbers is self._z_checked and
+                atomic_numbers._version == self._z_checked_version):
+            return
+        if atomic_numbers.numel():
+            lo, hi = torch.aminmax(atomic_numbers)
+            zmin, zmax = (int(v) for v in torch.stack([lo, hi]).tolist())
+        else:
+            zmin = zmax = 1
+        if zmin < 0 or zmax > self.max_z:
+            raise ValueError(f"atomic numbers must lie in [0, {self.max_z}]")
+        self._z_checked = atomic_numbers
+        self._z_checked_version = atomic_numbers._version
+
+    def _empty_results(self, sizes, device, need_forces, need_virial):
+        # no atoms: every sum is empty, as in the reference's forward
+        # (student_model.py:734-757), so no plan is needed
+        B = max(len(sizes), 1)
+        return (torch.zeros(B, dtype=torch.float64, device=device),
+                torch.zeros(0, 3, dtype=torch.float32, device=device) if need_forces else None,
+                torch.zeros(B, 3, 3, dtype=torch.float64, device=device) if need_virial else None)
+
+    def _prepare(self, atomic_numbers, positions, sizes, cells, pbc):
+        device = positions.device
+        if device.type != "cuda":
+            raise RuntimeError("StudentForceField (MI355X build) needs tensors on a ROCm device; "
+                               f"got {device}")
+        periodic = pbc is not None and bool(np.asarray(pbc).any())
+        if periodic:
+            engine.check_cells(cells, pbc, self.cutoff)
+        self._check_numbers(atomic_numbers)
+        return device, periodic
+
+    def _launch(self, entry, pos, B, need_forces, need_virial):
+        device = pos.device
+        plan = entry["plan"]
+        N = pos.shape[0]
+        energy = torch.empty(B, dtype=torch.float64, device=device)
+        forces = torch.empty(N, 3, dtype=torch.float32, device=device) if need_forces else None
+        virial = torch.empty(B, 9, dtype=torch.float64, device=device) if need_virial else None
+        plan.counts[C_OVERFLOW].zero_()
+        plan.energy_forces(pos, energy, forces, virial)
+        return energy, forces, (virial.view(B, 3, 3) if virial is not None else None)
+
     def evaluate(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
                  sizes: np.ndarray, cells=None, pbc=None, need_forces: bool = True,
                  need_virial: bool = False):
@@ -463,44 +512,57 @@ class StudentForceField(nn.Module):
         Returns (energy float64 [B], forces float32 [N,3] or None,
         virial float64 [B,3,3] or None).
         """
-        device = positions.device
-        if device.type != "cuda":
-            raise RuntimeError("StudentForceField (MI355X build) needs tensors on a ROCm device; "
-                               f"got {device}")
-        periodic = pbc is not None and bool(np.asarray(pbc).any())
-        if periodic:
-            engine.check_cells(cells, pbc, self.cutoff)
-        zmin = int(atomic_numbers.min()) if atomic_numbers.numel() else 1
-        zmax = int(atomic_numbers.max()) if atomic_numbers.numel() else 1
-        if zmin < 0 or zmax > self.max_z:
-            raise ValueError(f"atomic numbers must lie in [0, {self.max_z}]")
+        device, periodic = self._prepare(atomic_numbers, positions, sizes, cells, pbc)
         if positions.shape[0] == 0:
-            # no atoms: every sum is empty, as in the reference's forward
-            # (student_model.py:734-757), so no plan is needed
-            B = max(len(sizes), 1)
-            return (torch.zeros(B, dtype=torch.float64, device=device),
-                    torch.zeros(0, 3, dtype=torch.float32, device=device) if need_forces else None,
-                    torch.zeros(B, 3, 3, dtype=torch.float64, device=device)
-                    if need_virial else None)
+            return self._empty_results(sizes, device, need_forces, need_virial)
         pos = positions.detach().to(torch.float32).contiguous()
         entry = self._plan_for(sizes, device, periodic, cells, pbc, pos)
         self.packed_weights(device)
         entry["numbers"].copy_(atomic_numbers.to(torch.int64))
-        B = len(sizes)
-        N = pos.shape[0]
         for _attempt in range(2):
-            plan = entry["plan"]
-            energy = torch.empty(B, dtype=torch.float64, device=device)
-            forces = torch.empty(N, 3, dtype=torch.float32, device=device) if need_forces else None
-            virial = torch.empty(B, 9, dtype=torch.float64, device=device) if need_virial else None
-            plan.counts[C_OVERFLOW].zero_()
-            plan.energy_forces(pos, energy, forces, virial)
-            counts = plan.read_counts()
+            out = self._launch(entry, pos, len(sizes), need_forces, need_virial)
+            counts = entry["plan"].read_counts()
             if counts[C_OVERFLOW] == 0:
-                return energy, forces, (virial.view(B, 3, 3) if virial is not None else None)
+                return out
             entry = self._grow(entry, sizes, device, periodic, cells, pbc,
                                int(counts[C_EDGES]))
         raise RuntimeError("neighbour-list capacity overflow persisted after regrowth")
+
+    def evaluate_async(self, atomic_numbers: torch.Tensor, positions: torch.Tensor,
+                       sizes: np.ndarray, cells=None, pbc=None, need_forces: bool = True,
+                       need_virial: bool = False):
+        """evaluate() enqueued on the current stream without waiting for it.
+
+        Returns a function that completes the call: it waits for the
+        evaluation's capacity check only (the counters travel to pinned host
+        memory behind the kernels) and returns (energy, forces, virial) as
+        evaluate() does.  If the neighbour list overflowed its capacity the
+        enqueued results are discarded and evaluate() runs again with a
+        regrown plan.  The results are ordered on the stream that was current
+        here: other streams wait for it before reading them.
+        """
+        device, periodic = self._prepare(atomic_numbers, positions, sizes, cells, pbc)
+        if positions.shape[0] == 0:
+            out = self._empty_results(sizes, device, need_forces, need_virial)
+            return lambda: out
+        pos = positions.detach().to(torch.float32).contiguous()
+        entry = self._plan_for(sizes, device, periodic, cells, pbc, pos)
+        self.packed_weights(device)
+        entry["numbers"].copy_(atomic_numbers.to(torch.int64))
+        out = self._launch(entry, pos, len(sizes), need_forces, need_virial)
+        # pinned (torch caches the pages): the copy stays asynchronous
+        host = torch.empty(entry["plan"].counts.numel(), dtype=torch.int32, pin_memory=True)
+        host.copy_(entry["plan"].counts, non_blocking=True)
+        ready = torch.cuda.Event()
+        ready.record()
+
+        def finish():
+            ready.synchronize()
+            if int(host[C_OVERFLOW]) == 0:
+                return out
+            return self.evaluate(atomic_numbers, positions, sizes, cells, pbc, need_forces,
+                                 need_virial)
+        return finish
 
     # ------------------------------------------------------------------
     # reference API

@@ -623,14 +623,19 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
 def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
                          E_label: Tensor, F_label: Tensor, forces: Tensor,
                          w_energy: float = 1.0, w_force: float = 100.0,
-                         rbf_grads: bool = False):
+                         rbf_grads: bool = False, primal=None):
     """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and its
     exact gradient with respect to every parameter in ``p``.  ``forces``
-    are the model's forces at ``x`` (from the fused evaluation)."""
+    are the model's forces at ``x`` (from the fused evaluation); ``primal``
+    is forward()'s (E, cache) at ``x`` when the caller already ran it."""
     F = forces.to(torch.float32)
     eF = F - F_label.to(torch.float32)
     dLdF = w_force * 2.0 * eF / eF.numel()
-    E, cache = forward(p, Z, x, g, rc, lam=dLdF)
+    if primal is None:
+        E, cache = forward(p, Z, x, g, rc, lam=dLdF)
+    else:
+        E, cache = primal
+        tangent(p, cache, g, dLdF)
     B = E.shape[0]
     eE = E - E_label.to(torch.float64)
     loss = w_energy * (eE ** 2).mean() + w_force * (eF.double() ** 2).mean()
@@ -667,6 +672,7 @@ class ForceMatchingTrainer:
         self.opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=weight_decay)
         self._p = None
         self._p_sig = None
+        self._side = None
 
     def _params(self) -> Dict[str, Tensor]:
         """Detached views of the model's state (the optimizer updates the
@@ -679,15 +685,33 @@ class ForceMatchingTrainer:
             self._p_sig = sig
         return self._p
 
+    def _side_stream(self, device):
+        # default priority, like every stream of this library
+        if self._side is None or self._side.device != device:
+            self._side = torch.cuda.Stream(device)
+        return self._side
+
     def loss_and_grads(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None):
+        """The fused HIP evaluation of the current weights (the forces the
+        loss needs) runs on a second stream while the host enqueues the
+        training forward on the current one; the two join before the
+        tangent along dL/dF, which needs those forces."""
         x = x.detach().to(torch.float32).contiguous()
-        g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
         p = self._params()
-        # forces of the current weights from the fused HIP evaluation
-        _, F, _ = self.model.evaluate(Z, x, np.asarray(sizes), cells, pbc)
+        main = torch.cuda.current_stream(x.device)
+        side = self._side_stream(x.device)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            finish = self.model.evaluate_async(Z, x, np.asarray(sizes), cells, pbc)
+        g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
         with torch.no_grad():
+            primal = forward(p, Z, x, g, self.model.cutoff)
+            _, F, _ = finish()
+            main.wait_stream(side)
+            F.record_stream(main)
             return force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label, F,
-                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads)
+                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads,
+                                        primal=primal)
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)

@@ -84,6 +84,10 @@ def test_plan_workspace_accounting():
     rc = C.lib().mlff_plan_bind(h, None, 0, None, None, None, None, None, None)
     assert rc == -4
     assert C.lib().mlff_energy_forces(h, None, None, None, None, None, None) == -5
+    # a weight rebind needs a bound plan; a null plan is refused the same way
+    assert C.lib().mlff_plan_set_weights(h, None, ctypes.c_void_p(1 << 20)) == -5
+    assert b"plan not bound" in C.lib().mlff_last_error()
+    assert C.lib().mlff_plan_set_weights(None, None, ctypes.c_void_p(1 << 20)) == -5
     C.lib().mlff_plan_destroy(h)
 
 

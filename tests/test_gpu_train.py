@@ -256,3 +256,60 @@ def test_periodic_training_gradients_vs_fp64_autograd(gpu):
         worst = max(worst, rel)
         assert rel < 2e-3, (k, rel)
     note(f"worst relative gradient error {worst:.2e}")
+
+
+def test_overlapped_evaluation_matches_serial_path(gpu):
+    """The trainer runs the fused evaluation on a second stream while the
+    training forward is enqueued; loss and gradients are bit-identical to the
+    serial path (evaluate, then forward with its tangent)."""
+    note("trainer: overlapped evaluation vs serial path")
+    synth = pkg_mod("synth")
+    T = pkg_mod("train")
+    torch.manual_seed(1)
+    teacher = pkg().StudentForceField()
+    torch.manual_seed(0)
+    student = pkg().StudentForceField().to(gpu)
+    mols = synth.molecule_set(16, 20, 60, seed=5)
+    Z, P, sizes, b, El, Fl = _labels({k: v.double() for k, v in teacher.state_dict().items()},
+                                     mols)
+    Zt, Pt = torch.tensor(Z, device=gpu), torch.tensor(P, device=gpu)
+    El, Fl = El.float().to(gpu), Fl.float().to(gpu)
+    tr = T.ForceMatchingTrainer(student)
+    l1, E1, F1, g1 = tr.loss_and_grads(Zt, Pt, sizes, El, Fl)
+    _, F, _ = student.evaluate(Zt, Pt, sizes)
+    g = T.device_graph(Zt, Pt, sizes, student.cutoff)
+    with torch.no_grad():
+        l2, E2, F2, g2 = T.force_matching_grads(tr._params(), Zt, Pt, g, student.cutoff, El, Fl,
+                                                F, rbf_grads=tr.rbf_grads)
+    assert float(l1) == float(l2)
+    assert torch.equal(E1, E2) and torch.equal(F1, F2)
+    assert set(g1) == set(g2)
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
+
+
+def test_async_evaluation_regrows_on_overflow(gpu):
+    """evaluate_async on another stream gives evaluate()'s results, and when
+    the neighbour list outgrows the plan's capacity (the same batch
+    compressed to half its size) the completion re-evaluates with a regrown
+    plan."""
+    note("evaluate_async: side stream, capacity overflow")
+    synth = pkg_mod("synth")
+    torch.manual_seed(0)
+    model = pkg().StudentForceField().to(gpu)
+    Z, P, mp = synth.flatten(synth.molecule_set(8, 30, 60, seed=9))
+    sizes = np.diff(mp)
+    Zt = torch.tensor(Z, device=gpu)
+    for scale in (1.0, 0.5):
+        Pt = torch.tensor(P * scale, dtype=torch.float32, device=gpu)
+        cap0 = [int(e["plan"].ecap) for e in model._plans.values()]
+        side = torch.cuda.Stream(gpu)
+        side.wait_stream(torch.cuda.current_stream(gpu))
+        with torch.cuda.stream(side):
+            finish = model.evaluate_async(Zt, Pt, sizes)
+        Ea, Fa, _ = finish()
+        torch.cuda.current_stream(gpu).wait_stream(side)
+        if scale < 1.0:
+            assert cap0 and int(list(model._plans.values())[-1]["plan"].ecap) > cap0[-1]
+        Eb, Fb, _ = model.evaluate(Zt, Pt, sizes)
+        assert torch.equal(Ea, Eb) and torch.equal(Fa, Fb)
