@@ -35,7 +35,8 @@ class BatchedNVE:
                  temperature_K: float = 300.0, seed: int = 0, momenta: Optional[np.ndarray] = None,
                  max_steps: int = 100000, headroom: float = 1.3, device="cuda:0",
                  persistent_blocks: int = 0, streams: Optional[int] = None,
-                 record_frames: bool = False, masses: Optional[np.ndarray] = None):
+                 record_frames: bool = False, masses: Optional[np.ndarray] = None,
+                 frame_window: Optional[int] = None):
         self.model = model
         self.device = engine.device_of(device)
         self.structures = list(structures)
@@ -75,9 +76,20 @@ class BatchedNVE:
         self.max_steps = int(max_steps)
         self.ledger = torch.zeros(self.max_steps + 1, self.B, 2, dtype=torch.float64, device=dev)
         # per-step frames (x, p, F), written by the second half-kick on the device
+        # into row (step mod rows): one row per step of the run, or a ring of
+        # frame_window rows drained to the host after every chunk of steps
         self.frame_x = self.frame_p = self.frame_f = None
+        self._frame_window = None
+        self._host_frames = None
         if record_frames:
-            shape = (self.max_steps + 1, self.n_atoms, 3)
+            rows = self.max_steps + 1
+            if frame_window is not None:
+                if int(frame_window) < 2:
+                    raise ValueError("frame_window must hold at least 2 frames")
+                rows = min(rows, int(frame_window))
+                self._frame_window = rows
+                self._host_frames = ([], [], [])
+            shape = (rows, self.n_atoms, 3)
             self.frame_x = torch.zeros(shape, dtype=torch.float64, device=dev)
             self.frame_p = torch.zeros(shape, dtype=torch.float64, device=dev)
             self.frame_f = torch.zeros(shape, dtype=torch.float32, device=dev)
@@ -113,6 +125,8 @@ class BatchedNVE:
             self.frame_x[0] = self.x
             self.frame_p[0] = self.p
             self.frame_f[0] = self.forces
+            if self._frame_window is not None:
+                self._drain_frames(0, 1)
         self.tuning = None
 
     def autotune(self, **kw):
@@ -151,7 +165,7 @@ class BatchedNVE:
         if self.frame_x is not None:
             C.check("mlff_md_bind_frames", self.lib.mlff_md_bind_frames(
                 self.plan.handle, C.ptr(self.frame_x), C.ptr(self.frame_p),
-                C.ptr(self.frame_f), self.max_steps + 1))
+                C.ptr(self.frame_f), self.frame_x.shape[0]))
         # the ledger row index is the device step counter (+1: row 0 holds t = 0)
         self.plan.counts[C.MLFF_COUNT_STEP] = 1
 
@@ -167,6 +181,9 @@ class BatchedNVE:
             raise ValueError("ledger too short; raise max_steps")
         done = 0
         chunk = max(steps_per_graph, 1) * 10
+        if self._frame_window is not None:
+            # a chunk's frames must not wrap onto each other before the drain
+            chunk = min(chunk, self._frame_window)
         replays = 0
         while done < n_steps:
             k = min(chunk, n_steps - done)
@@ -189,8 +206,17 @@ class BatchedNVE:
                 self.plan.counts[C.MLFF_COUNT_STEP] = step
                 self.pos32.copy_(self.x.to(torch.float32))
                 continue
+            if self._frame_window is not None:
+                self._drain_frames(self.steps_done + 1, k)
             done += k
             self.steps_done += k
+
+    def _drain_frames(self, first: int, count: int):
+        """Copy the frames of steps first .. first+count-1 from the ring to
+        the host."""
+        rows = (torch.arange(first, first + count, device=self.device) % self._frame_window)
+        for buf, dev_buf in zip(self._host_frames, (self.frame_x, self.frame_p, self.frame_f)):
+            buf.append(dev_buf.index_select(0, rows).cpu())
 
     def _run_chunk(self, k: int, graph: bool, steps_per_graph: int):
         h = self.plan.handle
@@ -220,6 +246,10 @@ class BatchedNVE:
         record_frames=True."""
         if self.frame_x is None:
             raise RuntimeError("frames were not recorded (record_frames=False)")
+        if self._frame_window is not None:
+            fx, fp, ff = (torch.cat(b) for b in self._host_frames)
+            v = fp * self.inv_mass.cpu()[None, :, None]
+            return fx.numpy(), v.numpy(), ff.double().numpy()
         n = self.steps_done + 1
         v = self.frame_p[:n] * self.inv_mass[None, :, None]
         return (self.frame_x[:n].cpu().numpy(), v.cpu().numpy(),
@@ -282,6 +312,10 @@ def write_extxyz(path, numbers, positions, velocities, forces, info, cell=None, 
                 fh.write(f"{_SYMBOLS[int(z)]:<2} " + " ".join(f"{c:.10f}" for c in x) + " "
                          + " ".join(f"{c:.10f}" for c in v) + " "
                          + " ".join(f"{c:.10f}" for c in f) + "\n")
+
+
+# device frames kept by NVEMDHarness between host drains
+FRAME_WINDOW = 1000
 
 
 class NVEMDHarness:
@@ -388,10 +422,13 @@ class NVEMDHarness:
         try:
             if initialize_velocities:
                 self.initialize_velocities()
+            # the device keeps a ring of FRAME_WINDOW frames, drained to the host
+            # after every chunk of steps
             md = BatchedNVE(self.calculator.model, [self._structure()],
                             timestep_fs=self.timestep_fs, momenta=self._atoms_momenta(),
                             max_steps=steps, device=self.calculator.device,
-                            record_frames=True, masses=self._masses())
+                            record_frames=True, masses=self._masses(),
+                            frame_window=FRAME_WINDOW)
             md.run(steps, graph=steps >= 10, steps_per_graph=10)
             epot, ekin = md.energies()
             x, v, f = md.frames()

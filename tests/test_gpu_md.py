@@ -128,3 +128,27 @@ def test_graph_with_filter_stream_matches_one_stream(gpu):
     for other in (one, eager):
         assert torch.equal(two.x, other.x) and torch.equal(two.p, other.p)
         assert np.array_equal(two.energies()[0], other.energies()[0])
+
+
+def test_frame_ring_drains_every_step(gpu):
+    """Frames recorded into a 7-row device ring, drained to the host after
+    every chunk (graph chunks of 3 steps and eager remainders), equal the
+    frames of a run that keeps one device row per step."""
+    note("frame ring of 7 rows vs one row per step")
+    synth = pkg_mod("synth")
+    md_mod = pkg_mod("md")
+    model = _model(64, 2, 20)
+    mols = synth.molecule_set(3, 10, 30, seed=8)
+    full = md_mod.BatchedNVE(model, mols, seed=3, max_steps=30, record_frames=True)
+    ring = md_mod.BatchedNVE(model, mols, seed=3, max_steps=30, record_frames=True,
+                             frame_window=7)
+    assert ring.frame_x.shape[0] == 7
+    for md in (full, ring):
+        md.run(20, graph=True, steps_per_graph=3)
+        md.run(10, graph=False)
+    a, b = full.frames(), ring.frames()
+    for u, w in zip(a, b):
+        assert u.shape == (31, full.n_atoms, 3)
+        assert np.array_equal(u, w)
+    with pytest.raises(ValueError):
+        md_mod.BatchedNVE(model, mols, seed=3, max_steps=5, record_frames=True, frame_window=1)
