@@ -431,6 +431,28 @@ int mlff_silu_chain(void* stream, int64_t n, const float* a_dev, const float* da
                     const float* g_dev, const float* dg_dev, float* h_dev, float* dh_dev,
                     float* ga_dev, float* dga_dev);
 
+/* The filter MLP of PaiNNMessage (student_model.py:313-322, 350-355) over
+ * n_pairs pairs in the training pass, fused in one pair-tile kernel:
+ *   tangent = 0:  a = x W1^T + b1,  h = silu(a),         W = h W2^T + b2
+ *   tangent = 1:  a = x W1^T,       h = silu'(a_dev) a,  W = h W2^T
+ * x [P][num_rbf] (the basis e, or its tangent), a and h [P][hidden],
+ * W [P][3 hidden], row-major float32.  w1_padded_dev is W1 [hidden][RP]
+ * with RP = 16 (num_rbf <= 16) or 32 and zeros past num_rbf; w2_planes_dev
+ * is W2 [3 hidden][hidden] split by mlff_split_bf16x3.  The second product
+ * runs on the bf16 matrix cores with both operands in three bf16 parts
+ * (float32 accuracy).  hidden 32, 64 or 128; num_rbf <= 32.  Replaces
+ * rbf_to_scalar's forward and its autograd tangent in the reference's
+ * double backward (distillation_wrapper.py:86-91). */
+int mlff_train_filter(void* stream, int tangent, int hidden, int n_pairs, int num_rbf,
+                      const float* x_dev, const float* w1_padded_dev, const float* b1_dev,
+                      const uint16_t* w2_planes_dev, const float* b2_dev, const float* a_dev,
+                      float* a_out_dev, float* h_out_dev, float* w_out_dev);
+
+/* Split n float32 values into three bfloat16 planes (planes_dev [3][n]):
+ * x = hi + mid + lo, each part the round-to-nearest-even bfloat16 of what the
+ * previous parts leave (24 significant bits together). */
+int mlff_split_bf16x3(void* stream, int64_t n, const float* src_dev, uint16_t* planes_dev);
+
 /* Elementwise parts of PaiNNUpdate.forward (student_model.py:430-470) in the
  * training pass, around the update MLP's two products.  s [N][H],
  * v [N][3][H], xin [N][2H], y = MLP(xin) = (ds, g1, g2) [N][3H], M the 3x3

@@ -242,6 +242,58 @@ def _silu_dd(x, g, dx, dg=None):
     return _silu_call(x, da=dx, g=g, dg=dg, dga=True)
 
 
+# prepared filter weights per (W1, W2) tensor pair: W1 padded to 16 or 32
+# columns and W2 as three bf16 planes, remade when the optimizer has
+# written either tensor (its version counter moved)
+_FILTER_W: Dict[tuple, tuple] = {}
+
+
+def _filter_weights(p: Dict[str, Tensor], pre: str):
+    W1 = _dense(p[pre + ".0.weight"])
+    W2 = _dense(p[pre + ".2.weight"])
+    key = (W1.data_ptr(), W2.data_ptr())
+    ver = (W1._version, W2._version)
+    ent = _FILTER_W.get(key)
+    if ent is None or ent[0] != ver:
+        H, R = W1.shape
+        RP = 16 if R <= 16 else 32
+        if ent is None:
+            if len(_FILTER_W) > 64:
+                _FILTER_W.clear()
+            w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
+            planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
+        else:
+            w1p, planes = ent[1], ent[2]
+        w1p[:, :R].copy_(W1)
+        C.check("mlff_split_bf16x3", C.lib().mlff_split_bf16x3(
+            C.stream_handle(), W2.numel(), C.ptr(W2), C.ptr(planes)))
+        ent = (ver, w1p, planes)
+        _FILTER_W[key] = ent
+    return ent[1], ent[2]
+
+
+def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = None):
+    """The filter MLP on the fused pair kernel (mlff_train_filter).  Forward
+    (a None): x = e gives (a1, h, W) with a1 = e W1^T + b1, h = silu(a1),
+    W = h W2^T + b2.  Tangent: x = de and a = a1 give (da1, dh, dW) with
+    da1 = de W1^T, dh = silu'(a1) da1, dW = dh W2^T."""
+    w1p, planes = _filter_weights(p, pre)
+    x = _dense(x)
+    P, R = x.shape
+    H = w1p.shape[0]
+    a_out = torch.empty(P, H, dtype=torch.float32, device=x.device)
+    h_out = torch.empty_like(a_out)
+    w_out = torch.empty(P, 3 * H, dtype=torch.float32, device=x.device)
+    tan = a is not None
+    b1 = None if tan else _dense(p[pre + ".0.bias"])
+    b2 = None if tan else _dense(p[pre + ".2.bias"])
+    C.check("mlff_train_filter", C.lib().mlff_train_filter(
+        C.stream_handle(), int(tan), H, P, R, C.ptr(x), C.ptr(w1p), C.ptr(b1), C.ptr(planes),
+        C.ptr(b2), C.ptr(_dense(a) if tan else None), C.ptr(a_out), C.ptr(h_out),
+        C.ptr(w_out)))
+    return a_out, h_out, w_out
+
+
 def _edge_features(d, centers, widths, rc):
     """e_k(d) = exp(-(d-mu_k)^2 / w_k^2) * 0.5 (cos(pi d/rc) + 1) [d < rc]
     (student_model.py:249-253, 285-290, 720-722) and de/dd."""
@@ -367,9 +419,7 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
     layers = []
     for k in range(L):
         pre = f"interactions.{k}.message.rbf_to_scalar"
-        a1 = _lin(e, p, pre + ".0")
-        h = _silu(a1)
-        W = _lin(h, p, pre + ".2")
+        a1, h, W = _filter(p, pre, e)
         s_m, v_m, _, _ = _message(g, H, s, v, W, u)
         upre = f"interactions.{k}.update"
         # PaiNNUpdate.forward (student_model.py:430-470)
@@ -414,9 +464,7 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
     for k, lay in enumerate(c["layers"]):
         pre = f"interactions.{k}.message.rbf_to_scalar"
         lay["ds"], lay["dv"] = ds_, dv_
-        da1 = de_t @ p[pre + ".0.weight"].T
-        dh = _silu_d(lay["a1"], da1)
-        dW = dh @ p[pre + ".2.weight"].T
+        da1, dh, dW = _filter(p, pre, de_t, lay["a1"])
         lay["da1"], lay["dh"], lay["dW"] = da1, dh, dW
         _, _, dsm, dvm = _message(g, H, lay["s"], lay["v"], lay["W"], c["u"], ds_, dv_, dW, du)
         v_m = lay["v_m"]
@@ -623,19 +671,14 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
 def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
                          E_label: Tensor, F_label: Tensor, forces: Tensor,
                          w_energy: float = 1.0, w_force: float = 100.0,
-                         rbf_grads: bool = False, primal=None):
+                         rbf_grads: bool = False):
     """loss = wE MSE(E) + wF MSE(F) (train_compact_models.py:340-342) and its
     exact gradient with respect to every parameter in ``p``.  ``forces``
-    are the model's forces at ``x`` (from the fused evaluation); ``primal``
-    is forward()'s (E, cache) at ``x`` when the caller already ran it."""
+    are the model's forces at ``x`` (from the fused evaluation)."""
     F = forces.to(torch.float32)
     eF = F - F_label.to(torch.float32)
     dLdF = w_force * 2.0 * eF / eF.numel()
-    if primal is None:
-        E, cache = forward(p, Z, x, g, rc, lam=dLdF)
-    else:
-        E, cache = primal
-        tangent(p, cache, g, dLdF)
+    E, cache = forward(p, Z, x, g, rc, lam=dLdF)
     B = E.shape[0]
     eE = E - E_label.to(torch.float64)
     loss = w_energy * (eE ** 2).mean() + w_force * (eF.double() ** 2).mean()
@@ -672,7 +715,6 @@ class ForceMatchingTrainer:
         self.opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=weight_decay)
         self._p = None
         self._p_sig = None
-        self._side = None
 
     def _params(self) -> Dict[str, Tensor]:
         """Detached views of the model's state (the optimizer updates the
@@ -685,33 +727,15 @@ class ForceMatchingTrainer:
             self._p_sig = sig
         return self._p
 
-    def _side_stream(self, device):
-        # default priority, like every stream of this library
-        if self._side is None or self._side.device != device:
-            self._side = torch.cuda.Stream(device)
-        return self._side
-
     def loss_and_grads(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None):
-        """The fused HIP evaluation of the current weights (the forces the
-        loss needs) runs on a second stream while the host enqueues the
-        training forward on the current one; the two join before the
-        tangent along dL/dF, which needs those forces."""
         x = x.detach().to(torch.float32).contiguous()
-        p = self._params()
-        main = torch.cuda.current_stream(x.device)
-        side = self._side_stream(x.device)
-        side.wait_stream(main)
-        with torch.cuda.stream(side):
-            finish = self.model.evaluate_async(Z, x, np.asarray(sizes), cells, pbc)
         g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
+        p = self._params()
+        # forces of the current weights from the fused HIP evaluation
+        _, F, _ = self.model.evaluate(Z, x, np.asarray(sizes), cells, pbc)
         with torch.no_grad():
-            primal = forward(p, Z, x, g, self.model.cutoff)
-            _, F, _ = finish()
-            main.wait_stream(side)
-            F.record_stream(main)
             return force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label, F,
-                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads,
-                                        primal=primal)
+                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads)
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)

@@ -218,6 +218,28 @@ def test_silu_chain_checks_arguments_on_the_host():
     assert lib.mlff_silu_chain(None, 0, f, None, None, None, f, None, None, None) == 0
 
 
+def test_train_filter_checks_arguments_on_the_host():
+    """The fused training filter kernel refuses unsupported hidden sizes, a
+    basis wider than 32, null buffers (the tangent needs a, the forward the
+    biases) and misaligned weights before any device call; no pairs is a
+    no-op.  The bf16 split refuses negative sizes and null buffers."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    ok = (f, f, f, f, f, None, f, f, f)               # forward: x w1 b1 w2 b2 a out...
+    assert lib.mlff_train_filter(None, 0, 48, 10, 20, *ok) == -2
+    assert lib.mlff_train_filter(None, 0, 128, 10, 33, *ok) == -1
+    assert lib.mlff_train_filter(None, 0, 128, -1, 20, *ok) == -1
+    assert lib.mlff_train_filter(None, 0, 128, 10, 20, f, f, None, f, f, None, f, f, f) == -1
+    assert lib.mlff_train_filter(None, 1, 128, 10, 20, f, f, None, f, None, None, f, f,
+                                 f) == -1          # tangent without a
+    assert lib.mlff_train_filter(None, 0, 128, 10, 20, f, odd, f, f, f, None, f, f, f) == -1
+    assert lib.mlff_train_filter(None, 0, 128, 0, 20, *ok) == 0
+    assert lib.mlff_split_bf16x3(None, -1, f, f) == -1
+    assert lib.mlff_split_bf16x3(None, 4, None, f) == -1
+    assert lib.mlff_split_bf16x3(None, 0, None, None) == 0
+
+
 def test_update_glue_checks_arguments_on_the_host():
     """The update-block kernels refuse bad sizes, null and misaligned
     buffers, and a tangent without its inputs, before any device call."""

@@ -258,36 +258,6 @@ def test_periodic_training_gradients_vs_fp64_autograd(gpu):
     note(f"worst relative gradient error {worst:.2e}")
 
 
-def test_overlapped_evaluation_matches_serial_path(gpu):
-    """The trainer runs the fused evaluation on a second stream while the
-    training forward is enqueued; loss and gradients are bit-identical to the
-    serial path (evaluate, then forward with its tangent)."""
-    note("trainer: overlapped evaluation vs serial path")
-    synth = pkg_mod("synth")
-    T = pkg_mod("train")
-    torch.manual_seed(1)
-    teacher = pkg().StudentForceField()
-    torch.manual_seed(0)
-    student = pkg().StudentForceField().to(gpu)
-    mols = synth.molecule_set(16, 20, 60, seed=5)
-    Z, P, sizes, b, El, Fl = _labels({k: v.double() for k, v in teacher.state_dict().items()},
-                                     mols)
-    Zt, Pt = torch.tensor(Z, device=gpu), torch.tensor(P, device=gpu)
-    El, Fl = El.float().to(gpu), Fl.float().to(gpu)
-    tr = T.ForceMatchingTrainer(student)
-    l1, E1, F1, g1 = tr.loss_and_grads(Zt, Pt, sizes, El, Fl)
-    _, F, _ = student.evaluate(Zt, Pt, sizes)
-    g = T.device_graph(Zt, Pt, sizes, student.cutoff)
-    with torch.no_grad():
-        l2, E2, F2, g2 = T.force_matching_grads(tr._params(), Zt, Pt, g, student.cutoff, El, Fl,
-                                                F, rbf_grads=tr.rbf_grads)
-    assert float(l1) == float(l2)
-    assert torch.equal(E1, E2) and torch.equal(F1, F2)
-    assert set(g1) == set(g2)
-    for k in g1:
-        assert torch.equal(g1[k], g2[k]), k
-
-
 def test_async_evaluation_regrows_on_overflow(gpu):
     """evaluate_async on another stream gives evaluate()'s results, and when
     the neighbour list outgrows the plan's capacity (the same batch

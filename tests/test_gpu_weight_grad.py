@@ -87,3 +87,48 @@ def test_silu_chain_matches_float64(gpu):
     assert rel(got_dd0, gg.double() * d2 * t.double()) < 4e-6
 
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H,R,P", [(128, 20, 50132), (128, 20, 37), (64, 16, 1000),
+                                   (32, 16, 4099), (128, 32, 300)])
+def test_train_filter_matches_float64(gpu, H, R, P):
+    """The fused filter MLP of the training pass (forward: a1, silu(a1) and
+    W; tangent: da1, silu'(a1) da1 and dW) against float64 products of the
+    same float32 inputs, on full and ragged tiles."""
+    print(f"[case] train_filter H={H} R={R} P={P}", flush=True)
+    g = torch.Generator().manual_seed(H + R + P)
+    pre = "f"
+    p = {pre + ".0.weight": torch.randn(H, R, generator=g) / R ** 0.5,
+         pre + ".0.bias": 0.1 * torch.randn(H, generator=g),
+         pre + ".2.weight": torch.randn(3 * H, H, generator=g) / H ** 0.5,
+         pre + ".2.bias": 0.1 * torch.randn(3 * H, generator=g)}
+    p = {k: v.to(gpu) for k, v in p.items()}
+    e = torch.rand(P, R, generator=g).to(gpu)
+    de = torch.randn(P, R, generator=g).to(gpu)
+    a1, h, W = train._filter(p, pre, e)
+    da1, dh, dW = train._filter(p, pre, de, a1)
+    again = train._filter(p, pre, de, a1)
+    torch.cuda.synchronize()
+    assert all(torch.equal(x, y) for x, y in zip((da1, dh, dW), again))
+    d = {k: v.double().cpu() for k, v in p.items()}
+    ra1 = e.double().cpu() @ d[pre + ".0.weight"].T + d[pre + ".0.bias"]
+    rh = torch.nn.functional.silu(ra1)
+    rW = rh @ d[pre + ".2.weight"].T + d[pre + ".2.bias"]
+    sig = torch.sigmoid(ra1)
+    rda1 = de.double().cpu() @ d[pre + ".0.weight"].T
+    rdh = sig * (1 + ra1 * (1 - sig)) * rda1
+    rdW = rdh @ d[pre + ".2.weight"].T
+    for got, ref, name in ((a1, ra1, "a1"), (h, rh, "h"), (W, rW, "W"), (da1, rda1, "da1"),
+                           (dh, rdh, "dh"), (dW, rdW, "dW")):
+        err = (got.double().cpu() - ref).abs().max().item()
+        scale = ref.abs().max().item()
+        # float32 inputs and float32 accumulation: a few ulps of the largest term
+        assert err <= 2e-6 * max(scale, 1.0), (name, err, scale)
+    # a weight update is seen: the prepared weights follow the version counter
+    with torch.no_grad():
+        p[pre + ".2.weight"].mul_(2.0)
+    _, _, W2x = train._filter(p, pre, e)
+    torch.cuda.synchronize()
+    err = (W2x.double().cpu() - (2.0 * (rW - d[pre + ".2.bias"]) + d[pre + ".2.bias"])).abs().max()
+    assert err.item() <= 4e-6 * max(rW.abs().max().item(), 1.0)
