@@ -103,6 +103,17 @@ enum mlff_tensor_id {
 int mlff_weights_layout(int hidden_dim, int num_interactions, int num_rbf, int max_z,
                         mlff_slot* slots, int max_slots, int* n_slots, int64_t* total_floats);
 
+/* Write the whole blob (total_floats, as mlff_weights_layout reports) on the
+ * device in one launch: table_dev holds n_slots rows of 8 int64, ascending
+ * slot offset: {slot offset, slot rows, slot ld, tensor rows, tensor cols,
+ * kind, address of the tensor's float32 data on the device, 0}; a kind-1
+ * slot holds the transpose of a [cols][rows] tensor.  Padding and the gaps
+ * between slots are written as zeros.  n_slots <= 256.  Replaces reading
+ * the parameters in place (student_model.py:585-605) after an optimizer
+ * step. */
+int mlff_pack_weights(void* stream, int n_slots, const int64_t* table_dev, int64_t total_floats,
+                      float* blob_dev);
+
 /* ---------------------------------------------------------------------------
  * Neighbour list.  Replaces radius_graph / radius_graph_native
  * (student_model.py:63-109, 165-191): all ordered pairs i != j of the same

@@ -47,7 +47,7 @@ EXPORTS = (
     "mlff_update_norm", "mlff_update_apply", "mlff_update_norm_tangent",
     "mlff_update_apply_tangent", "mlff_update_gate_adjoint", "mlff_update_input_adjoint",
     "mlff_probe_mfma_flops", "mlff_probe_mfma", "mlff_plan_set_weights",
-    "mlff_train_filter", "mlff_split_bf16x3",
+    "mlff_train_filter", "mlff_split_bf16x3", "mlff_pack_weights",
 )
 
 # bytes of the static device scratch behind mlff_energy_forces_static
@@ -174,6 +174,8 @@ def _declare(lib):
                                       ctypes.c_int] + [vp] * 9
     lib.mlff_split_bf16x3.restype = ctypes.c_int
     lib.mlff_split_bf16x3.argtypes = [vp, ctypes.c_int64, vp, vp]
+    lib.mlff_pack_weights.restype = ctypes.c_int
+    lib.mlff_pack_weights.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int64, vp]
     lib.mlff_probe_mfma_flops.restype = ctypes.c_int64
     lib.mlff_probe_mfma_flops.argtypes = [ctypes.c_int] * 3
     lib.mlff_probe_mfma.restype = ctypes.c_int

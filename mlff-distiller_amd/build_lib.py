@@ -23,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "libmlff_amd.so"
 SOURCES = ("capi.hip", "nl.hip", "painn.hip", "md.hip", "train.hip", "train_update.hip", "train_filter.hip",
-           "probe.hip")
+           "pack.hip", "probe.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -63,40 +63,64 @@ def hyper_from_state(sd: Dict[str, torch.Tensor]) -> Tuple[int, int, int, int]:
 
 
 _LAYOUTS: Dict[tuple, tuple] = {}
+# device slot tables of mlff_pack_weights, per (layout, tensor addresses)
+_PACK_TABLES: Dict[tuple, torch.Tensor] = {}
 
 
-def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
-    """Pack a StudentForceField state dict into the library's float32 blob
-    (on the device directly when the parameters already live there, so a
-    training step's weight update costs device copies only: one fused
-    multi-tensor copy)."""
+def _layout(sd: Dict[str, torch.Tensor]):
     H, L, R, max_z = hyper_from_state(sd)
     key = (H, L, R, max_z)
     if key not in _LAYOUTS:
         _LAYOUTS[key] = C.weights_layout(H, L, R, max_z)
-    slots, total = _LAYOUTS[key]
+    return key, _LAYOUTS[key]
+
+
+def pack_weights(sd: Dict[str, torch.Tensor], device) -> torch.Tensor:
+    """Pack a StudentForceField state dict into the library's float32 blob.
+
+    When every tensor is a contiguous float32 tensor on the device already
+    (a model being trained), one kernel (mlff_pack_weights) writes the whole
+    blob from the tensors in place, so a weight update costs one launch and no
+    host round trip; otherwise the blob is assembled on the host and copied
+    over.  Either way the blob is a fresh tensor."""
+    key, (slots, total) = _layout(sd)
     device = device_of(device)
-    on_device = all(v.device == device for v in sd.values())
-    where = device if on_device else torch.device("cpu")
-    host = torch.zeros(total, dtype=torch.float32, device=where)
-    dsts, srcs = [], []
+    srcs = []
     for s in slots:
-        name = _NAMES[s.tensor].format(l=s.layer)
-        t = sd[name].detach()
-        if t.device != where or t.dtype != torch.float32:
-            t = t.to(where, torch.float32)
+        srcs.append(sd[_NAMES[s.tensor].format(l=s.layer)].detach())
+    if device.type == "cuda" and all(t.device == device and t.dtype == torch.float32 and
+                                     t.is_contiguous() for t in srcs):
+        tkey = (key, device) + tuple(t.data_ptr() for t in srcs)
+        table = _PACK_TABLES.get(tkey)
+        if table is None:
+            rows = []
+            for s, t in sorted(zip(slots, srcs), key=lambda st: st[0].offset):
+                tr, tc = (1, t.numel()) if t.dim() == 1 else tuple(t.shape)
+                if s.kind == 1:
+                    tr, tc = tc, tr
+                if tr > s.rows or tc > s.ld:
+                    raise ValueError(f"slot too small for {tuple(t.shape)}")
+                rows.append([s.offset, s.rows, s.ld, tr, tc, s.kind, t.data_ptr(), 0])
+            if len(_PACK_TABLES) > 32:
+                _PACK_TABLES.clear()
+            table = torch.tensor(rows, dtype=torch.int64).to(device)
+            _PACK_TABLES[tkey] = table
+        blob = torch.empty(total, dtype=torch.float32, device=device)
+        C.check("mlff_pack_weights", C.lib().mlff_pack_weights(
+            C.stream_handle(), table.shape[0], C.ptr(table), total, C.ptr(blob)))
+        return blob
+    host = torch.zeros(total, dtype=torch.float32)
+    for s, t in zip(slots, srcs):
+        t = t.to("cpu", torch.float32)
         if t.dim() == 1:
             t = t.unsqueeze(0)
         if s.kind == 1:
             t = t.t()
         rows, cols = t.shape
         if rows > s.rows or cols > s.ld:
-            raise ValueError(f"slot for {name} too small ({rows}x{cols} into {s.rows}x{s.ld})")
-        view = host[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)
-        dsts.append(view[:rows, :cols])
-        srcs.append(t)
-    torch._foreach_copy_(dsts, srcs)
-    return host if on_device else host.to(device)
+            raise ValueError(f"slot too small ({rows}x{cols} into {s.rows}x{s.ld})")
+        host[s.offset:s.offset + s.rows * s.ld].view(s.rows, s.ld)[:rows, :cols] = t
+    return host if device.type == "cpu" else host.to(device)
 
 
 # ---------------------------------------------------------------------------

@@ -39,6 +39,7 @@ device; energies are summed in float64.
 from __future__ import annotations
 
 import math
+import weakref
 from dataclasses import dataclass
 from typing import Dict, Optional, Sequence
 
@@ -243,33 +244,33 @@ def _silu_dd(x, g, dx, dg=None):
 
 
 # prepared filter weights per (W1, W2) tensor pair: W1 padded to 16 or 32
-# columns and W2 as three bf16 planes, remade when the optimizer has
-# written either tensor (its version counter moved)
+# columns and W2 as three bf16 planes.  An entry belongs to the two tensor
+# objects themselves (weak references: a tensor allocated later at a freed
+# one's address, or under its id, never matches) and is remade when the
+# optimizer has written either tensor (its version counter moved).
 _FILTER_W: Dict[tuple, tuple] = {}
 
 
 def _filter_weights(p: Dict[str, Tensor], pre: str):
-    W1 = _dense(p[pre + ".0.weight"])
-    W2 = _dense(p[pre + ".2.weight"])
-    key = (W1.data_ptr(), W2.data_ptr())
-    ver = (W1._version, W2._version)
+    W1o, W2o = p[pre + ".0.weight"], p[pre + ".2.weight"]
+    key = (id(W1o), id(W2o))
+    state = (W1o.data_ptr(), W2o.data_ptr(), W1o._version, W2o._version)
     ent = _FILTER_W.get(key)
-    if ent is None or ent[0] != ver:
-        H, R = W1.shape
-        RP = 16 if R <= 16 else 32
-        if ent is None:
-            if len(_FILTER_W) > 64:
-                _FILTER_W.clear()
-            w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
-            planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
-        else:
-            w1p, planes = ent[1], ent[2]
-        w1p[:, :R].copy_(W1)
-        C.check("mlff_split_bf16x3", C.lib().mlff_split_bf16x3(
-            C.stream_handle(), W2.numel(), C.ptr(W2), C.ptr(planes)))
-        ent = (ver, w1p, planes)
-        _FILTER_W[key] = ent
-    return ent[1], ent[2]
+    if ent is not None and ent[0]() is W1o and ent[1]() is W2o and ent[2] == state:
+        return ent[3], ent[4]
+    W1, W2 = _dense(W1o), _dense(W2o)
+    H, R = W1.shape
+    RP = 16 if R <= 16 else 32
+    if len(_FILTER_W) > 64:
+        for k in [k for k, e in _FILTER_W.items() if e[0]() is None or e[1]() is None]:
+            del _FILTER_W[k]
+    w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
+    w1p[:, :R].copy_(W1)
+    planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
+    C.check("mlff_split_bf16x3", C.lib().mlff_split_bf16x3(
+        C.stream_handle(), W2.numel(), C.ptr(W2), C.ptr(planes)))
+    _FILTER_W[key] = (weakref.ref(W1o), weakref.ref(W2o), state, w1p, planes)
+    return w1p, planes
 
 
 def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = None):
@@ -280,7 +281,9 @@ def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = Non
     w1p, planes = _filter_weights(p, pre)
     x = _dense(x)
     P, R = x.shape
-    H = w1p.shape[0]
+    H = p[pre + ".0.weight"].shape[0]
+    if w1p.shape[0] != H or planes.numel() != 3 * p[pre + ".2.weight"].numel():
+        raise RuntimeError("prepared filter weights do not match the layer")
     a_out = torch.empty(P, H, dtype=torch.float32, device=x.device)
     h_out = torch.empty_like(a_out)
     w_out = torch.empty(P, 3 * H, dtype=torch.float32, device=x.device)

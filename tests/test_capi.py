@@ -240,6 +240,16 @@ def test_train_filter_checks_arguments_on_the_host():
     assert lib.mlff_split_bf16x3(None, 0, None, None) == 0
 
 
+def test_pack_weights_checks_arguments_on_the_host():
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    assert lib.mlff_pack_weights(None, -1, f, 10, f) == -1
+    assert lib.mlff_pack_weights(None, 257, f, 10, f) == -1
+    assert lib.mlff_pack_weights(None, 3, None, 10, f) == -1
+    assert lib.mlff_pack_weights(None, 3, f, 10, None) == -1
+    assert lib.mlff_pack_weights(None, 3, f, 0, None) == 0
+
+
 def test_update_glue_checks_arguments_on_the_host():
     """The update-block kernels refuse bad sizes, null and misaligned
     buffers, and a tangent without its inputs, before any device call."""

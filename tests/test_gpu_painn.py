@@ -766,3 +766,27 @@ def test_every_launch_shape_is_bit_identical(gpu, H, L, R_):
             assert torch.equal(a, b), combo
     with pytest.raises(C.MlffError):
         plan.set_tiles(48, 0, 0, 0)
+
+
+@pytest.mark.parametrize("H,L,R_", [(128, 3, 20), (32, 2, 16), (64, 1, 32)])
+def test_device_weight_pack_equals_host_pack(gpu, H, L, R_):
+    """mlff_pack_weights writes the blob from the parameters on the device in
+    one launch: bit-identical to the host assembly (transposed slots, biases,
+    padding and gaps zero), and an in-place update of a parameter shows in
+    the next pack."""
+    note(f"device weight pack H{H} L{L} R{R_}")
+    engine = pkg_mod("engine")
+    model = _model(H, L, R_, seed=3, bias_scale=0.1)
+    sd = model.state_dict()
+    dev_blob = engine.pack_weights(sd, gpu)
+    host_blob = engine.pack_weights({k: v.cpu() for k, v in sd.items()}, "cpu")
+    assert dev_blob.device.type == "cuda"
+    assert torch.equal(dev_blob.cpu(), host_blob)
+    with torch.no_grad():
+        model.interactions[0].message.rbf_to_scalar[2].weight.mul_(-1.5)
+        model.energy_head[4].bias.add_(0.25)
+    sd = model.state_dict()
+    again = engine.pack_weights(sd, gpu)
+    assert torch.equal(again.cpu(), engine.pack_weights({k: v.cpu() for k, v in sd.items()},
+                                                        "cpu"))
+    assert not torch.equal(again.cpu(), host_blob)

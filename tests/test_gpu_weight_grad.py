@@ -132,3 +132,28 @@ def test_train_filter_matches_float64(gpu, H, R, P):
     torch.cuda.synchronize()
     err = (W2x.double().cpu() - (2.0 * (rW - d[pre + ".2.bias"]) + d[pre + ".2.bias"])).abs().max()
     assert err.item() <= 4e-6 * max(rW.abs().max().item(), 1.0)
+
+
+@pytest.mark.gpu
+def test_train_filter_weights_follow_the_tensor_objects(gpu):
+    """The prepared filter weights belong to the W1/W2 tensor objects: other
+    tensors at the same addresses (here views of one buffer with another
+    hidden size, as a later model can be allocated where a freed one was)
+    get their own preparation, and the results match float64 products."""
+    print("[case] train_filter weights at reused addresses", flush=True)
+    g = torch.Generator().manual_seed(11)
+    buf1 = torch.randn(1 << 16, generator=g).to(gpu) * 0.1
+    buf2 = torch.randn(1 << 17, generator=g).to(gpu) * 0.05
+    e = torch.rand(300, 16, generator=g).to(gpu)
+    for H in (32, 128):
+        p = {"f.0.weight": buf1[:H * 16].view(H, 16), "f.0.bias": torch.zeros(H, device=gpu),
+             "f.2.weight": buf2[:3 * H * H].view(3 * H, H),
+             "f.2.bias": torch.zeros(3 * H, device=gpu)}
+        a1, h, W = train._filter(p, "f", e)
+        torch.cuda.synchronize()
+        d = {k: v.double().cpu() for k, v in p.items()}
+        ra1 = e.double().cpu() @ d["f.0.weight"].T
+        rW = torch.nn.functional.silu(ra1) @ d["f.2.weight"].T
+        assert W.shape == (300, 3 * H)
+        err = (W.double().cpu() - rW).abs().max().item()
+        assert err <= 2e-6 * max(rW.abs().max().item(), 1.0), (H, err)
