@@ -48,20 +48,6 @@ def default_candidates(cus: Optional[int] = None) -> Iterable[int]:
     return [k * cus for k in (2, 3, 4, 6, 8, 12, 16)]
 
 
-def time_stage(plan, stage: int, layer: int = 0, reps: int = 10) -> float:
-    lib = C.lib()
-    s = torch.cuda.current_stream()
-    C.check("stage", lib.mlff_launch_stage(plan.handle, s.cuda_stream, stage, layer))
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record(s)
-    for _ in range(reps):
-        C.check("stage", lib.mlff_launch_stage(plan.handle, s.cuda_stream, stage, layer))
-    b.record(s)
-    b.synchronize()
-    return a.elapsed_time(b) / 1e3 / reps
-
-
 def time_evaluation(plan, positions: torch.Tensor, e: torch.Tensor, f: torch.Tensor,
                     reps: int = 10) -> float:
     """Seconds per full energy + forces evaluation on the current stream."""
