@@ -53,6 +53,7 @@ enum mlff_count_slot {
   MLFF_COUNT_OVERFLOW = 2,    /* sticky: non-zero when a capacity was exceeded */
   MLFF_COUNT_CELL_SMALL = 3,  /* sticky: periodic cell narrower than 2*cutoff */
   MLFF_COUNT_STEP = 4,        /* MD step counter */
+  MLFF_COUNT_EDGES_MAX = 5,   /* sticky: largest edge count any neighbour list found */
   MLFF_COUNT_SLOTS = 8
 };
 
@@ -152,7 +153,7 @@ typedef struct mlff_config {
   int32_t edge_capacity;      /* directed edges */
   int32_t pair_capacity;      /* unordered pairs */
   int32_t any_periodic;       /* non-zero when some structure is periodic */
-  int32_t persistent_blocks;  /* 0 = library default (6 workgroups per CU) */
+  int32_t persistent_blocks;  /* 0 = library default (32 workgroups per CU) */
   int32_t cell_capacity;      /* cells reserved for cell-list neighbour search (0 = none) */
 } mlff_config;
 

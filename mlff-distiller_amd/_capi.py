@@ -23,6 +23,7 @@ MLFF_COUNT_PAIRS = 1
 MLFF_COUNT_OVERFLOW = 2
 MLFF_COUNT_CELL_SMALL = 3
 MLFF_COUNT_STEP = 4
+MLFF_COUNT_EDGES_MAX = 5
 MLFF_COUNT_SLOTS = 8
 
 T_EMBEDDING, T_RBF_CENTERS, T_RBF_WIDTHS = 0, 1, 2

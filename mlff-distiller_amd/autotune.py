@@ -45,7 +45,7 @@ def _key(plan) -> str:
 def default_candidates(cus: Optional[int] = None) -> Iterable[int]:
     if cus is None:
         cus = torch.cuda.get_device_properties(0).multi_processor_count
-    return [k * cus for k in (2, 3, 4, 6, 8, 12, 16)]
+    return [k * cus for k in (2, 3, 4, 6, 8, 12, 16, 24, 32)]
 
 
 def time_evaluation(plan, positions: torch.Tensor, e: torch.Tensor, f: torch.Tensor,

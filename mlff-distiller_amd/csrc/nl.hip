@@ -446,6 +446,8 @@ __global__ __launch_bounds__(kScanThreads) void nl_scan_offset_kernel(
     pair_ptr[N] = all2;
     counts[MLFF_COUNT_EDGES] = all1;
     counts[MLFF_COUNT_PAIRS] = all2;
+    // the largest count of a run of lists (an MD chunk), for the regrowth
+    if (all1 > counts[MLFF_COUNT_EDGES_MAX]) counts[MLFF_COUNT_EDGES_MAX] = all1;
     if (all1 > ecap || all2 > pcap) counts[MLFF_COUNT_OVERFLOW] = 1;
   }
 }

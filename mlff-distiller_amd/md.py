@@ -172,7 +172,9 @@ class BatchedNVE:
     def _check_overflow(self):
         c = self.plan.read_counts()
         if c[C.MLFF_COUNT_OVERFLOW]:
-            raise OverflowError(int(c[C.MLFF_COUNT_EDGES]))
+            # the largest count of the chunk: the overflow may have happened at an
+            # earlier step than the last, whose count alone is in MLFF_COUNT_EDGES
+            raise OverflowError(int(max(c[C.MLFF_COUNT_EDGES], c[C.MLFF_COUNT_EDGES_MAX])))
 
     def run(self, n_steps: int, graph: bool = True, steps_per_graph: int = 10):
         """Advance n_steps; on a neighbour-capacity overflow the chunk is

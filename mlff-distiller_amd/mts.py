@@ -166,6 +166,7 @@ class MTSNVE:
         self.outer_done = 0
         self.fallback_steps = 0
         self.replays = 0
+        self.regrowths = 0
         row_ptr, col, _ = engine.neighbor_list(self.pos32, self.layout, full_model.cutoff)
         self._make_plans(max(int(col.numel()), 16) * headroom)
         self._on_stream(lambda: C.check("mlff_mts_init_forces", self.lib.mlff_mts_init_forces(
@@ -203,7 +204,8 @@ class MTSNVE:
     def _check_overflow(self):
         c = self.plan_fast.read_counts()
         if c[C.MLFF_COUNT_OVERFLOW]:
-            raise OverflowError(int(c[C.MLFF_COUNT_EDGES]))
+            # the largest count of the chunk (see BatchedNVE._check_overflow)
+            raise OverflowError(int(max(c[C.MLFF_COUNT_EDGES], c[C.MLFF_COUNT_EDGES_MAX])))
 
     def _state(self):
         return [self.x, self.p, self.pos32, self.F_full, self.F_fast, self.F_mix, self.E_full,
@@ -232,7 +234,12 @@ class MTSNVE:
             except OverflowError as exc:
                 for dst, src in zip(self._state(), saved):
                     dst.copy_(src)
-                self._make_plans(int(exc.args[0]) * max(self.headroom, 1.1), step)
+                need = int(exc.args[0])
+                self.regrowths += 1
+                if self.regrowths > 8 or need <= self.plan_fast.ecap:
+                    raise RuntimeError(f"neighbour capacity did not converge: {need} edges "
+                                       f"needed, capacity {self.plan_fast.ecap}") from exc
+                self._make_plans(need * max(self.headroom, 1.1), step)
                 continue
             exceed = self.dis_max > self.threshold
             if not fallback and bool(exceed.any()):

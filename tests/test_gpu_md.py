@@ -152,3 +152,33 @@ def test_frame_ring_drains_every_step(gpu):
         assert np.array_equal(u, w)
     with pytest.raises(ValueError):
         md_mod.BatchedNVE(model, mols, seed=3, max_steps=5, record_frames=True, frame_window=1)
+
+
+def test_edge_count_maximum_is_kept_across_lists(gpu):
+    """MLFF_COUNT_EDGES holds the last neighbour list's count and
+    MLFF_COUNT_EDGES_MAX the largest since the plan was bound: an MD chunk
+    whose overflow happened before its last step regrows from the peak."""
+    note("sticky maximum edge count")
+    synth = pkg_mod("synth")
+    engine = pkg_mod("engine")
+    C = pkg_mod("_capi")
+    model = _model(32, 2, 16)
+    mols = synth.molecule_set(4, 30, 60, seed=9)
+    Z, P, mp = synth.flatten(mols)
+    sizes = np.diff(mp)
+    layout = engine.make_layout(sizes, gpu)
+    numbers = torch.tensor(Z, dtype=torch.int64, device=gpu)
+    plan = engine.Plan(32, 2, 16, model.max_z, model.cutoff, layout, numbers,
+                       model.packed_weights(gpu), engine.edge_capacity_bound(sizes, False))
+    e = torch.empty(len(sizes), dtype=torch.float64, device=gpu)
+    f = torch.empty(len(Z), 3, dtype=torch.float32, device=gpu)
+    counts = []
+    for scale in (0.6, 1.0):
+        plan.energy_forces(torch.tensor(P * scale, dtype=torch.float32, device=gpu), e, f)
+        c = plan.read_counts()
+        counts.append(int(c[C.MLFF_COUNT_EDGES]))
+        assert c[C.MLFF_COUNT_OVERFLOW] == 0
+    assert counts[0] > counts[1]
+    c = plan.read_counts()
+    assert int(c[C.MLFF_COUNT_EDGES]) == counts[1]
+    assert int(c[C.MLFF_COUNT_EDGES_MAX]) == counts[0]
