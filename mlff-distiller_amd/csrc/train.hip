@@ -514,16 +514,27 @@ __global__ __launch_bounds__(64 * kWgWaves) void weight_grad_kernel(
   }
 }
 
-// out[i] = sum_s part[s][i] over S partials of n4 float4 elements.  A block
-// takes kWgRedCols elements; its G = 256 / kWgRedCols thread groups add the
-// partials s = q, q + G, ... in order, then the G group sums are added in
-// group order (a fixed partition and order).
+// out[i] = sum_s part[s][i] over S partials of n4 float4 elements, and the
+// same for the bias partials (nb4 elements; the blocks after the weight's
+// take them, so one launch reduces both).  A block takes kWgRedCols elements;
+// its G = 256 / kWgRedCols thread groups add the partials s = q, q + G, ...
+// in order, then the G group sums are added in group order (a fixed partition
+// and order).
 __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_kernel(
-    int S, int n4, const f4* __restrict__ part, f4* __restrict__ out) {
+    int S, int n4, const f4* __restrict__ part, f4* __restrict__ out, int nb4,
+    const f4* __restrict__ part_b, f4* __restrict__ out_b) {
   constexpr int G = kTrainBlock / kWgRedCols;
   __shared__ f4 grp[kTrainBlock];
   const int col = threadIdx.x % kWgRedCols, q = threadIdx.x / kWgRedCols;
-  const int i = blockIdx.x * kWgRedCols + col;
+  const int blocks0 = (n4 + kWgRedCols - 1) / kWgRedCols;
+  int blk = blockIdx.x;
+  if (blk >= blocks0) {   // the bias partials (uniform per block)
+    blk -= blocks0;
+    part = part_b;
+    out = out_b;
+    n4 = nb4;
+  }
+  const int i = blk * kWgRedCols + col;
   f4 sum = {0.f, 0.f, 0.f, 0.f};
   if (i < n4) {
 #pragma unroll 4
@@ -967,12 +978,11 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
     launch_weight_grad<false, false>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S,
                                      a0_dev, lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1,
                                      row_w_dev, part, part_bias);
-  const int n4 = M * N / 4;
-  weight_grad_reduce_kernel<<<(n4 + kWgRedCols - 1) / kWgRedCols, kTrainBlock, 0, st>>>(
-      S, n4, reinterpret_cast<const f4*>(part), reinterpret_cast<f4*>(out_dev));
-  if (bias)
-    weight_grad_reduce_kernel<<<(M / 4 + kWgRedCols - 1) / kWgRedCols, kTrainBlock, 0, st>>>(
-        S, M / 4, reinterpret_cast<const f4*>(part_bias), reinterpret_cast<f4*>(bias_out_dev));
+  const int n4 = M * N / 4, nb4 = bias ? M / 4 : 0;
+  const int red_blocks = (n4 + kWgRedCols - 1) / kWgRedCols + (nb4 + kWgRedCols - 1) / kWgRedCols;
+  weight_grad_reduce_kernel<<<red_blocks, kTrainBlock, 0, st>>>(
+      S, n4, reinterpret_cast<const f4*>(part), reinterpret_cast<f4*>(out_dev), nb4,
+      reinterpret_cast<const f4*>(part_bias), reinterpret_cast<f4*>(bias_out_dev));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("mlff_weight_grad: %s", hipGetErrorString(e));

@@ -256,7 +256,8 @@ def _filter_weights(p: Dict[str, Tensor], pre: str):
     key = (id(W1o), id(W2o))
     state = (W1o.data_ptr(), W2o.data_ptr(), W1o._version, W2o._version)
     ent = _FILTER_W.get(key)
-    if ent is not None and ent[0]() is W1o and ent[1]() is W2o and ent[2] == state:
+    same = ent is not None and ent[0]() is W1o and ent[1]() is W2o
+    if same and ent[2] == state:
         return ent[3], ent[4]
     W1, W2 = _dense(W1o), _dense(W2o)
     H, R = W1.shape
@@ -264,9 +265,14 @@ def _filter_weights(p: Dict[str, Tensor], pre: str):
     if len(_FILTER_W) > 64:
         for k in [k for k, e in _FILTER_W.items() if e[0]() is None or e[1]() is None]:
             del _FILTER_W[k]
-    w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
+    if same and ent[3].shape == (H, RP) and ent[4].numel() == 3 * W2.numel():
+        # the same tensors, written by the optimizer: refill the buffers in place
+        # (W1's padding columns stay zero)
+        w1p, planes = ent[3], ent[4]
+    else:
+        w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
+        planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
     w1p[:, :R].copy_(W1)
-    planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
     C.check("mlff_split_bf16x3", C.lib().mlff_split_bf16x3(
         C.stream_handle(), W2.numel(), C.ptr(W2), C.ptr(planes)))
     _FILTER_W[key] = (weakref.ref(W1o), weakref.ref(W2o), state, w1p, planes)
@@ -692,12 +698,22 @@ def force_matching_grads(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGrap
     gE, tg = reverse(p, cache, g, torch.ones(B, dtype=torch.float32, device=x.device),
                      lam=dLdF, gw=dLdE, rbf_grads=rbf_grads)
     grads = {}
+    both, A, B = [], [], []
     for name in p:
         a = gE.get(name)
         b = tg.get(name)
         if a is None and b is None:
             continue
-        grads[name] = (a if a is not None else 0) - (b if b is not None else 0)
+        if a is not None and b is not None:
+            both.append(name)
+            A.append(a)
+            B.append(b)
+        else:
+            grads[name] = a if b is None else torch.neg(b)
+    # one multi-tensor launch for every parameter with both parts
+    if both:
+        for name, d in zip(both, torch._foreach_sub(A, B)):
+            grads[name] = d
     return loss, E, F, grads
 
 
