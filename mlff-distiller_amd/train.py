@@ -69,30 +69,52 @@ class PairGraph:
 
 
 def pair_graph_from_csr(row_ptr: Tensor, col: Tensor, batch: Tensor, n_structures: int,
-                        shift_vec: Optional[Tensor] = None) -> PairGraph:
+                        shift_vec: Optional[Tensor] = None, n_pairs: Optional[int] = None,
+                        mol_ptr: Optional[Tensor] = None) -> PairGraph:
     """Pairs (row < col) of a symmetric receiver CSR with rows in ascending
     column order, and the pair of every directed edge: an upper edge
     (row < col) numbers its pair in CSR order; a lower edge takes the
     number of its reverse edge, found by a binary search of the sorted
-    (row, col) keys."""
+    (row, col) keys.  With the pair count (``n_pairs``, known on the host
+    when no edge is a self-image) and the structure offsets (``mol_ptr``)
+    given, nothing here waits for the device."""
     dev = col.device
     n = row_ptr.numel() - 1
+    E = col.numel()
     deg = (row_ptr[1:] - row_ptr[:-1]).long()
-    row = torch.repeat_interleave(torch.arange(n, device=dev), deg)
+    row = torch.repeat_interleave(torch.arange(n, device=dev), deg, output_size=E)
     colL = col.long()
     up = colL > row
     up_id = torch.cumsum(up.to(torch.int64), 0) - 1
     key = row * n + colL
     rev = torch.searchsorted(key, colL * n + row)
     edge_pair = torch.where(up, up_id, up_id[rev]).to(torch.int32)
-    lo, hi = row[up], colL[up]
-    sizes = torch.bincount(batch, minlength=n_structures)
-    mol_ptr = torch.zeros(n_structures + 1, dtype=torch.int32, device=dev)
-    mol_ptr[1:] = torch.cumsum(sizes, 0).to(torch.int32)
-    sv = None if shift_vec is None else shift_vec[up].contiguous()
+    if n_pairs is None:
+        lo, hi = row[up], colL[up]
+        sv = None if shift_vec is None else shift_vec[up].contiguous()
+    else:
+        # every upper edge to its pair slot; the other edges to a spare slot
+        dst = torch.where(up, up_id, torch.full_like(up_id, n_pairs))
+        pick = lambda v: torch.empty(n_pairs + 1, dtype=v.dtype, device=dev).scatter_(
+            0, dst, v)[:n_pairs]
+        lo, hi = pick(row), pick(colL)
+        sv = None
+        if shift_vec is not None:
+            sv = torch.empty(n_pairs + 1, 3, dtype=shift_vec.dtype, device=dev).index_copy_(
+                0, dst, shift_vec)[:n_pairs].contiguous()
+    if mol_ptr is None:
+        sizes = torch.bincount(batch, minlength=n_structures)
+        mol_ptr = torch.zeros(n_structures + 1, dtype=torch.int32, device=dev)
+        mol_ptr[1:] = torch.cumsum(sizes, 0).to(torch.int32)
     return PairGraph(row_ptr.to(torch.int32).contiguous(), col.to(torch.int32).contiguous(),
                      edge_pair.contiguous(), lo, hi, lo.to(torch.int32).contiguous(),
-                     hi.to(torch.int32).contiguous(), sv, batch.long(), mol_ptr, n_structures)
+                     hi.to(torch.int32).contiguous(), sv, batch.long(),
+                     mol_ptr.to(torch.int32).contiguous(), n_structures)
+
+
+# device layouts of non-periodic batches by (sizes, device): the training loop
+# builds a pair graph per step, usually of the same batch shape
+_GRAPH_LAYOUTS: "Dict[tuple, object]" = {}
 
 
 def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=None,
@@ -100,8 +122,16 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
     """Pair graph of a structure-sorted batch from the HIP neighbour list."""
     from . import engine
     periodic = pbc is not None and bool(np.asarray(pbc).any())
-    layout = engine.make_layout(sizes, x.device, cells if periodic else None,
-                                pbc if periodic else None)
+    if periodic:
+        layout = engine.make_layout(sizes, x.device, cells, pbc)
+    else:
+        key = (tuple(int(v) for v in sizes), str(x.device))
+        layout = _GRAPH_LAYOUTS.get(key)
+        if layout is None:
+            if len(_GRAPH_LAYOUTS) > 16:
+                _GRAPH_LAYOUTS.clear()
+            layout = engine.make_layout(sizes, x.device)
+            _GRAPH_LAYOUTS[key] = layout
     pos32 = x.detach().to(torch.float32).contiguous()
     row_ptr, col, shift = engine.neighbor_list(pos32, layout, rc, with_shifts=periodic)
     batch = layout.atom_mol.long()
@@ -111,12 +141,19 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
                                  device=x.device)
         n = row_ptr.numel() - 1
         deg = (row_ptr[1:] - row_ptr[:-1]).long()
-        row = torch.repeat_interleave(torch.arange(n, device=x.device), deg)
+        row = torch.repeat_interleave(torch.arange(n, device=x.device), deg,
+                                      output_size=col.numel())
         cells_e = cell_t[batch[row]]
         sh = shift.to(torch.float32)
         shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
                      sh[:, 2:3] * cells_e[:, 2])
-    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec)
+        # a periodic list may hold self-image edges (no upper partner): count on the device
+        return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec,
+                                   mol_ptr=layout.mol_ptr)
+    # molecules: every edge has its reverse and none is a self-edge, so the pair
+    # count is half the edge count (known here without a device read)
+    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), None,
+                               n_pairs=col.numel() // 2, mol_ptr=layout.mol_ptr)
 
 
 # ---------------------------------------------------------------------------

@@ -195,3 +195,32 @@ def test_trainer_parameter_views_follow_the_model():
     p2 = tr._params()
     assert p2 is not p1
     assert p2["energy_head.0.weight"].data_ptr() == m.energy_head[0].weight.data_ptr()
+
+
+def test_pair_graph_without_device_reads_equals_counting_form():
+    """pair_graph_from_csr given the pair count and the structure offsets
+    (the molecules' form, no host read of device values) builds the same
+    pairs, edge-to-pair map and offsets as the form that counts them, on
+    random symmetric neighbour lists of two structures."""
+    train = pkg_mod("train")
+    g = torch.Generator().manual_seed(0)
+    for trial in range(30):
+        na, nb = (int(v) for v in torch.randint(1, 25, (2,), generator=g))
+        n = na + nb
+        A = torch.zeros(n, n, dtype=torch.bool)
+        for lo_, hi_ in ((0, na), (na, n)):
+            blk = torch.rand(hi_ - lo_, hi_ - lo_, generator=g) < 0.4
+            A[lo_:hi_, lo_:hi_] = blk | blk.T
+        A.fill_diagonal_(False)
+        rows, cols = torch.nonzero(A, as_tuple=True)
+        row_ptr = torch.zeros(n + 1, dtype=torch.int32)
+        row_ptr[1:] = torch.cumsum(torch.bincount(rows, minlength=n), 0)
+        col = cols.to(torch.int32)
+        batch = torch.cat([torch.zeros(na), torch.ones(nb)]).long()
+        a = train.pair_graph_from_csr(row_ptr, col, batch, 2)
+        b = train.pair_graph_from_csr(row_ptr, col, batch, 2, n_pairs=col.numel() // 2,
+                                      mol_ptr=torch.tensor([0, na, n], dtype=torch.int32))
+        for f in ("row_ptr", "col", "edge_pair", "lo", "hi", "lo32", "hi32", "mol_ptr"):
+            assert torch.equal(getattr(a, f), getattr(b, f)), (trial, f)
+        # every upper edge (row < col) is its own pair's slot; every pair is (lo < hi)
+        assert bool((b.lo < b.hi).all())
