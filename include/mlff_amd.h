@@ -434,6 +434,27 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
                      const float* b1_dev, int ldb1, const float* row_w_dev, void* ws_dev,
                      size_t ws_bytes, float* out_dev, float* bias_out_dev);
 
+/* The same weight gradients in two steps, so that the partial sums of many
+ * gradients (one training pass's) are added by one launch.  The products the
+ * reference's backward computes are the same as for mlff_weight_grad.
+ * mlff_weight_grad_partials takes mlff_weight_grad's inputs and workspace and
+ * writes only the partial sums: S = mlff_weight_grad_partial_count(K, M, N)
+ * weight partials [S][M][N] at the start of ws_dev, then, with with_bias,
+ * S bias partials [S][M].  mlff_weight_grad_reduce_group then adds partials
+ * for n_rows outputs (at most 256): row r of table_dev (int64, 16-byte
+ * aligned, on the device) is {S, n4, partials address, output address}, n4
+ * being the output's float count / 4; partials and outputs 16-byte aligned.
+ * Any n_blocks >= 1 covers every row; the sum of ceil(n4 / 8) over the rows
+ * runs each block once.  Every element is added in the partition and order of
+ * mlff_weight_grad, so the outputs are bit-identical to it. */
+int mlff_weight_grad_partial_count(int K, int M, int N);
+int mlff_weight_grad_partials(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
+                              const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
+                              const float* b1_dev, int ldb1, const float* row_w_dev,
+                              int with_bias, void* ws_dev, size_t ws_bytes);
+int mlff_weight_grad_reduce_group(void* stream, int n_rows, const int64_t* table_dev,
+                                  int n_blocks);
+
 /* SiLU of the MLPs (student_model.py:318-322, 407-411, 599-605) and its
  * derivatives, elementwise over n float32 values (n a multiple of 4, buffers
  * 16-byte aligned), each output skipped when its pointer is NULL:

@@ -369,8 +369,10 @@ __global__ __launch_bounds__(kTrainBlock) void segment_sum_kernel(
 // ---------------------------------------------------------------------------
 constexpr int kWgWaves = 4;
 constexpr int kWgMaxSplits = 256;
-// k-steps per operand batch (two terms: half as many, for two waves per SIMD)
-template <bool TWO> constexpr int kWgUnroll = TWO ? 2 : 4;
+// k-steps per operand batch (two terms: half as many).  With one batch in
+// flight beside the current one the kernel fits 168 VGPRs, three waves per
+// SIMD (three blocks per CU, also the LDS limit: 52 KB per block).
+template <bool TWO> constexpr int kWgUnroll = TWO ? 1 : 2;
 constexpr int kWgRedCols = 8;  // float4 columns per block of the partial reduction
 
 __device__ __forceinline__ void wg_mfma(f4 (&acc)[4][4], const f4& a, const f4& b) {
@@ -413,7 +415,7 @@ struct WgBatch {
 };
 
 template <bool TWO, bool ROWW, bool BIAS>
-__global__ __launch_bounds__(64 * kWgWaves) void weight_grad_kernel(
+__global__ __launch_bounds__(64 * kWgWaves, 3) void weight_grad_kernel(
     int K, int M, int N, int RW, int tiles_m, int tiles_n, int S,
     const float* __restrict__ a0, int lda0, const float* __restrict__ b0, int ldb0,
     const float* __restrict__ a1, int lda1, const float* __restrict__ b1, int ldb1,
@@ -549,11 +551,68 @@ __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_kernel(
   }
 }
 
+// The same sums for several weight gradients in one launch.  Row r of the
+// table is {S, n4, partials address, output address} (int64 each); row r takes
+// ceil(n4 / kWgRedCols) consecutive virtual blocks, and the grid walks the
+// virtual blocks with a stride of its size, so any grid covers every row.
+// Each element is added exactly as weight_grad_reduce_kernel adds it (same
+// partition and order), so the results are bit-identical to it.
+constexpr int kWgGroupMaxRows = 256;
+struct WgRedRow {
+  int64_t S, n4, part, out;
+};
+
+__global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_group_kernel(
+    int n_rows, const WgRedRow* __restrict__ table) {
+  constexpr int G = kTrainBlock / kWgRedCols;
+  __shared__ WgRedRow rows[kWgGroupMaxRows];
+  __shared__ int64_t first[kWgGroupMaxRows + 1];   // first virtual block of each row
+  __shared__ f4 grp[kTrainBlock];
+  for (int r = threadIdx.x; r < n_rows; r += kTrainBlock) rows[r] = table[r];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t v = 0;
+    for (int r = 0; r < n_rows; ++r) {
+      first[r] = v;
+      v += (rows[r].n4 + kWgRedCols - 1) / kWgRedCols;
+    }
+    first[n_rows] = v;
+  }
+  __syncthreads();
+  const int col = threadIdx.x % kWgRedCols, q = threadIdx.x / kWgRedCols;
+  for (int64_t vb = blockIdx.x; vb < first[n_rows]; vb += gridDim.x) {
+    int lo = 0, hi = n_rows - 1;   // the last row whose first block is <= vb
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= vb) lo = mid; else hi = mid - 1;
+    }
+    const WgRedRow& row = rows[lo];
+    const f4* part = reinterpret_cast<const f4*>(row.part);
+    f4* out = reinterpret_cast<f4*>(row.out);
+    const int64_t n4 = row.n4;
+    const int S = (int)row.S;
+    const int64_t i = (vb - first[lo]) * kWgRedCols + col;
+    f4 sum = {0.f, 0.f, 0.f, 0.f};
+    if (i < n4) {
+#pragma unroll 4
+      for (int s = q; s < S; s += G) sum += part[(size_t)s * n4 + i];
+    }
+    grp[threadIdx.x] = sum;
+    __syncthreads();
+    if (q == 0 && i < n4) {
+      f4 tot = grp[col];
+      for (int k = 1; k < G; ++k) tot += grp[k * kWgRedCols + col];
+      out[i] = tot;
+    }
+    __syncthreads();   // grp is reused by the next virtual block
+  }
+}
+
 // Number of partials S and rows per wave RW (a multiple of 4: whole k-steps):
-// at most kWgSlots blocks, the ones resident at once on MI355X (256 CUs, two
+// at most kWgSlots blocks, the ones resident at once on MI355X (256 CUs, three
 // blocks of this kernel each), so the grid runs as a single round instead of
 // leaving a few blocks for a second one; at most kWgMaxSplits partials.
-constexpr long kWgSlots = 512;
+constexpr long kWgSlots = 768;
 void weight_grad_split(int K, int M, int N, int& RW, int& S) {
   const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
   long s = kWgSlots / tiles;
@@ -923,34 +982,36 @@ size_t mlff_weight_grad_workspace_bytes(int K, int M, int N) {
   return (size_t)S * ((size_t)M * N + M) * sizeof(float);
 }
 
-int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
-                     const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
-                     const float* b1_dev, int ldb1, const float* row_w_dev, void* ws_dev,
-                     size_t ws_bytes, float* out_dev, float* bias_out_dev) {
+// Checks the arguments and launches the partial-sum kernel (weight partials
+// [S][M][N] at ws, bias partials [S][M] after them); returns S through s_out.
+static int weight_grad_partials(const char* who, void* stream, int K, int M, int N,
+                                const float* a0_dev, int lda0, const float* b0_dev, int ldb0,
+                                const float* a1_dev, int lda1, const float* b1_dev, int ldb1,
+                                const float* row_w_dev, bool bias, void* ws_dev,
+                                size_t ws_bytes, int* s_out) {
   if (K < 0 || M < 4 || N < 4 || M % 4 || N % 4) {
-    set_error("mlff_weight_grad: need K >= 0 and M, N positive multiples of 4 (K=%d M=%d N=%d)",
-              K, M, N);
+    set_error("%s: need K >= 0 and M, N positive multiples of 4 (K=%d M=%d N=%d)", who, K, M,
+              N);
     return MLFF_ERR_INVALID;
   }
   const bool two = a1_dev != nullptr;
-  if ((K > 0 && (!a0_dev || !b0_dev)) || !out_dev || !ws_dev || two != (b1_dev != nullptr)) {
-    set_error("mlff_weight_grad: need a0, b0 (K > 0), out, workspace, and a1 and b1 together");
+  if ((K > 0 && (!a0_dev || !b0_dev)) || !ws_dev || two != (b1_dev != nullptr)) {
+    set_error("%s: need a0, b0 (K > 0), workspace, and a1 and b1 together", who);
     return MLFF_ERR_INVALID;
   }
   if (lda0 < M || ldb0 < N || lda0 % 4 || ldb0 % 4 ||
       (two && (lda1 < M || ldb1 < N || lda1 % 4 || ldb1 % 4))) {
-    set_error("mlff_weight_grad: row strides must be multiples of 4 and at least M, N");
+    set_error("%s: row strides must be multiples of 4 and at least M, N", who);
     return MLFF_ERR_INVALID;
   }
-  if (!aligned16(a0_dev) || !aligned16(b0_dev) || !aligned16(out_dev) || !aligned16(ws_dev) ||
-      (two && (!aligned16(a1_dev) || !aligned16(b1_dev))) ||
-      (bias_out_dev && !aligned16(bias_out_dev))) {
-    set_error("mlff_weight_grad: buffers must be 16-byte aligned");
+  if (!aligned16(a0_dev) || !aligned16(b0_dev) || !aligned16(ws_dev) ||
+      (two && (!aligned16(a1_dev) || !aligned16(b1_dev)))) {
+    set_error("%s: buffers must be 16-byte aligned", who);
     return MLFF_ERR_INVALID;
   }
   const size_t need = mlff_weight_grad_workspace_bytes(K, M, N);
   if (ws_bytes < need) {
-    set_error("mlff_weight_grad: workspace of %zu bytes, need %zu", ws_bytes, need);
+    set_error("%s: workspace of %zu bytes, need %zu", who, ws_bytes, need);
     return MLFF_ERR_INVALID;
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -961,7 +1022,6 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
   const int blocks = (tiles_m * tiles_n * S + 7) / 8 * 8;
   float* part = static_cast<float*>(ws_dev);
   float* part_bias = part + (size_t)S * M * N;
-  const bool bias = bias_out_dev != nullptr;
   if (two && row_w_dev)
     launch_weight_grad<true, true>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S, a0_dev,
                                    lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev,
@@ -978,14 +1038,77 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
     launch_weight_grad<false, false>(st, bias, blocks, K, M, N, RW, tiles_m, tiles_n, S,
                                      a0_dev, lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1,
                                      row_w_dev, part, part_bias);
+  *s_out = S;
+  return MLFF_OK;
+}
+
+int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
+                     const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
+                     const float* b1_dev, int ldb1, const float* row_w_dev, void* ws_dev,
+                     size_t ws_bytes, float* out_dev, float* bias_out_dev) {
+  if (!out_dev || !aligned16(out_dev) || (bias_out_dev && !aligned16(bias_out_dev))) {
+    set_error("mlff_weight_grad: need out, and out and bias_out 16-byte aligned");
+    return MLFF_ERR_INVALID;
+  }
+  const bool bias = bias_out_dev != nullptr;
+  int S = 0;
+  const int rc = weight_grad_partials("mlff_weight_grad", stream, K, M, N, a0_dev, lda0, b0_dev,
+                                      ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev, bias, ws_dev,
+                                      ws_bytes, &S);
+  if (rc != MLFF_OK) return rc;
+  const float* part = static_cast<const float*>(ws_dev);
+  const float* part_bias = part + (size_t)S * M * N;
   const int n4 = M * N / 4, nb4 = bias ? M / 4 : 0;
   const int red_blocks = (n4 + kWgRedCols - 1) / kWgRedCols + (nb4 + kWgRedCols - 1) / kWgRedCols;
-  weight_grad_reduce_kernel<<<red_blocks, kTrainBlock, 0, st>>>(
+  weight_grad_reduce_kernel<<<red_blocks, kTrainBlock, 0, static_cast<hipStream_t>(stream)>>>(
       S, n4, reinterpret_cast<const f4*>(part), reinterpret_cast<f4*>(out_dev), nb4,
       reinterpret_cast<const f4*>(part_bias), reinterpret_cast<f4*>(bias_out_dev));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("mlff_weight_grad: %s", hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+int mlff_weight_grad_partial_count(int K, int M, int N) {
+  if (K < 0 || M < 1 || N < 1) return 0;
+  int RW, S;
+  weight_grad_split(K, M, N, RW, S);
+  return S;
+}
+
+int mlff_weight_grad_partials(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
+                              const float* b0_dev, int ldb0, const float* a1_dev, int lda1,
+                              const float* b1_dev, int ldb1, const float* row_w_dev,
+                              int with_bias, void* ws_dev, size_t ws_bytes) {
+  int S = 0;
+  const int rc = weight_grad_partials("mlff_weight_grad_partials", stream, K, M, N, a0_dev,
+                                      lda0, b0_dev, ldb0, a1_dev, lda1, b1_dev, ldb1, row_w_dev,
+                                      with_bias != 0, ws_dev, ws_bytes, &S);
+  if (rc != MLFF_OK) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("mlff_weight_grad_partials: %s", hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+int mlff_weight_grad_reduce_group(void* stream, int n_rows, const int64_t* table_dev,
+                                  int n_blocks) {
+  if (n_rows < 0 || n_rows > kWgGroupMaxRows || (n_rows > 0 && (!table_dev || n_blocks < 1)) ||
+      !aligned16(table_dev)) {
+    set_error("mlff_weight_grad_reduce_group: need 0 <= n_rows <= %d, a 16-byte aligned table "
+              "and n_blocks >= 1", kWgGroupMaxRows);
+    return MLFF_ERR_INVALID;
+  }
+  if (n_rows == 0) return MLFF_OK;
+  weight_grad_reduce_group_kernel<<<n_blocks, kTrainBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      n_rows, reinterpret_cast<const WgRedRow*>(table_dev));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("mlff_weight_grad_reduce_group: %s", hipGetErrorString(e));
     return MLFF_ERR_HIP;
   }
   return MLFF_OK;

@@ -386,15 +386,46 @@ def _wg_operand(t: Tensor) -> Tensor:
 
 
 _WG_WS: Dict[tuple, int] = {}   # workspace floats of mlff_weight_grad by (K, M, N)
+_WG_S: Dict[tuple, int] = {}    # partial sums of mlff_weight_grad by (K, M, N)
+
+
+class _WgGroup:
+    """Weight gradients whose split-K partial sums are added later, all in one
+    launch (mlff_weight_grad_reduce_group), instead of one launch each.  The
+    outputs _wgrad returns with a group hold their values only after
+    flush(); the partial buffers are kept alive until then."""
+
+    def __init__(self):
+        self.rows = []
+        self.keep = []
+
+    def add(self, S: int, n4: int, part: int, out: int, buf: Tensor):
+        self.rows.append((S, n4, part, out))
+        self.keep.append(buf)
+
+    def flush(self):
+        if not self.rows:
+            return
+        lib = C.lib()
+        dev = self.keep[0].device
+        for i in range(0, len(self.rows), 256):   # at most 256 rows per launch
+            rows = self.rows[i:i + 256]
+            table = torch.tensor(rows, dtype=torch.int64, pin_memory=True).to(
+                dev, non_blocking=True)
+            n_blocks = sum((n4 + 7) // 8 for _, n4, _, _ in rows)
+            C.check("mlff_weight_grad_reduce_group", lib.mlff_weight_grad_reduce_group(
+                C.stream_handle(), len(rows), table.data_ptr(), min(n_blocks, 1 << 16)))
+        self.rows, self.keep = [], []
 
 
 def _wgrad(a0: Tensor, b0: Tensor, a1: Optional[Tensor] = None, b1: Optional[Tensor] = None,
-           w: Optional[Tensor] = None, bias: bool = False):
+           w: Optional[Tensor] = None, bias: bool = False, group: Optional[_WgGroup] = None):
     """sum_k w_k (a0[k] (x) b0[k] + a1[k] (x) b1[k]) [M, N] and, with
     ``bias``, sum_k w_k a0[k] [M] over the K rows (pairs or atoms): a Linear
     layer's weight and bias gradients, on the MFMA kernel with a fixed split
     and order of the row sum (mlff_weight_grad).  Returns (weight, bias or
-    None)."""
+    None).  With ``group`` only the partial sums are launched here; the
+    outputs are filled, with the same bits, by group.flush()."""
     M, N = a0.shape[1], b0.shape[1]
     K = a0.shape[0]
     A0, B0 = _wg_operand(a0), _wg_operand(b0)
@@ -415,11 +446,21 @@ def _wgrad(a0: Tensor, b0: Tensor, a1: Optional[Tensor] = None, b1: Optional[Ten
     wv = None
     if w is not None:
         wv = w if (w.dtype == torch.float32 and w.is_contiguous()) else w.to(torch.float32).contiguous()
-    C.check("mlff_weight_grad", lib.mlff_weight_grad(
-        C.stream_handle(), K, M4, N4, A0.data_ptr(), A0.stride(0), B0.data_ptr(), B0.stride(0),
-        C.ptr(A1), A1.stride(0) if A1 is not None else 0,
-        C.ptr(B1), B1.stride(0) if B1 is not None else 0,
-        C.ptr(wv), buf.data_ptr(), ws_floats * 4, out.data_ptr(), C.ptr(bout)))
+    operands = (C.stream_handle(), K, M4, N4, A0.data_ptr(), A0.stride(0), B0.data_ptr(),
+                B0.stride(0), C.ptr(A1), A1.stride(0) if A1 is not None else 0,
+                C.ptr(B1), B1.stride(0) if B1 is not None else 0, C.ptr(wv))
+    if group is None:
+        C.check("mlff_weight_grad", lib.mlff_weight_grad(
+            *operands, buf.data_ptr(), ws_floats * 4, out.data_ptr(), C.ptr(bout)))
+    else:
+        C.check("mlff_weight_grad_partials", lib.mlff_weight_grad_partials(
+            *operands, int(bias), buf.data_ptr(), ws_floats * 4))
+        S = _WG_S.get(key)
+        if S is None:
+            S = _WG_S[key] = int(lib.mlff_weight_grad_partial_count(K, M4, N4))
+        group.add(S, M4 * N4 // 4, buf.data_ptr(), out.data_ptr(), buf)
+        if bout is not None:
+            group.add(S, M4 // 4, buf.data_ptr() + S * M4 * N4 * 4, bout.data_ptr(), buf)
     if M4 != M or N4 != N:
         out = out[:M, :N]
     if bout is not None and M4 != M:
@@ -565,8 +606,15 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     tg: Dict[str, Tensor] = {}
     dev = c["d"].device
 
+    # the weight gradients' partial sums are added by one launch at the end
+    grp = _WgGroup()
+
     def acc(store, name, val):
-        store[name] = store[name] + val if name in store else val
+        if name in store:
+            grp.flush()   # both terms must hold their values
+            store[name] = store[name] + val
+        else:
+            store[name] = val
 
     gE = gE.to(torch.float32)
     w_atom = w_pair = None
@@ -588,12 +636,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     acc(grads, "energy_head.4.bias", wgeps.sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
     gr2 = _silu_d(r2, gh4)
-    gw_, gb_ = _wgrad(gr2, h3, w=w_atom, bias=True)
+    gw_, gb_ = _wgrad(gr2, h3, w=w_atom, bias=True, group=grp)
     acc(grads, "energy_head.2.weight", gw_)
     acc(grads, "energy_head.2.bias", gb_)
     gh3 = gr2 @ p["energy_head.2.weight"]
     gr1 = _silu_d(r1, gh3)
-    gw_, gb_ = _wgrad(gr1, sL, w=w_atom, bias=True)
+    gw_, gb_ = _wgrad(gr1, sL, w=w_atom, bias=True, group=grp)
     acc(grads, "energy_head.0.weight", gw_)
     acc(grads, "energy_head.0.bias", gb_)
     gs = (gr1 @ p["energy_head.0.weight"]).contiguous()
@@ -603,12 +651,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         acc(tg, "energy_head.4.weight", (geps @ dh4).reshape(1, -1))
         acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=torch.float32, device=dev))
         dgr2 = _silu_dd(r2, gh4, dr2)
-        gw_, gb_ = _wgrad(dgr2, h3, gr2, dh3, bias=True)
+        gw_, gb_ = _wgrad(dgr2, h3, gr2, dh3, bias=True, group=grp)
         acc(tg, "energy_head.2.weight", gw_)
         acc(tg, "energy_head.2.bias", gb_)
         dgh3 = dgr2 @ p["energy_head.2.weight"]
         dgr1 = _silu_dd(r1, gh3, dr1, dgh3)
-        gw_, gb_ = _wgrad(dgr1, sL, gr1, c["dsL"], bias=True)
+        gw_, gb_ = _wgrad(dgr1, sL, gr1, c["dsL"], bias=True, group=grp)
         acc(tg, "energy_head.0.weight", gw_)
         acc(tg, "energy_head.0.bias", gb_)
         dgs = (dgr1 @ p["energy_head.0.weight"]).contiguous()
@@ -631,23 +679,23 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         dvm, dMv, dxin = (lay["dvm"], lay["dMv"], lay["dxin"]) if T else (None, None, None)
         _glue("mlff_update_gate_adjoint", n, H, gs, gv, v_m, Mv, dgs if T else None,
               dgv if T else None, dvm, dMv, gy, dgy)
-        gw_, gb_ = _wgrad(gy, lay["h2"], w=w_atom, bias=True)
+        gw_, gb_ = _wgrad(gy, lay["h2"], w=w_atom, bias=True, group=grp)
         acc(grads, upre + ".update_mlp.2.weight", gw_)
         acc(grads, upre + ".update_mlp.2.bias", gb_)
         gh2 = gy @ U2
         ga2 = _silu_d(lay["a2"], gh2)
-        gw_, gb_ = _wgrad(ga2, xin, w=w_atom, bias=True)
+        gw_, gb_ = _wgrad(ga2, xin, w=w_atom, bias=True, group=grp)
         acc(grads, upre + ".update_mlp.0.weight", gw_)
         acc(grads, upre + ".update_mlp.0.bias", gb_)
         gx = ga2 @ U1
         dgx = None
         if T:
-            gw_, gb_ = _wgrad(dgy, lay["h2"], gy, lay["dh2"], bias=True)
+            gw_, gb_ = _wgrad(dgy, lay["h2"], gy, lay["dh2"], bias=True, group=grp)
             acc(tg, upre + ".update_mlp.2.weight", gw_)
             acc(tg, upre + ".update_mlp.2.bias", gb_)
             dgh2 = dgy @ U2
             dga2 = _silu_dd(lay["a2"], gh2, lay["da2"], dgh2)
-            gw_, gb_ = _wgrad(dga2, xin, ga2, dxin, bias=True)
+            gw_, gb_ = _wgrad(dga2, xin, ga2, dxin, bias=True, group=grp)
             acc(tg, upre + ".update_mlp.0.weight", gw_)
             acc(tg, upre + ".update_mlp.0.bias", gb_)
             dgx = dga2 @ U1
@@ -672,12 +720,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             gW, _ = _pair_grad(g, H, u, s, v, gs_m, gv_m)
         W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
-        gw_, gb_ = _wgrad(gW, lay["h"], w=w_pair, bias=True)
+        gw_, gb_ = _wgrad(gW, lay["h"], w=w_pair, bias=True, group=grp)
         acc(grads, mpre + ".2.weight", gw_)
         acc(grads, mpre + ".2.bias", gb_)
         gh = gW @ W2
         ga1 = _silu_d(lay["a1"], gh)
-        gw_, gb_ = _wgrad(ga1, c["e"], w=w_pair, bias=True)
+        gw_, gb_ = _wgrad(ga1, c["e"], w=w_pair, bias=True, group=grp)
         acc(grads, mpre + ".0.weight", gw_)
         acc(grads, mpre + ".0.bias", gb_)
         if rbf_grads:
@@ -686,12 +734,12 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             acc(grads, "rbf.widths", (wp(ge) * e_w).sum(0))
         if T:
             dh, da1 = lay["dh"], lay["da1"]
-            gw_, gb_ = _wgrad(dgW, lay["h"], gW, dh, bias=True)
+            gw_, gb_ = _wgrad(dgW, lay["h"], gW, dh, bias=True, group=grp)
             acc(tg, mpre + ".2.weight", gw_)
             acc(tg, mpre + ".2.bias", gb_)
             dgh = dgW @ W2
             dga1 = _silu_dd(lay["a1"], gh, da1, dgh)
-            gw_, gb_ = _wgrad(dga1, c["e"], ga1, c["de_t"], bias=True)
+            gw_, gb_ = _wgrad(dga1, c["e"], ga1, c["de_t"], bias=True, group=grp)
             acc(tg, mpre + ".0.weight", gw_)
             acc(tg, mpre + ".0.bias", gb_)
             if rbf_grads:
@@ -707,6 +755,7 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     acc(grads, "embedding.weight", _segment_sum(zptr, wa(gs), zorder))
     if T:
         acc(tg, "embedding.weight", _segment_sum(zptr, dgs, zorder))
+    grp.flush()
     return grads, (tg if T else None)
 
 

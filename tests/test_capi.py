@@ -206,6 +206,38 @@ def test_weight_grad_checks_arguments_on_the_host():
     assert lib.mlff_weight_grad_workspace_bytes(0, 8, 8) == (8 * 8 + 8) * 4
 
 
+def test_grouped_weight_grad_checks_arguments_on_the_host():
+    """mlff_weight_grad_partials checks what mlff_weight_grad checks, its
+    partial count matches the workspace size, and the grouped reduce rejects
+    bad tables before any device call."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    big = 1 << 30
+
+    def call(K, M, N, a0=f, b0=f, a1=None, b1=None, ws=f, nbytes=big):
+        return lib.mlff_weight_grad_partials(None, K, M, N, a0, M, b0, N, a1, M, b1, N, None, 1,
+                                             ws, nbytes)
+
+    assert call(100, 6, 8) == -1
+    assert call(100, 8, 8, a1=f) == -1
+    assert call(100, 8, 8, b0=odd) == -1
+    assert call(100, 8, 8, ws=None) == -1
+    need = lib.mlff_weight_grad_workspace_bytes(100, 8, 8)
+    assert call(100, 8, 8, nbytes=need - 4) == -1
+    for K, M, N in ((0, 8, 8), (100, 8, 8), (50132, 384, 128), (2736, 128, 20)):
+        S = lib.mlff_weight_grad_partial_count(K, M, N)
+        assert S >= 1
+        assert lib.mlff_weight_grad_workspace_bytes(K, M, N) == S * (M * N + M) * 4
+    assert lib.mlff_weight_grad_reduce_group(None, 0, None, 0) == 0      # nothing to add
+    assert lib.mlff_weight_grad_reduce_group(None, -1, f, 1) == -1
+    assert lib.mlff_weight_grad_reduce_group(None, 257, f, 1) == -1
+    assert lib.mlff_weight_grad_reduce_group(None, 2, None, 1) == -1
+    assert lib.mlff_weight_grad_reduce_group(None, 2, f, 0) == -1
+    assert lib.mlff_weight_grad_reduce_group(None, 2, odd, 1) == -1
+    assert "n_rows" in lib.mlff_last_error().decode()
+
+
 def test_silu_chain_checks_arguments_on_the_host():
     lib = C.lib()
     f = ctypes.c_void_p(1 << 20)

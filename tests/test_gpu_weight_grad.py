@@ -47,6 +47,39 @@ def test_weight_grad_matches_float64(gpu, K, M, N):
 
 
 @pytest.mark.gpu
+def test_grouped_partial_sums_equal_single_launches(gpu):
+    """Weight gradients whose partial sums are added by one grouped launch
+    (mlff_weight_grad_partials + mlff_weight_grad_reduce_group) have the
+    bits of mlff_weight_grad, for several shapes in one group, an empty row
+    range, and grids smaller than the work (the blocks then loop)."""
+    print("[case] weight_grad grouped reduce", flush=True)
+    g = torch.Generator().manual_seed(21)
+    cases = [(50132, 384, 128, True, True), (2736, 128, 256, False, True),
+             (37, 20, 20, True, False), (0, 8, 8, False, False), (50133, 64, 36, False, True)]
+    ops = []
+    for K, M, N, two, rw in cases:
+        mk = lambda c: torch.randn(K, c, generator=g).to(gpu)
+        a0, b0 = mk(M), mk(N)
+        extra = (mk(M), mk(N)) if two else ()
+        w = torch.rand(K, generator=g).to(gpu) if rw else None
+        ops.append(((a0, b0) + extra, w))
+    want = [train._wgrad(*args, w=w, bias=True) for args, w in ops]
+    lib = train.C.lib()
+    for n_blocks in (None, 1, 7):
+        grp = train._WgGroup()
+        got = [train._wgrad(*args, w=w, bias=True, group=grp) for args, w in ops]
+        if n_blocks is None:
+            grp.flush()
+        else:   # the same table on a fixed small grid
+            table = torch.tensor(grp.rows, dtype=torch.int64).to(gpu)
+            train.C.check("reduce_group", lib.mlff_weight_grad_reduce_group(
+                train.C.stream_handle(), len(grp.rows), table.data_ptr(), n_blocks))
+        torch.cuda.synchronize()
+        for (o1, b1), (o2, b2) in zip(want, got):
+            assert torch.equal(o1, o2) and torch.equal(b1, b2)
+
+
+@pytest.mark.gpu
 def test_weight_grad_on_strided_views(gpu):
     """Column slices of wider rows (row stride above the width) give the
     same bits as their contiguous copies."""
