@@ -539,12 +539,19 @@ class StudentForceField(nn.Module):
         evaluate() does.  If the neighbour list overflowed its capacity the
         enqueued results are discarded and evaluate() runs again with a
         regrown plan.  The results are ordered on the stream that was current
-        here: other streams wait for it before reading them.
+        here: other streams wait for it before reading them.  The enqueued
+        results are also the function's ``pending`` attribute, for work that
+        is enqueued behind them before the check (finish() returns that same
+        tuple when the check passes, and new results when it does not).
         """
         device, periodic = self._prepare(atomic_numbers, positions, sizes, cells, pbc)
         if positions.shape[0] == 0:
             out = self._empty_results(sizes, device, need_forces, need_virial)
-            return lambda: out
+
+            def done():
+                return out
+            done.pending = out
+            return done
         pos = positions.detach().to(torch.float32).contiguous()
         entry = self._plan_for(sizes, device, periodic, cells, pbc, pos)
         self.packed_weights(device)
@@ -562,6 +569,7 @@ class StudentForceField(nn.Module):
                 return out
             return self.evaluate(atomic_numbers, positions, sizes, cells, pbc, need_forces,
                                  need_virial)
+        finish.pending = out
         return finish
 
     # ------------------------------------------------------------------

@@ -836,11 +836,21 @@ class ForceMatchingTrainer:
         x = x.detach().to(torch.float32).contiguous()
         g = device_graph(Z, x, sizes, self.model.cutoff, cells, pbc)
         p = self._params()
-        # forces of the current weights from the fused HIP evaluation
-        _, F, _ = self.model.evaluate(Z, x, np.asarray(sizes), cells, pbc)
+        # forces of the current weights from the fused HIP evaluation.  The
+        # gradient pass is enqueued behind it before its neighbour-capacity
+        # check is read, so the host does not wait for the evaluation; if the
+        # list overflowed, the pass runs again on the regrown evaluation.
+        finish = self.model.evaluate_async(Z, x, np.asarray(sizes), cells, pbc)
         with torch.no_grad():
-            return force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label, F,
-                                        self.w_energy, self.w_force, rbf_grads=self.rbf_grads)
+            res = force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label,
+                                       finish.pending[1], self.w_energy, self.w_force,
+                                       rbf_grads=self.rbf_grads)
+            done = finish()
+            if done is not finish.pending:
+                res = force_matching_grads(p, Z, x, g, self.model.cutoff, E_label, F_label,
+                                           done[1], self.w_energy, self.w_force,
+                                           rbf_grads=self.rbf_grads)
+        return res
 
     def step(self, Z, x, sizes, E_label, F_label, cells=None, pbc=None) -> float:
         loss, E, F, grads = self.loss_and_grads(Z, x, sizes, E_label, F_label, cells, pbc)

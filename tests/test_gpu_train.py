@@ -283,3 +283,38 @@ def test_async_evaluation_regrows_on_overflow(gpu):
             assert cap0 and int(list(model._plans.values())[-1]["plan"].ecap) > cap0[-1]
         Eb, Fb, _ = model.evaluate(Zt, Pt, sizes)
         assert torch.equal(Ea, Eb) and torch.equal(Fa, Fb)
+
+
+def test_trainer_recomputes_after_capacity_overflow(gpu):
+    """The trainer enqueues its gradient pass behind the force evaluation
+    before reading the evaluation's capacity check.  When the neighbour list
+    overflows (the same batch compressed to half its size after a call that
+    sized the plan), the pass runs again on the regrown evaluation: loss and
+    gradients equal those of a trainer that meets the compressed batch
+    first."""
+    note("trainer: capacity overflow behind the enqueued gradient pass")
+    synth = pkg_mod("synth")
+    train = pkg_mod("train")
+    Z, P, mp = synth.flatten(synth.molecule_set(8, 30, 60, seed=9))
+    sizes = np.diff(mp)
+    Zt = torch.tensor(Z, device=gpu)
+    El = torch.zeros(len(sizes), device=gpu)
+    Fl = torch.zeros(len(Z), 3, device=gpu)
+    Ph = torch.tensor(P * 0.5, dtype=torch.float32, device=gpu)
+    runs = []
+    for warm in (True, False):
+        torch.manual_seed(0)
+        model = pkg().StudentForceField().to(gpu)
+        tr = train.ForceMatchingTrainer(model)
+        if warm:
+            tr.loss_and_grads(Zt, torch.tensor(P, dtype=torch.float32, device=gpu), sizes, El, Fl)
+            cap0 = int(list(model._plans.values())[-1]["plan"].ecap)
+        loss, E, F, grads = tr.loss_and_grads(Zt, Ph, sizes, El, Fl)
+        if warm:
+            assert int(list(model._plans.values())[-1]["plan"].ecap) > cap0
+        runs.append((loss, E, F, grads))
+    (l1, E1, F1, g1), (l2, E2, F2, g2) = runs
+    assert torch.equal(E1, E2) and torch.equal(F1, F2) and float(l1) == float(l2)
+    assert g1.keys() == g2.keys()
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
