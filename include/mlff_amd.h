@@ -444,9 +444,10 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
  * for n_rows outputs (at most 256): row r of table_dev (int64, 16-byte
  * aligned, on the device) is {S, n4, partials address, output address}, n4
  * being the output's float count / 4; partials and outputs 16-byte aligned.
- * Any n_blocks >= 1 covers every row; the sum of ceil(n4 / 8) over the rows
- * runs each block once.  Every element is added in the partition and order of
- * mlff_weight_grad, so the outputs are bit-identical to it. */
+ * Any n_blocks >= 1 covers every row; the sum of ceil(n4 / 256) over the
+ * rows runs each block once.  Every element is added in the partition and
+ * order of mlff_weight_grad, so the outputs are bit-identical to it.  Each
+ * row's S must be the partial count of the gradient it adds. */
 int mlff_weight_grad_partial_count(int K, int M, int N);
 int mlff_weight_grad_partials(void* stream, int K, int M, int N, const float* a0_dev, int lda0,
                               const float* b0_dev, int ldb0, const float* a1_dev, int lda1,

@@ -373,7 +373,6 @@ constexpr int kWgMaxSplits = 256;
 // flight beside the current one the kernel fits 168 VGPRs, three waves per
 // SIMD (three blocks per CU, also the LDS limit: 52 KB per block).
 template <bool TWO> constexpr int kWgUnroll = TWO ? 1 : 2;
-constexpr int kWgRedCols = 8;  // float4 columns per block of the partial reduction
 
 __device__ __forceinline__ void wg_mfma(f4 (&acc)[4][4], const f4& a, const f4& b) {
 #pragma unroll
@@ -516,19 +515,35 @@ __global__ __launch_bounds__(64 * kWgWaves, 3) void weight_grad_kernel(
   }
 }
 
-// out[i] = sum_s part[s][i] over S partials of n4 float4 elements, and the
-// same for the bias partials (nb4 elements; the blocks after the weight's
-// take them, so one launch reduces both).  A block takes kWgRedCols elements;
-// its G = 256 / kWgRedCols thread groups add the partials s = q, q + G, ...
-// in order, then the G group sums are added in group order (a fixed partition
-// and order).
+// One float4 of a weight gradient from its S partials: the partials added in
+// order in chunks of kWgChunk (loaded together), then the chunk sums added in
+// order.  The partition and order depend on S only (bit-reproducible).
+constexpr int kWgChunk = 8;
+
+__device__ __forceinline__ f4 wg_column_sum(int S, int64_t n4, const f4* __restrict__ part,
+                                            int64_t i) {
+  f4 tot = {0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < S; c += kWgChunk) {
+    f4 v[kWgChunk];
+#pragma unroll
+    for (int k = 0; k < kWgChunk; ++k)
+      if (c + k < S) v[k] = part[(size_t)(c + k) * n4 + i];
+    f4 cs = v[0];
+#pragma unroll
+    for (int k = 1; k < kWgChunk; ++k)
+      if (c + k < S) cs += v[k];
+    tot = c == 0 ? cs : tot + cs;
+  }
+  return tot;
+}
+
+// out[i] = sum_s part[s][i] over S partials of n4 float4 elements, one thread
+// per element, and the same for the bias partials (nb4 elements; the blocks
+// after the weight's take them, so one launch reduces both).
 __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_kernel(
     int S, int n4, const f4* __restrict__ part, f4* __restrict__ out, int nb4,
     const f4* __restrict__ part_b, f4* __restrict__ out_b) {
-  constexpr int G = kTrainBlock / kWgRedCols;
-  __shared__ f4 grp[kTrainBlock];
-  const int col = threadIdx.x % kWgRedCols, q = threadIdx.x / kWgRedCols;
-  const int blocks0 = (n4 + kWgRedCols - 1) / kWgRedCols;
+  const int blocks0 = (n4 + kTrainBlock - 1) / kTrainBlock;
   int blk = blockIdx.x;
   if (blk >= blocks0) {   // the bias partials (uniform per block)
     blk -= blocks0;
@@ -536,75 +551,49 @@ __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_kernel(
     out = out_b;
     n4 = nb4;
   }
-  const int i = blk * kWgRedCols + col;
-  f4 sum = {0.f, 0.f, 0.f, 0.f};
-  if (i < n4) {
-#pragma unroll 4
-    for (int s = q; s < S; s += G) sum += part[(size_t)s * n4 + i];
-  }
-  grp[threadIdx.x] = sum;
-  __syncthreads();
-  if (q == 0 && i < n4) {
-    f4 tot = grp[col];
-    for (int k = 1; k < G; ++k) tot += grp[k * kWgRedCols + col];
-    out[i] = tot;
-  }
+  const int i = blk * kTrainBlock + threadIdx.x;
+  if (i < n4) out[i] = wg_column_sum(S, n4, part, i);
 }
 
 // The same sums for several weight gradients in one launch.  Row r of the
 // table is {S, n4, partials address, output address} (int64 each); row r takes
-// ceil(n4 / kWgRedCols) consecutive virtual blocks, and the grid walks the
+// ceil(n4 / kTrainBlock) consecutive virtual blocks, and the grid walks the
 // virtual blocks with a stride of its size, so any grid covers every row.
-// Each element is added exactly as weight_grad_reduce_kernel adds it (same
-// partition and order), so the results are bit-identical to it.
-constexpr int kWgGroupMaxRows = 256;
+// Each element is added by wg_column_sum, as in weight_grad_reduce_kernel,
+// so the results are bit-identical to it.
+constexpr int kWgGroupMaxRows = kTrainBlock;
 struct WgRedRow {
   int64_t S, n4, part, out;
 };
 
 __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_group_kernel(
     int n_rows, const WgRedRow* __restrict__ table) {
-  constexpr int G = kTrainBlock / kWgRedCols;
-  __shared__ WgRedRow rows[kWgGroupMaxRows];
+  __shared__ int64_t scan[kTrainBlock];
   __shared__ int64_t first[kWgGroupMaxRows + 1];   // first virtual block of each row
-  __shared__ f4 grp[kTrainBlock];
-  for (int r = threadIdx.x; r < n_rows; r += kTrainBlock) rows[r] = table[r];
+  const int t = threadIdx.x;
+  // blocks per row, then an inclusive scan over the rows
+  scan[t] = t < n_rows ? (table[t].n4 + kTrainBlock - 1) / kTrainBlock : 0;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t v = 0;
-    for (int r = 0; r < n_rows; ++r) {
-      first[r] = v;
-      v += (rows[r].n4 + kWgRedCols - 1) / kWgRedCols;
-    }
-    first[n_rows] = v;
+  for (int off = 1; off < kTrainBlock; off <<= 1) {
+    const int64_t add = t >= off ? scan[t - off] : 0;
+    __syncthreads();
+    scan[t] += add;
+    __syncthreads();
   }
+  if (t == 0) first[0] = 0;
+  if (t < n_rows) first[t + 1] = scan[t];
   __syncthreads();
-  const int col = threadIdx.x % kWgRedCols, q = threadIdx.x / kWgRedCols;
   for (int64_t vb = blockIdx.x; vb < first[n_rows]; vb += gridDim.x) {
     int lo = 0, hi = n_rows - 1;   // the last row whose first block is <= vb
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (first[mid] <= vb) lo = mid; else hi = mid - 1;
     }
-    const WgRedRow& row = rows[lo];
-    const f4* part = reinterpret_cast<const f4*>(row.part);
-    f4* out = reinterpret_cast<f4*>(row.out);
-    const int64_t n4 = row.n4;
-    const int S = (int)row.S;
-    const int64_t i = (vb - first[lo]) * kWgRedCols + col;
-    f4 sum = {0.f, 0.f, 0.f, 0.f};
-    if (i < n4) {
-#pragma unroll 4
-      for (int s = q; s < S; s += G) sum += part[(size_t)s * n4 + i];
-    }
-    grp[threadIdx.x] = sum;
-    __syncthreads();
-    if (q == 0 && i < n4) {
-      f4 tot = grp[col];
-      for (int k = 1; k < G; ++k) tot += grp[k * kWgRedCols + col];
-      out[i] = tot;
-    }
-    __syncthreads();   // grp is reused by the next virtual block
+    const WgRedRow row = table[lo];
+    const int64_t i = (vb - first[lo]) * kTrainBlock + t;
+    if (i < row.n4)
+      reinterpret_cast<f4*>(row.out)[i] =
+          wg_column_sum((int)row.S, row.n4, reinterpret_cast<const f4*>(row.part), i);
   }
 }
 
@@ -1059,7 +1048,7 @@ int mlff_weight_grad(void* stream, int K, int M, int N, const float* a0_dev, int
   const float* part = static_cast<const float*>(ws_dev);
   const float* part_bias = part + (size_t)S * M * N;
   const int n4 = M * N / 4, nb4 = bias ? M / 4 : 0;
-  const int red_blocks = (n4 + kWgRedCols - 1) / kWgRedCols + (nb4 + kWgRedCols - 1) / kWgRedCols;
+  const int red_blocks = (n4 + kTrainBlock - 1) / kTrainBlock + (nb4 + kTrainBlock - 1) / kTrainBlock;
   weight_grad_reduce_kernel<<<red_blocks, kTrainBlock, 0, static_cast<hipStream_t>(stream)>>>(
       S, n4, reinterpret_cast<const f4*>(part), reinterpret_cast<f4*>(out_dev), nb4,
       reinterpret_cast<const f4*>(part_bias), reinterpret_cast<f4*>(bias_out_dev));

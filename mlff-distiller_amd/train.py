@@ -412,7 +412,7 @@ class _WgGroup:
             rows = self.rows[i:i + 256]
             table = torch.tensor(rows, dtype=torch.int64, pin_memory=True).to(
                 dev, non_blocking=True)
-            n_blocks = sum((n4 + 7) // 8 for _, n4, _, _ in rows)
+            n_blocks = sum((n4 + 255) // 256 for _, n4, _, _ in rows)
             C.check("mlff_weight_grad_reduce_group", lib.mlff_weight_grad_reduce_group(
                 C.stream_handle(), len(rows), table.data_ptr(), min(n_blocks, 1 << 16)))
         self.rows, self.keep = [], []
