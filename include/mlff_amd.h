@@ -482,6 +482,21 @@ int mlff_train_filter(void* stream, int tangent, int hidden, int n_pairs, int nu
                       const uint16_t* w2_planes_dev, const float* b2_dev, const float* a_dev,
                       float* a_out_dev, float* h_out_dev, float* w_out_dev);
 
+/* The reverse of the same MLP's first layer in the training pass (the
+ * autograd backward of rbf_to_scalar, student_model.py:318-322, in the
+ * reference's double backward, distillation_wrapper.py:86-91), over n_pairs
+ * pairs, fused in one pair-tile kernel:
+ *   ga  = silu'(a) (gw W2)
+ *   dga = silu'(a) (dgw W2) + silu''(a) da (gw W2)     (tangent = 1 only)
+ * gw, dgw [P][3 hidden] (the adjoint of the filter output W and its tangent),
+ * a, da, ga, dga [P][hidden], row-major float32; w2t_dev is W2 transposed,
+ * [hidden][3 hidden].  The products run on the FP32 matrix cores.  gw, dgw and
+ * w2t 16-byte aligned; hidden 32, 64 or 128. */
+int mlff_train_filter_adjoint(void* stream, int tangent, int hidden, int n_pairs,
+                              const float* gw_dev, const float* dgw_dev, const float* w2t_dev,
+                              const float* a_dev, const float* da_dev, float* ga_out_dev,
+                              float* dga_out_dev);
+
 /* Split n float32 values into three bfloat16 planes (planes_dev [3][n]):
  * x = hi + mid + lo, each part the round-to-nearest-even bfloat16 of what the
  * previous parts leave (24 significant bits together). */

@@ -49,7 +49,7 @@ EXPORTS = (
     "mlff_update_norm", "mlff_update_apply", "mlff_update_norm_tangent",
     "mlff_update_apply_tangent", "mlff_update_gate_adjoint", "mlff_update_input_adjoint",
     "mlff_probe_mfma_flops", "mlff_probe_mfma", "mlff_plan_set_weights",
-    "mlff_train_filter", "mlff_split_bf16x3", "mlff_pack_weights",
+    "mlff_train_filter", "mlff_train_filter_adjoint", "mlff_split_bf16x3", "mlff_pack_weights",
 )
 
 # bytes of the static device scratch behind mlff_energy_forces_static
@@ -181,6 +181,8 @@ def _declare(lib):
     lib.mlff_train_filter.restype = ctypes.c_int
     lib.mlff_train_filter.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int] + [vp] * 9
+    lib.mlff_train_filter_adjoint.restype = ctypes.c_int
+    lib.mlff_train_filter_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [vp] * 7
     lib.mlff_split_bf16x3.restype = ctypes.c_int
     lib.mlff_split_bf16x3.argtypes = [vp, ctypes.c_int64, vp, vp]
     lib.mlff_pack_weights.restype = ctypes.c_int

@@ -6,7 +6,16 @@
 //   tangent:  da = de W1^T,       dh = silu'(a) da,     dW = dh W2^T
 //
 // e (or de) is [P][R] per pair; a, h (da, dh) are [P][H] and W (dW) is
-// [P][3H], all row-major float32.  The first product runs on the FP32 matrix
+// [P][3H], all row-major float32.
+//
+// A third kernel runs the reverse of the first layer's product and SiLU for
+// the reverse pass (student_model.py:318-322 under autograd), given the
+// filter-output adjoint gW [P][3H] and its tangent dgW:
+//
+//   gh = gW W2,   ga = silu'(a) gh,
+//   dgh = dgW W2, dga = silu'(a) dgh + silu''(a) da gh
+//
+// with the products on the FP32 matrix cores (W2 comes transposed, [H][3H]).  The first product runs on the FP32 matrix
 // cores; the second, the large one, on the bf16 matrix cores with both
 // operands split into three bf16 parts (six part products, float32
 // accuracy), as in the evaluation's filter kernel.  W1 comes padded to RP
@@ -122,6 +131,77 @@ __global__ __launch_bounds__(kTfBlock, 4) void train_filter_kernel(
   }
 }
 
+// The filter adjoint kernel: TM pairs per tile, the K = 3H sum of the
+// product in three chunks of H columns staged in LDS (for TAN both gW and
+// dgW).  The 8 waves split the H output columns into CT tiles of 16 and the
+// tile's rows into RG = 8 / CT groups.
+template <int H>
+constexpr int kTaPairs = H <= 32 ? 128 : 64;
+
+template <int H, int TM, bool TAN>
+__global__ __launch_bounds__(kTfBlock, 4) void train_filter_adjoint_kernel(
+    int P, const float* __restrict__ gw, const float* __restrict__ dgw,
+    const float* __restrict__ w2t, const float* __restrict__ a, const float* __restrict__ da,
+    float* __restrict__ ga, float* __restrict__ dga) {
+  constexpr int KC = H;                 // columns per chunk
+  constexpr int LDA = KC + 4;
+  constexpr int CT = H / 16, RG = kTfWaves / CT, RTW = TM / 16 / RG;
+  static_assert(kTfWaves % CT == 0 && (TM / 16) % RG == 0, "wave split");
+  constexpr int NSTAGE = TAN ? 2 : 1;
+  __shared__ __align__(16) float a_lds[NSTAGE][TM * LDA];
+  const int ntiles = (P + TM - 1) / TM;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int ct = wave % CT, rg = wave / CT;
+  const int cb[1] = {ct * 16};
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int pbase = tile * TM;
+    f4 acc[RTW][1], dacc[RTW][1];
+    zero_acc(acc);
+    if (TAN) zero_acc(dacc);
+#pragma unroll 1
+    for (int ch = 0; ch < 3; ++ch) {
+      __syncthreads();   // the previous chunk's products are done with the LDS
+      for (int idx = threadIdx.x; idx < TM * (KC / 4); idx += kTfBlock) {
+        const int m = idx / (KC / 4), k4 = idx - m * (KC / 4), p = pbase + m;
+        const size_t o = (size_t)p * 3 * H + ch * KC + 4 * k4;
+        const f4 zero = {0.f, 0.f, 0.f, 0.f};
+        *reinterpret_cast<f4*>(&a_lds[0][m * LDA + 4 * k4]) =
+            p < P ? *reinterpret_cast<const f4*>(gw + o) : zero;
+        if (TAN)
+          *reinterpret_cast<f4*>(&a_lds[NSTAGE - 1][m * LDA + 4 * k4]) =
+              p < P ? *reinterpret_cast<const f4*>(dgw + o) : zero;
+      }
+      __syncthreads();
+      const float* arows = &a_lds[0][rg * RTW * 16 * LDA];
+      mma_tile_k<RTW, 1, KC>(arows, LDA, w2t + ch * KC, 3 * H, cb, acc);
+      if (TAN)
+        mma_tile_k<RTW, 1, KC>(&a_lds[NSTAGE - 1][rg * RTW * 16 * LDA], LDA, w2t + ch * KC,
+                               3 * H, cb, dacc);
+    }
+    // lane (r, g) holds rows (rg RTW + rt) 16 + 4g + i of column cb + r
+    const int c = cb[0] + r;
+#pragma unroll
+    for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = pbase + (rg * RTW + rt) * 16 + 4 * g + i;
+        if (p >= P) continue;
+        const size_t o = (size_t)p * H + c;
+        const float x = a[o];
+        // s = sigmoid(x); 1 - s as 1 / (1 + exp(x)) (exact for large x)
+        const float sg = 1.0f / (1.0f + expf(-x));
+        const float sm = 1.0f / (1.0f + expf(x));
+        const float d1 = sg * (1.0f + x * sm);
+        const float gh = acc[rt][0][i];
+        ga[o] = gh * d1;
+        if (TAN) {
+          const float d2 = sg * sm * (2.0f + x * (sm - sg));
+          dga[o] = dacc[rt][0][i] * d1 + gh * d2 * da[o];
+        }
+      }
+  }
+}
+
 __global__ void split_bf16x3_kernel(int64_t n, const float* __restrict__ src,
                                     uint16_t* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -145,6 +225,18 @@ hipError_t launch_train_filter(hipStream_t s, int P, int R, int RP, const float*
   const int grid = tiles < 2048 ? tiles : 2048;
   train_filter_kernel<H, TM, TAN><<<grid, kTfBlock, 0, s>>>(P, R, RP, x, w1p, b1, w2bf, b2,
                                                             a_in, a_out, h_out, w_out);
+  return hipGetLastError();
+}
+
+template <int H, bool TAN>
+hipError_t launch_train_filter_adjoint(hipStream_t s, int P, const float* gw, const float* dgw,
+                                       const float* w2t, const float* a, const float* da,
+                                       float* ga, float* dga) {
+  constexpr int TM = kTaPairs<H>;
+  const int tiles = (P + TM - 1) / TM;
+  const int grid = tiles < 2048 ? tiles : 2048;
+  train_filter_adjoint_kernel<H, TM, TAN><<<grid, kTfBlock, 0, s>>>(P, gw, dgw, w2t, a, da, ga,
+                                                                    dga);
   return hipGetLastError();
 }
 
@@ -215,6 +307,55 @@ int mlff_train_filter(void* stream, int tangent, int hidden, int n_pairs, int nu
     MLFF_TF(128, false)
     MLFF_TF(128, true)
 #undef MLFF_TF
+    default:
+      e = hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) {
+    set_error("%s: %s", nm, hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+int mlff_train_filter_adjoint(void* stream, int tangent, int hidden, int n_pairs,
+                              const float* gw_dev, const float* dgw_dev, const float* w2t_dev,
+                              const float* a_dev, const float* da_dev, float* ga_out_dev,
+                              float* dga_out_dev) {
+  const char* nm = "mlff_train_filter_adjoint";
+  if (hidden != 32 && hidden != 64 && hidden != 128) {
+    set_error("%s: hidden_dim %d not supported (32, 64, 128)", nm, hidden);
+    return MLFF_ERR_UNSUPPORTED;
+  }
+  if (n_pairs < 0) {
+    set_error("%s: need n_pairs >= 0", nm);
+    return MLFF_ERR_INVALID;
+  }
+  const bool tan = tangent != 0;
+  if (!gw_dev || !w2t_dev || !a_dev || !ga_out_dev ||
+      (tan && (!dgw_dev || !da_dev || !dga_out_dev))) {
+    set_error("%s: null buffer (the tangent needs dgw, da and dga_out)", nm);
+    return MLFF_ERR_INVALID;
+  }
+  if (!aligned16(gw_dev) || !aligned16(w2t_dev) || (tan && !aligned16(dgw_dev))) {
+    set_error("%s: gw, dgw and w2t must be 16-byte aligned", nm);
+    return MLFF_ERR_INVALID;
+  }
+  if (n_pairs == 0) return MLFF_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e;
+  switch (hidden * 2 + (tan ? 1 : 0)) {
+#define MLFF_TA(HH, TT)                                                                    \
+  case HH * 2 + TT:                                                                        \
+    e = launch_train_filter_adjoint<HH, TT>(s, n_pairs, gw_dev, dgw_dev, w2t_dev, a_dev,   \
+                                            da_dev, ga_out_dev, dga_out_dev);              \
+    break;
+    MLFF_TA(32, false)
+    MLFF_TA(32, true)
+    MLFF_TA(64, false)
+    MLFF_TA(64, true)
+    MLFF_TA(128, false)
+    MLFF_TA(128, true)
+#undef MLFF_TA
     default:
       e = hipErrorInvalidValue;
   }

@@ -281,7 +281,7 @@ def _silu_dd(x, g, dx, dg=None):
 
 
 # prepared filter weights per (W1, W2) tensor pair: W1 padded to 16 or 32
-# columns and W2 as three bf16 planes.  An entry belongs to the two tensor
+# columns, W2 as three bf16 planes and W2 transposed.  An entry belongs to the two tensor
 # objects themselves (weak references: a tensor allocated later at a freed
 # one's address, or under its id, never matches) and is remade when the
 # optimizer has written either tensor (its version counter moved).
@@ -295,25 +295,28 @@ def _filter_weights(p: Dict[str, Tensor], pre: str):
     ent = _FILTER_W.get(key)
     same = ent is not None and ent[0]() is W1o and ent[1]() is W2o
     if same and ent[2] == state:
-        return ent[3], ent[4]
+        return ent[3], ent[4], ent[5]
     W1, W2 = _dense(W1o), _dense(W2o)
     H, R = W1.shape
     RP = 16 if R <= 16 else 32
     if len(_FILTER_W) > 64:
         for k in [k for k, e in _FILTER_W.items() if e[0]() is None or e[1]() is None]:
             del _FILTER_W[k]
-    if same and ent[3].shape == (H, RP) and ent[4].numel() == 3 * W2.numel():
+    if (same and ent[3].shape == (H, RP) and ent[4].numel() == 3 * W2.numel()
+            and ent[5].shape == W2.T.shape):
         # the same tensors, written by the optimizer: refill the buffers in place
         # (W1's padding columns stay zero)
-        w1p, planes = ent[3], ent[4]
+        w1p, planes, w2t = ent[3], ent[4], ent[5]
     else:
         w1p = torch.zeros(H, RP, dtype=torch.float32, device=W1.device)
         planes = torch.empty(3 * W2.numel(), dtype=torch.int16, device=W2.device)
+        w2t = torch.empty(W2.shape[1], W2.shape[0], dtype=torch.float32, device=W2.device)
     w1p[:, :R].copy_(W1)
+    w2t.copy_(W2.T)
     C.check("mlff_split_bf16x3", C.lib().mlff_split_bf16x3(
         C.stream_handle(), W2.numel(), C.ptr(W2), C.ptr(planes)))
-    _FILTER_W[key] = (weakref.ref(W1o), weakref.ref(W2o), state, w1p, planes)
-    return w1p, planes
+    _FILTER_W[key] = (weakref.ref(W1o), weakref.ref(W2o), state, w1p, planes, w2t)
+    return w1p, planes, w2t
 
 
 def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = None):
@@ -321,7 +324,7 @@ def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = Non
     (a None): x = e gives (a1, h, W) with a1 = e W1^T + b1, h = silu(a1),
     W = h W2^T + b2.  Tangent: x = de and a = a1 give (da1, dh, dW) with
     da1 = de W1^T, dh = silu'(a1) da1, dW = dh W2^T."""
-    w1p, planes = _filter_weights(p, pre)
+    w1p, planes, _ = _filter_weights(p, pre)
     x = _dense(x)
     P, R = x.shape
     H = p[pre + ".0.weight"].shape[0]
@@ -338,6 +341,26 @@ def _filter(p: Dict[str, Tensor], pre: str, x: Tensor, a: Optional[Tensor] = Non
         C.ptr(b2), C.ptr(_dense(a) if tan else None), C.ptr(a_out), C.ptr(h_out),
         C.ptr(w_out)))
     return a_out, h_out, w_out
+
+
+def _filter_adjoint(p: Dict[str, Tensor], pre: str, gW: Tensor, a1: Tensor,
+                    dgW: Optional[Tensor] = None, da1: Optional[Tensor] = None):
+    """Reverse of the filter's first layer on the fused pair kernel
+    (mlff_train_filter_adjoint): ga1 = silu'(a1) (gW W2) and, with the
+    tangent (dgW, da1), dga1 = silu'(a1) (dgW W2) + silu''(a1) da1 (gW W2).
+    Returns (ga1, dga1 or None)."""
+    _, _, w2t = _filter_weights(p, pre)
+    gW, a1 = _dense(gW), _dense(a1)
+    P, H = a1.shape
+    if w2t.shape != (H, 3 * H) or gW.shape != (P, 3 * H):
+        raise RuntimeError("filter adjoint operands do not match the layer")
+    tan = dgW is not None
+    ga = torch.empty(P, H, dtype=torch.float32, device=a1.device)
+    dga = torch.empty_like(ga) if tan else None
+    C.check("mlff_train_filter_adjoint", C.lib().mlff_train_filter_adjoint(
+        C.stream_handle(), int(tan), H, P, C.ptr(gW), C.ptr(_dense(dgW) if tan else None),
+        C.ptr(w2t), C.ptr(a1), C.ptr(_dense(da1) if tan else None), C.ptr(ga), C.ptr(dga)))
+    return ga, dga
 
 
 def _edge_features(d, centers, widths, rc):
@@ -718,13 +741,13 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
                                  dgs_m, dgv_m)
         else:
             gW, _ = _pair_grad(g, H, u, s, v, gs_m, gv_m)
-        W2 = p[mpre + ".2.weight"]
         W1 = p[mpre + ".0.weight"]
         gw_, gb_ = _wgrad(gW, lay["h"], w=w_pair, bias=True, group=grp)
         acc(grads, mpre + ".2.weight", gw_)
         acc(grads, mpre + ".2.bias", gb_)
-        gh = gW @ W2
-        ga1 = _silu_d(lay["a1"], gh)
+        # gW W2 through the SiLU (and its tangent) on the fused pair kernel
+        ga1, dga1 = _filter_adjoint(p, mpre, gW, lay["a1"], dgW if T else None,
+                                    lay["da1"] if T else None)
         gw_, gb_ = _wgrad(ga1, c["e"], w=w_pair, bias=True, group=grp)
         acc(grads, mpre + ".0.weight", gw_)
         acc(grads, mpre + ".0.bias", gb_)
@@ -733,12 +756,10 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
             acc(grads, "rbf.centers", (wp(ge) * e_mu).sum(0))
             acc(grads, "rbf.widths", (wp(ge) * e_w).sum(0))
         if T:
-            dh, da1 = lay["dh"], lay["da1"]
+            dh = lay["dh"]
             gw_, gb_ = _wgrad(dgW, lay["h"], gW, dh, bias=True, group=grp)
             acc(tg, mpre + ".2.weight", gw_)
             acc(tg, mpre + ".2.bias", gb_)
-            dgh = dgW @ W2
-            dga1 = _silu_dd(lay["a1"], gh, da1, dgh)
             gw_, gb_ = _wgrad(dga1, c["e"], ga1, c["de_t"], bias=True, group=grp)
             acc(tg, mpre + ".0.weight", gw_)
             acc(tg, mpre + ".0.bias", gb_)

@@ -267,6 +267,16 @@ def test_train_filter_checks_arguments_on_the_host():
                                  f) == -1          # tangent without a
     assert lib.mlff_train_filter(None, 0, 128, 10, 20, f, odd, f, f, f, None, f, f, f) == -1
     assert lib.mlff_train_filter(None, 0, 128, 0, 20, *ok) == 0
+    # the fused filter adjoint: hidden size, counts, buffers, alignment
+    ta = lib.mlff_train_filter_adjoint
+    assert ta(None, 1, 48, 10, f, f, f, f, f, f, f) == -2
+    assert ta(None, 1, 128, -1, f, f, f, f, f, f, f) == -1
+    assert ta(None, 0, 128, 10, None, None, f, f, None, f, None) == -1
+    assert ta(None, 1, 128, 10, f, None, f, f, f, f, f) == -1    # tangent without dgw
+    assert ta(None, 1, 128, 10, f, f, f, f, f, f, None) == -1    # tangent without dga
+    assert ta(None, 0, 128, 10, odd, None, f, f, None, f, None) == -1
+    assert ta(None, 0, 128, 10, f, None, odd, f, None, f, None) == -1
+    assert ta(None, 1, 128, 0, f, f, f, f, f, f, f) == 0
     assert lib.mlff_split_bf16x3(None, -1, f, f) == -1
     assert lib.mlff_split_bf16x3(None, 4, None, f) == -1
     assert lib.mlff_split_bf16x3(None, 0, None, None) == 0
