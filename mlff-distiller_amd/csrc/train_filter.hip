@@ -131,37 +131,96 @@ __global__ __launch_bounds__(kTfBlock, 4) void train_filter_kernel(
   }
 }
 
-// The filter adjoint kernel: TM pairs per tile, the K = 3H sum of the
-// product in three chunks of H columns staged in LDS (for TAN both gW and
-// dgW).  The 8 waves split the H output columns into CT tiles of 16 and the
-// tile's rows into RG = 8 / CT groups.
+// The filter adjoint kernel: 256-thread blocks (four per CU, so one block's
+// staging overlaps the others' products), TM pairs per tile, the K = 3H sum
+// of the product in three chunks of H columns staged in LDS (for TAN both gW
+// and dgW).  The 4 waves split the CT = H/16 output column tiles (NTW each)
+// and, when there are fewer than 4, the tile's row tiles (RG groups).
+constexpr int kTaBlock = 256;
+
+// mma_tile_k over two A operands (the adjoint and its tangent, same rows) with
+// one B operand: each B fetch, PF slabs ahead, feeds both products.
+template <int RT, int NT, int K, int PF = 2>
+__device__ __forceinline__ void mma_tile_k_pair(const float* __restrict__ A0,
+                                                const float* __restrict__ A1, int lda,
+                                                const float* __restrict__ B, int ldb,
+                                                const int (&cb)[NT], f4 (&acc0)[RT][NT],
+                                                f4 (&acc1)[RT][NT]) {
+  static_assert(K % 16 == 0, "K must be a multiple of 16");
+  constexpr int S = K / 16;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const int aoff = r * lda + 4 * g;
+  int boff[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) boff[ct] = (cb[ct] + r) * ldb + 4 * g;
+  f4 b[PF + 1][NT];
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct)
+      if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * q);
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    if (sl + PF < S) {
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct)
+        b[(sl + PF) % (PF + 1)][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * (sl + PF));
+    }
+    f4 a0[RT], a1[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      a0[rt] = *reinterpret_cast<const f4*>(A0 + aoff + rt * 16 * lda + 16 * sl);
+      a1[rt] = *reinterpret_cast<const f4*>(A1 + aoff + rt * 16 * lda + 16 * sl);
+    }
+    const int cur = sl % (PF + 1);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          acc0[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt][s], b[cur][ct][s],
+                                                              acc0[rt][ct], 0, 0, 0);
+          acc1[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt][s], b[cur][ct][s],
+                                                              acc1[rt][ct], 0, 0, 0);
+        }
+  }
+}
+constexpr int kTaWaves = kTaBlock / 64;
 template <int H>
-constexpr int kTaPairs = H <= 32 ? 128 : 64;
+constexpr int kTaPairs = H >= 128 ? 32 : (H >= 64 ? 64 : 128);
 
 template <int H, int TM, bool TAN>
-__global__ __launch_bounds__(kTfBlock, 4) void train_filter_adjoint_kernel(
+__global__ __launch_bounds__(kTaBlock, 4) void train_filter_adjoint_kernel(
     int P, const float* __restrict__ gw, const float* __restrict__ dgw,
     const float* __restrict__ w2t, const float* __restrict__ a, const float* __restrict__ da,
     float* __restrict__ ga, float* __restrict__ dga) {
   constexpr int KC = H;                 // columns per chunk
   constexpr int LDA = KC + 4;
-  constexpr int CT = H / 16, RG = kTfWaves / CT, RTW = TM / 16 / RG;
-  static_assert(kTfWaves % CT == 0 && (TM / 16) % RG == 0, "wave split");
+  constexpr int CT = H / 16;
+  constexpr int NTW = CT >= kTaWaves ? CT / kTaWaves : 1;
+  constexpr int RG = CT >= kTaWaves ? 1 : kTaWaves / CT;
+  constexpr int RTW = TM / 16 / RG;
+  static_assert(NTW * kTaWaves == CT * RG && (TM / 16) % RG == 0, "wave split");
   constexpr int NSTAGE = TAN ? 2 : 1;
   __shared__ __align__(16) float a_lds[NSTAGE][TM * LDA];
   const int ntiles = (P + TM - 1) / TM;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const int ct = wave % CT, rg = wave / CT;
-  const int cb[1] = {ct * 16};
+  const int cw = wave % (CT / NTW), rg = wave / (CT / NTW);
+  int cb[NTW];
+#pragma unroll
+  for (int ct = 0; ct < NTW; ++ct) cb[ct] = (cw * NTW + ct) * 16;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int pbase = tile * TM;
-    f4 acc[RTW][1], dacc[RTW][1];
+    f4 acc[RTW][NTW], dacc[RTW][NTW];
     zero_acc(acc);
     if (TAN) zero_acc(dacc);
 #pragma unroll 1
     for (int ch = 0; ch < 3; ++ch) {
       __syncthreads();   // the previous chunk's products are done with the LDS
-      for (int idx = threadIdx.x; idx < TM * (KC / 4); idx += kTfBlock) {
+      for (int idx = threadIdx.x; idx < TM * (KC / 4); idx += kTaBlock) {
         const int m = idx / (KC / 4), k4 = idx - m * (KC / 4), p = pbase + m;
         const size_t o = (size_t)p * 3 * H + ch * KC + 4 * k4;
         const f4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -172,31 +231,35 @@ __global__ __launch_bounds__(kTfBlock, 4) void train_filter_adjoint_kernel(
               p < P ? *reinterpret_cast<const f4*>(dgw + o) : zero;
       }
       __syncthreads();
-      const float* arows = &a_lds[0][rg * RTW * 16 * LDA];
-      mma_tile_k<RTW, 1, KC>(arows, LDA, w2t + ch * KC, 3 * H, cb, acc);
       if (TAN)
-        mma_tile_k<RTW, 1, KC>(&a_lds[NSTAGE - 1][rg * RTW * 16 * LDA], LDA, w2t + ch * KC,
-                               3 * H, cb, dacc);
+        mma_tile_k_pair<RTW, NTW, KC>(&a_lds[0][rg * RTW * 16 * LDA],
+                                      &a_lds[NSTAGE - 1][rg * RTW * 16 * LDA], LDA,
+                                      w2t + ch * KC, 3 * H, cb, acc, dacc);
+      else
+        mma_tile_k<RTW, NTW, KC>(&a_lds[0][rg * RTW * 16 * LDA], LDA, w2t + ch * KC, 3 * H,
+                                 cb, acc);
     }
-    // lane (r, g) holds rows (rg RTW + rt) 16 + 4g + i of column cb + r
-    const int c = cb[0] + r;
+    // lane (r, g) holds rows (rg RTW + rt) 16 + 4g + i of columns cb[ct] + r
 #pragma unroll
     for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int p = pbase + (rg * RTW + rt) * 16 + 4 * g + i;
         if (p >= P) continue;
-        const size_t o = (size_t)p * H + c;
-        const float x = a[o];
-        // s = sigmoid(x); 1 - s as 1 / (1 + exp(x)) (exact for large x)
-        const float sg = 1.0f / (1.0f + expf(-x));
-        const float sm = 1.0f / (1.0f + expf(x));
-        const float d1 = sg * (1.0f + x * sm);
-        const float gh = acc[rt][0][i];
-        ga[o] = gh * d1;
-        if (TAN) {
-          const float d2 = sg * sm * (2.0f + x * (sm - sg));
-          dga[o] = dacc[rt][0][i] * d1 + gh * d2 * da[o];
+#pragma unroll
+        for (int ct = 0; ct < NTW; ++ct) {
+          const size_t o = (size_t)p * H + cb[ct] + r;
+          const float x = a[o];
+          // s = sigmoid(x); 1 - s as 1 / (1 + exp(x)) (exact for large x)
+          const float sg = 1.0f / (1.0f + expf(-x));
+          const float sm = 1.0f / (1.0f + expf(x));
+          const float d1 = sg * (1.0f + x * sm);
+          const float gh = acc[rt][ct][i];
+          ga[o] = gh * d1;
+          if (TAN) {
+            const float d2 = sg * sm * (2.0f + x * (sm - sg));
+            dga[o] = dacc[rt][ct][i] * d1 + gh * d2 * da[o];
+          }
         }
       }
   }
@@ -234,8 +297,9 @@ hipError_t launch_train_filter_adjoint(hipStream_t s, int P, const float* gw, co
                                        float* ga, float* dga) {
   constexpr int TM = kTaPairs<H>;
   const int tiles = (P + TM - 1) / TM;
-  const int grid = tiles < 2048 ? tiles : 2048;
-  train_filter_adjoint_kernel<H, TM, TAN><<<grid, kTfBlock, 0, s>>>(P, gw, dgw, w2t, a, da, ga,
+  // one tile per block (the dispatcher balances the CUs), strided past 8192
+  const int grid = tiles < 8192 ? tiles : 8192;
+  train_filter_adjoint_kernel<H, TM, TAN><<<grid, kTaBlock, 0, s>>>(P, gw, dgw, w2t, a, da, ga,
                                                                     dga);
   return hipGetLastError();
 }
