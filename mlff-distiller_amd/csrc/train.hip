@@ -601,7 +601,11 @@ __global__ __launch_bounds__(kTrainBlock) void weight_grad_reduce_group_kernel(
 // at most kWgSlots blocks, the ones resident at once on MI355X (256 CUs, three
 // blocks of this kernel each), so the grid runs as a single round instead of
 // leaving a few blocks for a second one; at most kWgMaxSplits partials.
+// A wave takes at least kWgMinRows rows: with fewer (a few thousand atoms
+// over hundreds of blocks) the partial sums written and re-read cost more
+// than the rows they split.
 constexpr long kWgSlots = 768;
+constexpr long kWgMinRows = 32;
 void weight_grad_split(int K, int M, int N, int& RW, int& S) {
   const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
   long s = kWgSlots / tiles;
@@ -609,7 +613,7 @@ void weight_grad_split(int K, int M, int N, int& RW, int& S) {
   if (s < 1) s = 1;
   long rw = ((long)K + 4 * s - 1) / (4 * s);
   rw = (rw + 3) / 4 * 4;
-  if (rw < 4) rw = 4;
+  if (rw < kWgMinRows) rw = kWgMinRows;
   RW = (int)rw;
   S = K > 0 ? (int)(((long)K + 4 * rw - 1) / (4 * rw)) : 1;
 }
