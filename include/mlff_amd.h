@@ -465,6 +465,21 @@ int mlff_silu_chain(void* stream, int64_t n, const float* a_dev, const float* da
                     const float* g_dev, const float* dg_dev, float* h_dev, float* dh_dev,
                     float* ga_dev, float* dga_dev);
 
+/* Gradient of PaiNNUpdate's 3x3 mixing_matrix (student_model.py:414 and 460
+ * under the reference's autograd) over n_atoms atoms:
+ *   out[x][z] = sum_n w_n sum_h (a0[n][x][h] b0[n][z][h] + a1[n][x][h] b1[n][z][h])
+ * a0, b0, a1, b1 [n_atoms][3][hidden] float32, 16-byte aligned (a1/b1 NULL for
+ * one term, row_w_dev NULL for w = 1; all may be NULL when n_atoms is 0);
+ * hidden a multiple of 4.  Writes
+ * S = mlff_xyz_outer_partial_count(n_atoms) partial sums of 12 floats each
+ * (the 9 sums row-major, then zeros) to part_dev [S][12]; the
+ * mlff_weight_grad_reduce_group row {S, 3, part_dev, out} adds them (in a
+ * fixed partition and order) into out[12]. */
+int mlff_xyz_outer_partial_count(int n_atoms);
+int mlff_xyz_outer_partials(void* stream, int n_atoms, int hidden, const float* a0_dev,
+                            const float* b0_dev, const float* a1_dev, const float* b1_dev,
+                            const float* row_w_dev, float* part_dev);
+
 /* The filter MLP of PaiNNMessage (student_model.py:313-322, 350-355) over
  * n_pairs pairs in the training pass, fused in one pair-tile kernel:
  *   tangent = 0:  a = x W1^T + b1,  h = silu(a),         W = h W2^T + b2

@@ -249,6 +249,72 @@ __global__ __launch_bounds__(kGlueBlock) void update_input_adjoint_kernel(
   }
 }
 
+// Gradient of the 3x3 mixing matrix (student_model.py:414 and 460 under
+// autograd):
+//   out[x][z] = sum_n w_n sum_h (a0[n][x][h] b0[n][z][h] + a1[n][x][h] b1[n][z][h])
+// over n atoms ([n][3][H] operands), as partial sums: block k takes the atoms
+// [k A, (k + 1) A) with A = xyz_atoms(n) (a partition that depends on n only),
+// each thread the items (atom, float4 of channels) t, t + 256, ... in order,
+// and the block adds its threads' sums in a fixed tree.  Partial k is 12
+// floats: the 9 sums row-major, then zeros (three float4 outputs, added by the
+// weight-gradient reduce).
+constexpr int kXyzMinAtoms = 32;
+constexpr int kXyzMaxParts = 256;
+int xyz_atoms(int n) {
+  const int a = (n + kXyzMaxParts - 1) / kXyzMaxParts;
+  return a > kXyzMinAtoms ? a : kXyzMinAtoms;
+}
+int xyz_parts(int n) { return n > 0 ? (n + xyz_atoms(n) - 1) / xyz_atoms(n) : 1; }
+
+template <bool TWO, bool ROWW>
+__global__ __launch_bounds__(kGlueBlock) void xyz_outer_kernel(
+    int n, int H, int A, const float* __restrict__ a0, const float* __restrict__ b0,
+    const float* __restrict__ a1, const float* __restrict__ b1, const float* __restrict__ w,
+    float* __restrict__ part) {
+  __shared__ float red[9][kGlueBlock];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * A, i1 = min(n, i0 + A);
+  const int G = H / 4;
+  float acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.f;
+  for (int item = t; item < (i1 - i0) * G; item += kGlueBlock) {
+    const int i = i0 + item / G, c = 4 * (item - (item / G) * G);
+    f4 av[3], bv[3];
+    load3(a0, i, H, c, av);
+    load3(b0, i, H, c, bv);
+    f4 av1[3], bv1[3];
+    if (TWO) {
+      load3(a1, i, H, c, av1);
+      load3(b1, i, H, c, bv1);
+    }
+    const float wi = ROWW ? w[i] : 1.f;
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int z = 0; z < 3; ++z) {
+        const f4 q = av[x] * bv[z];
+        float d = (q[0] + q[1]) + (q[2] + q[3]);
+        if (TWO) {
+          const f4 q1 = av1[x] * bv1[z];
+          d += (q1[0] + q1[1]) + (q1[2] + q1[3]);
+        }
+        acc[3 * x + z] += ROWW ? wi * d : d;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) red[k][t] = acc[k];
+  __syncthreads();
+  for (int h = kGlueBlock / 2; h > 0; h >>= 1) {
+    if (t < h) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) red[k][t] += red[k][t + h];
+    }
+    __syncthreads();
+  }
+  if (t < 12) part[(size_t)blockIdx.x * 12 + t] = t < 9 ? red[t][0] : 0.f;
+}
+
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // shared checks: sizes, required pointers non-null, every pointer aligned
@@ -411,6 +477,33 @@ int mlff_update_input_adjoint(void* stream, int n_atoms, int hidden, const float
         n_atoms, hidden, m_dev, gx_dev, gs_dev, gv_dev, y_dev, v_dev, xin_dev, dgx_dev, dgs_dev,
         dgv_dev, dy_dev, dv_dev, dxin_dev, gs_out_dev, gv_out_dev, gvg2_out_dev, dgs_out_dev,
         dgv_out_dev, dgvg2_out_dev);
+  return finish(nm);
+}
+
+int mlff_xyz_outer_partial_count(int n_atoms) { return n_atoms < 0 ? 0 : xyz_parts(n_atoms); }
+
+int mlff_xyz_outer_partials(void* stream, int n_atoms, int hidden, const float* a0_dev,
+                            const float* b0_dev, const float* a1_dev, const float* b1_dev,
+                            const float* row_w_dev, float* part_dev) {
+  const char* nm = "mlff_xyz_outer_partials";
+  const bool two = a1_dev != nullptr;
+  if (int rc = check_glue(nm, n_atoms, hidden, {part_dev},
+                          {a0_dev, b0_dev, a1_dev, b1_dev, part_dev}))
+    return rc;
+  if ((n_atoms > 0 && (!a0_dev || !b0_dev)) || two != (b1_dev != nullptr)) {
+    set_error("%s: need a0 and b0 (n_atoms > 0), and a1 and b1 together", nm);
+    return MLFF_ERR_INVALID;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int A = xyz_atoms(n_atoms), S = xyz_parts(n_atoms);
+#define MLFF_XYZ(T, W)                                                                        \
+  xyz_outer_kernel<T, W><<<S, kGlueBlock, 0, st>>>(n_atoms, hidden, A, a0_dev, b0_dev, a1_dev, \
+                                                   b1_dev, row_w_dev, part_dev)
+  if (two && row_w_dev) MLFF_XYZ(true, true);
+  else if (two) MLFF_XYZ(true, false);
+  else if (row_w_dev) MLFF_XYZ(false, true);
+  else MLFF_XYZ(false, false);
+#undef MLFF_XYZ
   return finish(nm);
 }
 

@@ -495,14 +495,26 @@ def _lin(x, p, name):
     return torch.addmm(p[name + ".bias"], x, p[name + ".weight"].T)
 
 
-def _xyz_outer(a, b, a1=None, b1=None, w=None):
+def _xyz_outer(a, b, a1=None, b1=None, w=None, group: Optional[_WgGroup] = None):
     """sum_n w_n sum_h (a[n,x,h] b[n,z,h] + a1[n,x,h] b1[n,z,h]) -> [3,3]
-    (gradient of a 3x3 mixing matrix): the [3H, 3H] weight-gradient product
-    of the rows [N, 3H], then the sum of the diagonal of each H x H block."""
+    (gradient of a 3x3 mixing matrix), on one partial-sum kernel
+    (mlff_xyz_outer_partials) whose partials the weight-gradient reduce adds:
+    at ``group``'s flush when one is given, else at once."""
     n, _, H = a.shape
-    flat = lambda t: None if t is None else t.reshape(n, 3 * H)
-    full, _ = _wgrad(flat(a), flat(b), flat(a1), flat(b1), w=w)
-    return full.view(3, H, 3, H).diagonal(dim1=1, dim2=3).sum(-1)
+    lib = C.lib()
+    S = int(lib.mlff_xyz_outer_partial_count(n))
+    buf = torch.empty(12 * S + 12, dtype=torch.float32, device=a.device)
+    opt = lambda t: None if t is None else _dense(t)
+    wv = None if w is None else _dense(w.to(torch.float32))
+    C.check("mlff_xyz_outer_partials", lib.mlff_xyz_outer_partials(
+        C.stream_handle(), n, H, C.ptr(_dense(a)), C.ptr(_dense(b)), C.ptr(opt(a1)),
+        C.ptr(opt(b1)), C.ptr(wv), buf.data_ptr()))
+    out = buf[12 * S:]
+    grp = group if group is not None else _WgGroup()
+    grp.add(S, 3, buf.data_ptr(), out.data_ptr(), buf)
+    if group is None:
+        grp.flush()
+    return out[:9].view(3, 3)
 
 
 # ---------------------------------------------------------------------------
@@ -731,9 +743,9 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         _glue("mlff_update_input_adjoint", n, H, M, gx, gs, gv, y, v_m, xin, dgx,
               dgs if T else None, dgv if T else None, lay["dy"] if T else None, dvm, dxin,
               gs_m, gv_m, gvg2, dgs_m, dgv_m, dgvg2)
-        acc(grads, upre + ".mixing_matrix", _xyz_outer(gvg2, v_m, w=w_atom))
+        acc(grads, upre + ".mixing_matrix", _xyz_outer(gvg2, v_m, w=w_atom, group=grp))
         if T:
-            acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m, gvg2, dvm))
+            acc(tg, upre + ".mixing_matrix", _xyz_outer(dgvg2, v_m, gvg2, dvm, group=grp))
         # message reverse (student_model.py:346-387): filter-output gradients per pair
         s, v, W = lay["s"], lay["v"], lay["W"]
         if T:

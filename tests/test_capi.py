@@ -238,6 +238,26 @@ def test_grouped_weight_grad_checks_arguments_on_the_host():
     assert "n_rows" in lib.mlff_last_error().decode()
 
 
+def test_mixing_gradient_checks_arguments_on_the_host():
+    """mlff_xyz_outer_partials refuses bad sizes, missing or unpaired
+    operands and misaligned buffers before any device call; the partial count
+    is at least one and at most 256."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    call = lib.mlff_xyz_outer_partials
+    assert call(None, -1, 128, f, f, None, None, None, f) == -1
+    assert call(None, 10, 6, f, f, None, None, None, f) == -1        # hidden not a multiple of 4
+    assert call(None, 10, 128, None, f, None, None, None, f) == -1
+    assert call(None, 10, 128, f, f, f, None, None, f) == -1         # a1 without b1
+    assert call(None, 10, 128, f, odd, None, None, None, f) == -1
+    assert call(None, 10, 128, f, f, None, None, None, odd) == -1
+    assert lib.mlff_xyz_outer_partial_count(0) == 1
+    assert lib.mlff_xyz_outer_partial_count(2736) == 86
+    assert lib.mlff_xyz_outer_partial_count(10 ** 7) <= 256
+    assert lib.mlff_xyz_outer_partial_count(-1) == 0
+
+
 def test_silu_chain_checks_arguments_on_the_host():
     lib = C.lib()
     f = ctypes.c_void_p(1 << 20)

@@ -80,6 +80,37 @@ def test_grouped_partial_sums_equal_single_launches(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,H", [(2736, 128), (37, 128), (0, 128), (100000, 32), (5000, 64)])
+def test_mixing_matrix_gradient_matches_float64(gpu, n, H):
+    """The 3x3 mixing-matrix gradient (sum over atoms and channels of the xyz
+    outer products, one or two terms, with and without row weights) against
+    float64; bit-identical on a repeat and through a deferred group."""
+    print(f"[case] xyz_outer n={n} H={H}", flush=True)
+    g = torch.Generator().manual_seed(n + H)
+    mk = lambda: torch.randn(n, 3, H, generator=g).to(gpu)
+    a, b, a1, b1 = mk(), mk(), mk(), mk()
+    w = torch.rand(n, generator=g).to(gpu)
+    d = lambda t: t.double().cpu()
+    for two in (False, True):
+        for rw in (None, w):
+            args = (a, b, a1, b1) if two else (a, b)
+            got = train._xyz_outer(*args, w=rw)
+            again = train._xyz_outer(*args, w=rw)
+            grp = train._WgGroup()
+            late = train._xyz_outer(*args, w=rw, group=grp)
+            grp.flush()
+            torch.cuda.synchronize()
+            assert got.shape == (3, 3)
+            assert torch.equal(got, again) and torch.equal(got, late)
+            wd = torch.ones(n, dtype=torch.float64) if rw is None else d(rw)
+            ref = torch.einsum("n,nxh,nzh->xz", wd, d(a), d(b))
+            if two:
+                ref = ref + torch.einsum("n,nxh,nzh->xz", wd, d(a1), d(b1))
+            scale = max(n * H, 1) ** 0.5
+            assert (d(got) - ref).abs().max().item() <= 4e-6 * scale * (2 if two else 1) + 1e-30
+
+
+@pytest.mark.gpu
 def test_weight_grad_on_strided_views(gpu):
     """Column slices of wider rows (row stride above the width) give the
     same bits as their contiguous copies."""
