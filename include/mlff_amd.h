@@ -456,6 +456,24 @@ int mlff_weight_grad_partials(void* stream, int K, int M, int N, const float* a0
 int mlff_weight_grad_reduce_group(void* stream, int n_rows, const int64_t* table_dev,
                                   int n_blocks);
 
+/* Pair geometry and radial basis of the training pass over n_pairs pairs
+ * (lo_dev, hi_dev int32; x_dev [N][3] float32 positions; shift_dev [P][3] the
+ * periodic image shift already times the cell, or NULL), as in
+ * student_model.py:706-722 (edge vectors, distances, unit vectors) and
+ * 249-253, 285-290 (Gaussian basis times cosine cutoff):
+ *   vec = x[hi] - x[lo] + shift, d = |vec|, den = d + eps, u = vec / den,
+ *   e [P][num_rbf] the basis and de = de/dd.
+ * With lam_dev ([N][3], a position direction) it writes the tangent instead:
+ *   dd = vec . (lam[hi] - lam[lo]) / d, du = d(u) along lam, de_t = de dd
+ * (dd_dev [P], du_dev [P][3], de_t_dev [P][num_rbf]; the forward outputs may
+ * then be NULL). */
+int mlff_train_geometry(void* stream, int n_pairs, int num_rbf, const float* x_dev,
+                        const float* lam_dev, const int32_t* lo_dev, const int32_t* hi_dev,
+                        const float* shift_dev, const float* centers_dev,
+                        const float* widths_dev, float cutoff, float eps, float* vec_dev,
+                        float* d_dev, float* den_dev, float* u_dev, float* e_dev,
+                        float* de_dev, float* dd_dev, float* du_dev, float* de_t_dev);
+
 /* SiLU of the MLPs (student_model.py:318-322, 407-411, 599-605) and its
  * derivatives, elementwise over n float32 values (n a multiple of 4, buffers
  * 16-byte aligned), each output skipped when its pointer is NULL:

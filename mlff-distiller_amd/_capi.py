@@ -46,7 +46,7 @@ EXPORTS = (
     "mlff_segment_sum", "mlff_plan_set_tiles", "mlff_edge_message", "mlff_edge_grad",
     "mlff_weight_grad_workspace_bytes", "mlff_weight_grad", "mlff_weight_grad_partial_count",
     "mlff_weight_grad_partials", "mlff_weight_grad_reduce_group", "mlff_xyz_outer_partial_count",
-    "mlff_xyz_outer_partials", "mlff_silu_chain",
+    "mlff_xyz_outer_partials", "mlff_train_geometry", "mlff_silu_chain",
     "mlff_update_norm", "mlff_update_apply", "mlff_update_norm_tangent",
     "mlff_update_apply_tangent", "mlff_update_gate_adjoint", "mlff_update_input_adjoint",
     "mlff_probe_mfma_flops", "mlff_probe_mfma", "mlff_plan_set_weights",
@@ -167,6 +167,9 @@ def _declare(lib):
     lib.mlff_weight_grad_partials.restype = ctypes.c_int
     lib.mlff_weight_grad_partials.argtypes = ([vp] + [ctypes.c_int] * 3 + [vp, ctypes.c_int] * 4
                                               + [vp, ctypes.c_int, vp, ctypes.c_size_t])
+    lib.mlff_train_geometry.restype = ctypes.c_int
+    lib.mlff_train_geometry.argtypes = ([vp, ctypes.c_int, ctypes.c_int] + [vp] * 7
+                                        + [ctypes.c_float, ctypes.c_float] + [vp] * 9)
     lib.mlff_xyz_outer_partial_count.restype = ctypes.c_int
     lib.mlff_xyz_outer_partial_count.argtypes = [ctypes.c_int]
     lib.mlff_xyz_outer_partials.restype = ctypes.c_int

@@ -633,6 +633,74 @@ void launch_weight_grad(hipStream_t st, bool bias, int blocks, int K, int M, int
         part_bias);
 }
 
+// Pair geometry and radial basis of the training pass, one thread per pair
+// p = (lo, hi) (student_model.py:706-722: edge vectors, distances and unit
+// vectors; 249-253 and 285-290: the Gaussian basis times the cosine cutoff):
+//   vec = (x[hi] - x[lo]) + shift,  d = |vec|,  den = d + eps,  u = vec / den,
+//   e_k = phi_k fc,  de_k = (-2 gamma_k (d - mu_k) phi_k) fc + phi_k dfc,
+// phi_k = exp(-gamma_k (d - mu_k)^2), gamma_k = 1 / w_k^2,
+// fc = (cos(pi d / rc) + 1) / 2 and dfc = -(pi / rc) sin(pi d / rc) / 2 for
+// d < rc, both 0 otherwise.  With a position direction lam (TAN) it writes
+// the tangent instead: dvec = lam[hi] - lam[lo], dd = (vec . dvec) / d,
+// du = dvec / den - vec (dd / den^2), de_t = de dd.
+template <bool TAN>
+__global__ __launch_bounds__(kTrainBlock) void pair_geometry_kernel(
+    int P, int R, const float* __restrict__ x, const float* __restrict__ lam,
+    const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+    const float* __restrict__ shift, const float* __restrict__ centers,
+    const float* __restrict__ widths, float rc, float eps, float* __restrict__ vec_out,
+    float* __restrict__ d_out, float* __restrict__ den_out, float* __restrict__ u_out,
+    float* __restrict__ e_out, float* __restrict__ de_out, float* __restrict__ dd_out,
+    float* __restrict__ du_out, float* __restrict__ de_t_out) {
+  const int p = blockIdx.x * kTrainBlock + threadIdx.x;
+  if (p >= P) return;
+  const int a = lo[p], b = hi[p];
+  float v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v[c] = x[3 * b + c] - x[3 * a + c];
+    if (shift) v[c] += shift[3 * p + c];
+  }
+  const float d = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const float den = d + eps;
+  const bool inside = d < rc;
+  const float arg = (float)M_PI * d / rc;
+  const float fc = inside ? 0.5f * (cosf(arg) + 1.0f) : 0.f;
+  const float dfc = inside ? (float)(-0.5 * M_PI / rc) * sinf(arg) : 0.f;
+  float dd = 0.f;
+  if (TAN) {
+    float dv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dv[c] = lam[3 * b + c] - lam[3 * a + c];
+    dd = (v[0] * dv[0] + v[1] * dv[1] + v[2] * dv[2]) / d;
+    dd_out[p] = dd;
+    const float q = dd / (den * den);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) du_out[3 * p + c] = dv[c] / den - v[c] * q;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      vec_out[3 * p + c] = v[c];
+      u_out[3 * p + c] = v[c] / den;
+    }
+    d_out[p] = d;
+    den_out[p] = den;
+  }
+  for (int k = 0; k < R; ++k) {
+    const float w = widths[k];
+    const float gamma = 1.0f / (w * w);
+    const float diff = d - centers[k];
+    const float phi = expf(-gamma * (diff * diff));
+    const float de = (-2.0f * gamma * diff * phi) * fc + phi * dfc;
+    if (TAN) {
+      de_t_out[(size_t)p * R + k] = de * dd;
+    } else {
+      e_out[(size_t)p * R + k] = phi * fc;
+      de_out[(size_t)p * R + k] = de;
+    }
+  }
+}
+
 // SiLU and its derivatives along a forward tangent and a reverse adjoint
 // (student_model.py:318-322, 407-411, 599-605: the SiLU of every MLP),
 // elementwise over n values, four per thread:
@@ -1102,6 +1170,44 @@ int mlff_weight_grad_reduce_group(void* stream, int n_rows, const int64_t* table
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("mlff_weight_grad_reduce_group: %s", hipGetErrorString(e));
+    return MLFF_ERR_HIP;
+  }
+  return MLFF_OK;
+}
+
+int mlff_train_geometry(void* stream, int n_pairs, int num_rbf, const float* x_dev,
+                        const float* lam_dev, const int32_t* lo_dev, const int32_t* hi_dev,
+                        const float* shift_dev, const float* centers_dev,
+                        const float* widths_dev, float cutoff, float eps, float* vec_dev,
+                        float* d_dev, float* den_dev, float* u_dev, float* e_dev,
+                        float* de_dev, float* dd_dev, float* du_dev, float* de_t_dev) {
+  const char* nm = "mlff_train_geometry";
+  const bool tan = lam_dev != nullptr;
+  if (n_pairs < 0 || num_rbf < 1 || !(cutoff > 0.f)) {
+    set_error("%s: need n_pairs >= 0, num_rbf >= 1 and a positive cutoff", nm);
+    return MLFF_ERR_INVALID;
+  }
+  if (n_pairs == 0) return MLFF_OK;
+  if (!x_dev || !lo_dev || !hi_dev || !centers_dev || !widths_dev ||
+      (tan ? (!dd_dev || !du_dev || !de_t_dev)
+           : (!vec_dev || !d_dev || !den_dev || !u_dev || !e_dev || !de_dev))) {
+    set_error("%s: null buffer (the forward writes vec, d, den, u, e and de; the tangent "
+              "dd, du and de_t)", nm);
+    return MLFF_ERR_INVALID;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int grid = (n_pairs + kTrainBlock - 1) / kTrainBlock;
+  if (tan)
+    pair_geometry_kernel<true><<<grid, kTrainBlock, 0, st>>>(
+        n_pairs, num_rbf, x_dev, lam_dev, lo_dev, hi_dev, shift_dev, centers_dev, widths_dev,
+        cutoff, eps, vec_dev, d_dev, den_dev, u_dev, e_dev, de_dev, dd_dev, du_dev, de_t_dev);
+  else
+    pair_geometry_kernel<false><<<grid, kTrainBlock, 0, st>>>(
+        n_pairs, num_rbf, x_dev, lam_dev, lo_dev, hi_dev, shift_dev, centers_dev, widths_dev,
+        cutoff, eps, vec_dev, d_dev, den_dev, u_dev, e_dev, de_dev, dd_dev, du_dev, de_t_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: %s", nm, hipGetErrorString(e));
     return MLFF_ERR_HIP;
   }
   return MLFF_OK;

@@ -376,6 +376,30 @@ def _edge_features(d, centers, widths, rc):
     return phi * fc, (-2.0 * gamma * diff * phi) * fc + phi * dfc
 
 
+def _geometry(g: PairGraph, x: Tensor, centers: Tensor, widths: Tensor, rc: float,
+              lam: Optional[Tensor] = None):
+    """Pair geometry and basis on one HIP launch (mlff_train_geometry):
+    (vec, d, den, u, e, de) with vec = x[hi] - x[lo] (+ shift), d = |vec|,
+    den = d + EPS, u = vec / den, e = _edge_features(d) and de = de/dd; with
+    a position direction ``lam``, the tangent (dd, du, de_t) instead."""
+    P = g.lo32.numel()
+    R = centers.numel()
+    dev = x.device
+    f = lambda *shape: torch.empty(*shape, dtype=torch.float32, device=dev)
+    shift = None if g.shift is None else _dense(g.shift.to(torch.float32))
+    if lam is None:
+        outs = (f(P, 3), f(P), f(P), f(P, 3), f(P, R), f(P, R))
+        ptrs = [C.ptr(t) for t in outs] + [None] * 3
+    else:
+        outs = (f(P), f(P, 3), f(P, R))
+        ptrs = [None] * 6 + [C.ptr(t) for t in outs]
+    C.check("mlff_train_geometry", C.lib().mlff_train_geometry(
+        C.stream_handle(), P, R, C.ptr(_dense(x)), C.ptr(None if lam is None else _dense(lam)),
+        C.ptr(g.lo32), C.ptr(g.hi32), C.ptr(shift), C.ptr(_dense(centers)),
+        C.ptr(_dense(widths)), float(rc), EPS, *ptrs))
+    return outs
+
+
 def _edge_feature_param_derivs(d, centers, widths, rc):
     """de_k/dmu_k, de_k/dw_k and their d-derivatives, each [P, R]
     (learnable_rbf, student_model.py:226-236): with phi = exp(-(d-mu)^2/w^2),
@@ -528,14 +552,8 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
     H = p["embedding.weight"].shape[1]
     L = len({int(k.split(".")[1]) for k in p if k.startswith("interactions.")})
     n = Z.shape[0]
-    x = x.to(torch.float32)
-    vec = x[g.hi] - x[g.lo]
-    if g.shift is not None:
-        vec = vec + g.shift.to(torch.float32)
-    d = torch.linalg.vector_norm(vec, dim=1)
-    den = d + EPS
-    u = (vec / den.unsqueeze(1)).contiguous()
-    e, de = _edge_features(d, p["rbf.centers"], p["rbf.widths"], rc)
+    x = _dense(x.to(torch.float32))
+    vec, d, den, u, e, de = _geometry(g, x, p["rbf.centers"], p["rbf.widths"], rc)
     s = p["embedding.weight"][Z].contiguous()
     v = torch.zeros(n, 3, H, dtype=torch.float32, device=x.device)
     layers = []
@@ -564,7 +582,7 @@ def forward(p: Dict[str, Tensor], Z: Tensor, x: Tensor, g: PairGraph, rc: float,
     h4 = _silu(r2)
     eps = _lin(h4, p, "energy_head.4")[:, 0]
     E = _segment_sum(g.mol_ptr, eps, f64=True)
-    cache = dict(H=H, L=L, n=n, vec=vec, d=d, den=den, u=u, e=e, de=de, layers=layers,
+    cache = dict(H=H, L=L, n=n, x=x, vec=vec, d=d, den=den, u=u, e=e, de=de, layers=layers,
                  sL=s, r1=r1, h3=h3, r2=r2, h4=h4, Z=Z, rc=rc, lam=None)
     if lam is not None:
         tangent(p, cache, g, lam)
@@ -576,11 +594,8 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
     stored in the cache (the message tangents on the HIP row kernel)."""
     H, n = c["H"], c["n"]
     lam32 = lam.to(torch.float32)
-    vec, d, den = c["vec"], c["d"], c["den"]
-    dvec = lam32[g.hi] - lam32[g.lo]
-    dd = (vec * dvec).sum(1) / d
-    du = (dvec / den.unsqueeze(1) - vec * (dd / den ** 2).unsqueeze(1)).contiguous()
-    de_t = c["de"] * dd.unsqueeze(-1)
+    dd, du, de_t = _geometry(g, c["x"], p["rbf.centers"], p["rbf.widths"], c["rc"], lam32)
+    vec = c["vec"]
     ds_ = torch.zeros(n, H, dtype=torch.float32, device=vec.device)
     dv_ = torch.zeros(n, 3, H, dtype=torch.float32, device=vec.device)
     for k, lay in enumerate(c["layers"]):

@@ -258,6 +258,24 @@ def test_mixing_gradient_checks_arguments_on_the_host():
     assert lib.mlff_xyz_outer_partial_count(-1) == 0
 
 
+def test_train_geometry_checks_arguments_on_the_host():
+    """mlff_train_geometry refuses bad sizes and missing outputs (the forward
+    writes six arrays, the tangent three) before any device call; no pairs is
+    a no-op."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    call = lib.mlff_train_geometry
+    fwd = (f, f, f, f, f, f, None, None, None)
+    tan = (None,) * 6 + (f, f, f)
+    assert call(None, -1, 20, f, None, f, f, None, f, f, 5.0, 1e-8, *fwd) == -1
+    assert call(None, 10, 0, f, None, f, f, None, f, f, 5.0, 1e-8, *fwd) == -1
+    assert call(None, 10, 20, f, None, f, f, None, f, f, 0.0, 1e-8, *fwd) == -1
+    assert call(None, 10, 20, f, None, f, f, None, f, f, 5.0, 1e-8, *tan) == -1  # forward, no e
+    assert call(None, 10, 20, f, f, f, f, None, f, f, 5.0, 1e-8, *fwd) == -1     # tangent, no dd
+    assert call(None, 10, 20, None, None, f, f, None, f, f, 5.0, 1e-8, *fwd) == -1
+    assert call(None, 0, 20, f, None, f, f, None, f, f, 5.0, 1e-8, *fwd) == 0
+
+
 def test_silu_chain_checks_arguments_on_the_host():
     lib = C.lib()
     f = ctypes.c_void_p(1 << 20)

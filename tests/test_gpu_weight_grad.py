@@ -111,6 +111,49 @@ def test_mixing_matrix_gradient_matches_float64(gpu, n, H):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("periodic", [False, True])
+def test_pair_geometry_matches_float64(gpu, periodic):
+    """The fused pair geometry and basis (vec, d, den, u, e, de/dd) and its
+    tangent along a position direction (dd, du, de_t) against float64 of the
+    same formulas, with pairs inside and beyond the cutoff and, for the
+    periodic case, an image shift per pair."""
+    print(f"[case] pair geometry periodic={periodic}", flush=True)
+    g = torch.Generator().manual_seed(17 + int(periodic))
+    n, P, R, rc = 400, 5000, 20, 5.0
+    x = (torch.rand(n, 3, generator=g) * 8.0).to(gpu)
+    lam = torch.randn(n, 3, generator=g).to(gpu)
+    lo = torch.randint(0, n - 1, (P,), generator=g)
+    hi = torch.minimum(lo + 1 + torch.randint(0, 30, (P,), generator=g), torch.tensor(n - 1))
+    shift = (torch.randn(P, 3, generator=g) * 2.0).to(gpu) if periodic else None
+    lo, hi = lo.to(gpu), hi.to(gpu)
+    pg = train.PairGraph(None, None, None, lo, hi, lo.to(torch.int32), hi.to(torch.int32),
+                         shift, None, None, 1)
+    centers = torch.linspace(0.0, rc, R).to(gpu)
+    widths = (0.5 + torch.rand(R, generator=g)).to(gpu)
+    vec, d, den, u, e, de = train._geometry(pg, x, centers, widths, rc)
+    dd, du, de_t = train._geometry(pg, x, centers, widths, rc, lam)
+    torch.cuda.synchronize()
+    D = lambda t: t.double().cpu()
+    rvec = D(x)[D(hi).long()] - D(x)[D(lo).long()]
+    if periodic:
+        rvec = rvec + D(shift)
+    rd = rvec.norm(dim=1)
+    rden = rd + train.EPS
+    ru = rvec / rden[:, None]
+    re, rde = train._edge_features(rd, D(centers), D(widths), rc)
+    rdvec = D(lam)[D(hi).long()] - D(lam)[D(lo).long()]
+    rdd = (rvec * rdvec).sum(1) / rd
+    rdu = rdvec / rden[:, None] - rvec * (rdd / rden ** 2)[:, None]
+    rde_t = rde * rdd[:, None]
+    assert bool((rd > rc).any()) and bool((rd < rc).any())
+    for got, ref, name in ((vec, rvec, "vec"), (d, rd, "d"), (den, rden, "den"), (u, ru, "u"),
+                           (e, re, "e"), (de, rde, "de"), (dd, rdd, "dd"), (du, rdu, "du"),
+                           (de_t, rde_t, "de_t")):
+        err = (D(got) - ref).abs().max().item()
+        assert err <= 4e-6 * max(ref.abs().max().item(), 1.0), (name, err)
+
+
+@pytest.mark.gpu
 def test_weight_grad_on_strided_views(gpu):
     """Column slices of wider rows (row stride above the width) give the
     same bits as their contiguous copies."""
