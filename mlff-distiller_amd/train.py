@@ -628,13 +628,24 @@ def tangent(p: Dict[str, Tensor], c: dict, g: PairGraph, lam: Tensor):
 # reverse pass (parameter gradients), optionally with its tangent
 # ---------------------------------------------------------------------------
 
+# element segments of the last element tensor seen: a training loop over one
+# batch passes the same tensor every step (weak reference: a tensor made later
+# at the same address never matches; the version counter catches writes)
+_Z_SEG: Dict[str, tuple] = {}
+
+
 def _z_segments(Z: Tensor, n_rows: int):
     """Atoms ordered by element (stable sort) and the element segments, for
     the fixed-order embedding-gradient sum."""
+    ent = _Z_SEG.get("last")
+    if ent is not None and ent[0]() is Z and ent[1] == (Z._version, n_rows):
+        return ent[2], ent[3]
     zs, order = torch.sort(Z, stable=True)
     # first position of each element in the sorted list (no host sync)
     ptr = torch.searchsorted(zs, torch.arange(n_rows + 1, dtype=zs.dtype, device=Z.device))
-    return ptr.to(torch.int32), order.to(torch.int32)
+    ptr, order = ptr.to(torch.int32), order.to(torch.int32)
+    _Z_SEG["last"] = (weakref.ref(Z), (Z._version, n_rows), ptr, order)
+    return ptr, order
 
 
 def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,

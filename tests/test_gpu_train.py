@@ -318,3 +318,25 @@ def test_trainer_recomputes_after_capacity_overflow(gpu):
     assert g1.keys() == g2.keys()
     for k in g1:
         assert torch.equal(g1[k], g2[k]), k
+
+
+def test_element_segments_follow_the_tensor(gpu):
+    """The element segments of the embedding-gradient sum are reused for the
+    same element tensor while it is unmodified, and remade for a written or
+    a different tensor (with the same values in the same order)."""
+    note("element segments cache")
+    train = pkg_mod("train")
+    Z = torch.tensor([6, 1, 8, 1, 6, 7, 1], device=gpu)
+    p1, o1 = train._z_segments(Z, 10)
+    p2, o2 = train._z_segments(Z, 10)
+    assert p1 is p2 and o1 is o2
+    Z.add_(0)   # a write moves the version counter
+    p3, o3 = train._z_segments(Z, 10)
+    assert p3 is not p1 and torch.equal(p3, p1) and torch.equal(o3, o1)
+    Z2 = Z.clone()
+    Z2[0] = 8
+    p4, o4 = train._z_segments(Z2, 10)
+    zs = Z2.cpu().tolist()
+    want = [sum(1 for z in zs if z < k) for k in range(11)]
+    assert p4.cpu().tolist() == want
+    assert [zs[i] for i in o4.cpu().tolist()] == sorted(zs)
