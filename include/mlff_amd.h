@@ -140,6 +140,21 @@ int mlff_neighbor_list(void* stream, const float* pos_dev, int n_atoms, int n_st
                        int32_t* row_ptr_dev, int32_t* col_dev, int32_t* shift_dev,
                        int32_t edge_capacity, int32_t* counts_dev, void* workspace_dev);
 
+/* The same list for isolated structures (no cell), with its unordered pairs:
+ * pair p = (pair_lo[p] < pair_hi[p]) numbered in CSR order of the upper edges
+ * (row < col), geom_dev [P][4] = (p_hi - p_lo, distance) as float4 (16-byte
+ * aligned), and edge_pair_dev [E] the pair of every directed edge.
+ * counts_dev[MLFF_COUNT_PAIRS] receives the pair count; pairs beyond
+ * pair_capacity are not written (edge_pair then -1) and MLFF_COUNT_OVERFLOW is
+ * set.  The pair list the reference's message passing iterates as directed
+ * edges (student_model.py:706-722), built once for both directions. */
+int mlff_neighbor_pairs(void* stream, const float* pos_dev, int n_atoms, int n_structures,
+                        const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev, float cutoff,
+                        int32_t* row_ptr_dev, int32_t* col_dev, int32_t edge_capacity,
+                        int32_t* edge_pair_dev, int32_t* pair_lo_dev, int32_t* pair_hi_dev,
+                        float* geom_dev, int32_t pair_capacity, int32_t* counts_dev,
+                        void* workspace_dev);
+
 /* ---------------------------------------------------------------------------
  * Plan: host-side descriptor of one batch shape.  Holds no device memory of
  * its own; mlff_plan_bind points it at caller buffers.

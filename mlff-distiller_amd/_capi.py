@@ -37,7 +37,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported", -3: "HIP error",
 # every symbol include/mlff_amd.h declares
 EXPORTS = (
     "mlff_version", "mlff_last_error", "mlff_weights_layout", "mlff_neighbor_workspace_bytes",
-    "mlff_neighbor_list", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
+    "mlff_neighbor_list", "mlff_neighbor_pairs", "mlff_plan_create", "mlff_plan_destroy", "mlff_plan_workspace_bytes",
     "mlff_plan_bind", "mlff_plan_counts", "mlff_energy_forces", "mlff_md_bind",
     "mlff_md_bind_frames", "mlff_md_init_forces", "mlff_md_run", "mlff_md_graph_run",
     "mlff_launch_stage", "mlff_plan_set_blocks", "mlff_plan_set_cell_grid", "mlff_mts_bind", "mlff_mts_init_forces",
@@ -105,6 +105,9 @@ def _declare(lib):
     lib.mlff_neighbor_list.restype = ctypes.c_int
     lib.mlff_neighbor_list.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
                                        ctypes.c_float, vp, vp, vp, i32, vp, vp]
+    lib.mlff_neighbor_pairs.restype = ctypes.c_int
+    lib.mlff_neighbor_pairs.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp,
+                                        ctypes.c_float, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp]
     lib.mlff_plan_create.restype = ctypes.c_int
     lib.mlff_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
     lib.mlff_plan_destroy.restype = ctypes.c_int

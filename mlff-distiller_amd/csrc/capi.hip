@@ -305,6 +305,45 @@ int mlff_neighbor_list(void* stream, const float* pos_dev, int n_atoms, int n_st
   return MLFF_OK;
 }
 
+int mlff_neighbor_pairs(void* stream, const float* pos_dev, int n_atoms, int n_structures,
+                        const int32_t* mol_ptr_dev, const int32_t* atom_mol_dev, float cutoff,
+                        int32_t* row_ptr_dev, int32_t* col_dev, int32_t edge_capacity,
+                        int32_t* edge_pair_dev, int32_t* pair_lo_dev, int32_t* pair_hi_dev,
+                        float* geom_dev, int32_t pair_capacity, int32_t* counts_dev,
+                        void* workspace_dev) {
+  if (n_atoms < 0 || n_structures < 0 || edge_capacity < 0 || pair_capacity < 0 ||
+      !(cutoff > 0.f)) {
+    set_error("mlff_neighbor_pairs: invalid sizes or cutoff");
+    return MLFF_ERR_INVALID;
+  }
+  if (n_atoms > 0 && (!pos_dev || !mol_ptr_dev || !atom_mol_dev || !row_ptr_dev ||
+                      !counts_dev || !workspace_dev ||
+                      (edge_capacity > 0 && (!col_dev || !edge_pair_dev)) ||
+                      (pair_capacity > 0 && (!pair_lo_dev || !pair_hi_dev || !geom_dev)))) {
+    set_error("mlff_neighbor_pairs: null pointer");
+    return MLFF_ERR_INVALID;
+  }
+  if (reinterpret_cast<uintptr_t>(geom_dev) & 15) {
+    set_error("mlff_neighbor_pairs: geom must be 16-byte aligned");
+    return MLFF_ERR_INVALID;
+  }
+  NLArgs a;
+  memset(&a, 0, sizeof(a));
+  int32_t* ws = static_cast<int32_t*>(workspace_dev);
+  a.pos = pos_dev; a.N = n_atoms; a.B = n_structures;
+  a.mol_ptr = mol_ptr_dev; a.atom_mol = atom_mol_dev;
+  a.rc = cutoff;
+  a.deg = ws; a.udeg = ws + n_atoms; a.pair_ptr = ws + 2 * n_atoms;
+  a.row_ptr = row_ptr_dev; a.col = col_dev; a.ecap = edge_capacity;
+  a.edge_pair = edge_pair_dev; a.pair_lo = pair_lo_dev; a.pair_hi = pair_hi_dev;
+  a.geom = reinterpret_cast<float4*>(geom_dev); a.pcap = pair_capacity;
+  a.counts = counts_dev;
+  if (n_atoms == 0) return MLFF_OK;
+  hipError_t e = launch_neighbor_list(a, static_cast<hipStream_t>(stream), true);
+  if (e != hipSuccess) return hip_fail(e, "mlff_neighbor_pairs");
+  return MLFF_OK;
+}
+
 int mlff_plan_create(mlff_plan** out, const mlff_config* cfg) {
   if (!out || !cfg) { set_error("mlff_plan_create: null argument"); return MLFF_ERR_INVALID; }
   *out = nullptr;

@@ -39,6 +39,10 @@ struct CellInfo {
 __device__ __forceinline__ void load_cell(const float* cells, const uint8_t* pbc, int b,
                                           CellInfo& ci, bool& periodic) {
   periodic = false;
+  // an isolated structure's shift vectors (always zero shifts) must read a
+  // defined cell: zeros
+#pragma unroll
+  for (int k = 0; k < 9; ++k) ci.c[k] = 0.f;
   if (cells == nullptr || pbc == nullptr) return;
   ci.per[0] = pbc[3 * b]; ci.per[1] = pbc[3 * b + 1]; ci.per[2] = pbc[3 * b + 2];
   periodic = ci.per[0] | ci.per[1] | ci.per[2];

@@ -364,6 +364,43 @@ def count_edges(positions: torch.Tensor, layout: BatchLayout, cutoff: float) -> 
     return int(counts[C.MLFF_COUNT_EDGES].item())
 
 
+def neighbor_pairs(positions: torch.Tensor, layout: BatchLayout, cutoff: float):
+    """Receiver-CSR neighbour list of isolated structures with its unordered
+    pairs, on the device (mlff_neighbor_pairs).
+
+    Returns (row_ptr [N+1], col [E], edge_pair [E], pair_lo [P], pair_hi [P]),
+    int32, pairs numbered in CSR order of the upper edges; the only wait for
+    the device is the count read that sizes the outputs.
+    """
+    if layout.periodic:
+        raise ValueError("neighbor_pairs covers isolated structures only")
+    lib = C.lib()
+    N = layout.n_atoms
+    dev = positions.device
+    cap = edge_capacity_bound(layout.sizes, False)
+    for _ in range(2):
+        pcap = cap // 2 + 1
+        row_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        col = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        edge_pair = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        pair_lo = torch.empty(pcap, dtype=torch.int32, device=dev)
+        pair_hi = torch.empty(pcap, dtype=torch.int32, device=dev)
+        geom = torch.empty(pcap, 4, dtype=torch.float32, device=dev)
+        counts = torch.zeros(C.MLFF_COUNT_SLOTS, dtype=torch.int32, device=dev)
+        ws = torch.empty(int(lib.mlff_neighbor_workspace_bytes(N)), dtype=torch.uint8, device=dev)
+        C.check("mlff_neighbor_pairs", lib.mlff_neighbor_pairs(
+            C.stream_handle(), C.ptr(positions), N, layout.n_structures, C.ptr(layout.mol_ptr),
+            C.ptr(layout.atom_mol), float(cutoff), C.ptr(row_ptr), C.ptr(col), cap,
+            C.ptr(edge_pair), C.ptr(pair_lo), C.ptr(pair_hi), C.ptr(geom), pcap, C.ptr(counts),
+            C.ptr(ws)))
+        c = counts.cpu().numpy()
+        E, P = int(c[C.MLFF_COUNT_EDGES]), int(c[C.MLFF_COUNT_PAIRS])
+        if E <= cap and P <= pcap:
+            return row_ptr, col[:E], edge_pair[:E], pair_lo[:P], pair_hi[:P]
+        cap = max(E, 2 * P)
+    raise RuntimeError("neighbour list capacity did not converge")
+
+
 def neighbor_list(positions: torch.Tensor, layout: BatchLayout, cutoff: float,
                   edge_capacity: Optional[int] = None, with_shifts: bool = False):
     """Receiver-CSR neighbour list on the device.

@@ -133,27 +133,25 @@ def device_graph(Z: Tensor, x: Tensor, sizes: Sequence[int], rc: float, cells=No
             layout = engine.make_layout(sizes, x.device)
             _GRAPH_LAYOUTS[key] = layout
     pos32 = x.detach().to(torch.float32).contiguous()
-    row_ptr, col, shift = engine.neighbor_list(pos32, layout, rc, with_shifts=periodic)
     batch = layout.atom_mol.long()
-    shift_vec = None
-    if periodic:
-        cell_t = torch.as_tensor(np.asarray(cells, dtype=np.float32).reshape(-1, 3, 3),
-                                 device=x.device)
-        n = row_ptr.numel() - 1
-        deg = (row_ptr[1:] - row_ptr[:-1]).long()
-        row = torch.repeat_interleave(torch.arange(n, device=x.device), deg,
-                                      output_size=col.numel())
-        cells_e = cell_t[batch[row]]
-        sh = shift.to(torch.float32)
-        shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
-                     sh[:, 2:3] * cells_e[:, 2])
-        # a periodic list may hold self-image edges (no upper partner): count on the device
-        return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec,
-                                   mol_ptr=layout.mol_ptr)
-    # molecules: every edge has its reverse and none is a self-edge, so the pair
-    # count is half the edge count (known here without a device read)
-    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), None,
-                               n_pairs=col.numel() // 2, mol_ptr=layout.mol_ptr)
+    if not periodic:
+        # isolated structures: the neighbour-list kernels number the pairs
+        # (upper edges in CSR order) and link every edge to its pair
+        row_ptr, col, edge_pair, lo32, hi32 = engine.neighbor_pairs(pos32, layout, rc)
+        return PairGraph(row_ptr, col, edge_pair, lo32.long(), hi32.long(), lo32, hi32, None,
+                         batch, layout.mol_ptr, len(sizes))
+    row_ptr, col, shift = engine.neighbor_list(pos32, layout, rc, with_shifts=True)
+    cell_t = torch.as_tensor(np.asarray(cells, dtype=np.float32).reshape(-1, 3, 3),
+                             device=x.device)
+    n = row_ptr.numel() - 1
+    deg = (row_ptr[1:] - row_ptr[:-1]).long()
+    row = torch.repeat_interleave(torch.arange(n, device=x.device), deg, output_size=col.numel())
+    cells_e = cell_t[batch[row]]
+    sh = shift.to(torch.float32)
+    shift_vec = (sh[:, 0:1] * cells_e[:, 0] + sh[:, 1:2] * cells_e[:, 1] +
+                 sh[:, 2:3] * cells_e[:, 2])
+    # a periodic list may hold self-image edges (no upper partner): count on the device
+    return pair_graph_from_csr(row_ptr, col, batch, len(sizes), shift_vec, mol_ptr=layout.mol_ptr)
 
 
 # ---------------------------------------------------------------------------

@@ -276,6 +276,30 @@ def test_train_geometry_checks_arguments_on_the_host():
     assert call(None, 0, 20, f, None, f, f, None, f, f, 5.0, 1e-8, *fwd) == 0
 
 
+def test_neighbor_pairs_checks_arguments_on_the_host():
+    """mlff_neighbor_pairs refuses bad sizes, missing outputs and a
+    misaligned pair geometry before any device call; no atoms is a no-op."""
+    lib = C.lib()
+    f = ctypes.c_void_p(1 << 20)
+    odd = ctypes.c_void_p((1 << 20) + 4)
+    call = lib.mlff_neighbor_pairs
+    ok = dict(edge_pair=f, lo=f, hi=f, geom=f)
+
+    def run(n=10, cap=100, pcap=50, rc=5.0, **kw):
+        a = {**ok, **kw}
+        return call(None, f, n, 2, f, f, rc, f, f, cap, a["edge_pair"], a["lo"], a["hi"],
+                    a["geom"], pcap, f, f)
+
+    assert run(n=-1) == -1
+    assert run(rc=0.0) == -1
+    assert run(pcap=-1) == -1
+    assert run(edge_pair=None) == -1
+    assert run(lo=None) == -1
+    assert run(geom=None) == -1
+    assert run(geom=odd) == -1
+    assert run(n=0) == 0
+
+
 def test_silu_chain_checks_arguments_on_the_host():
     lib = C.lib()
     f = ctypes.c_void_p(1 << 20)

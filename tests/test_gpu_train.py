@@ -340,3 +340,25 @@ def test_element_segments_follow_the_tensor(gpu):
     want = [sum(1 for z in zs if z < k) for k in range(11)]
     assert p4.cpu().tolist() == want
     assert [zs[i] for i in o4.cpu().tolist()] == sorted(zs)
+
+
+def test_pair_graph_from_the_neighbour_kernels_equals_the_csr_form(gpu):
+    """For isolated structures the training pair graph comes from the
+    neighbour-list kernels (pairs numbered by the fill pass, edges linked by
+    the link pass); it equals the graph built from the plain receiver CSR by
+    pair_graph_from_csr, field by field."""
+    note("pair graph: neighbour kernels vs CSR form")
+    synth = pkg_mod("synth")
+    train = pkg_mod("train")
+    engine = pkg_mod("engine")
+    Z, P, mp = synth.flatten(synth.molecule_set(24, 5, 70, seed=13))
+    sizes = np.diff(mp)
+    x = torch.tensor(P, dtype=torch.float32, device=gpu)
+    g = train.device_graph(torch.tensor(Z, device=gpu), x, sizes, 5.0)
+    layout = engine.make_layout(sizes, gpu)
+    row_ptr, col, _ = engine.neighbor_list(x, layout, 5.0)
+    ref = train.pair_graph_from_csr(row_ptr, col, layout.atom_mol.long(), len(sizes))
+    for name in ("row_ptr", "col", "edge_pair", "lo", "hi", "lo32", "hi32", "batch", "mol_ptr"):
+        a, b = getattr(g, name), getattr(ref, name)
+        assert a.dtype == b.dtype and torch.equal(a, b), name
+    assert g.shift is None and g.n_structures == ref.n_structures
