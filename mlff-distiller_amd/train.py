@@ -262,6 +262,19 @@ def _silu_call(a, da=None, g=None, dg=None, h=False, dh=False, dga=False):
     return [o for o in outs if o is not None][0]
 
 
+def _silu_pair(x, g, dx, dg=None):
+    """The adjoint g silu'(x) and its tangent dg silu'(x) + g silu''(x) dx
+    (dg None is zero) from one pass, stacked in one [2, *x.shape] buffer, so
+    the products that follow take both as one operand."""
+    x, g, dx = _dense(x), _dense(g), _dense(dx)
+    dg = None if dg is None else _dense(dg)
+    out = torch.empty((2,) + tuple(x.shape), dtype=torch.float32, device=x.device)
+    C.check("mlff_silu_chain", C.lib().mlff_silu_chain(
+        C.stream_handle(), x.numel(), C.ptr(x), C.ptr(dx), C.ptr(g), C.ptr(dg), None, None,
+        C.ptr(out[0]), C.ptr(out[1])))
+    return out
+
+
 def _silu(x):
     """silu(x) = x sigmoid(x) (mlff_silu_chain)."""
     return _silu_call(x, h=True)
@@ -694,31 +707,38 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
     acc(grads, "energy_head.4.weight", (wgeps @ h4).reshape(1, -1))
     acc(grads, "energy_head.4.bias", wgeps.sum().reshape(1))
     gh4 = geps.unsqueeze(1) * Q3
-    gr2 = _silu_d(r2, gh4)
+    R2w, R1w = p["energy_head.2.weight"], p["energy_head.0.weight"]
+    if T:
+        # each adjoint is stacked with its tangent: one product for both
+        dr1, dh3, dr2, dh4 = c["dr1"], c["dh3"], c["dr2"], c["dh4"]
+        GR2 = _silu_pair(r2, gh4, dr2)
+        gr2, dgr2 = GR2[0], GR2[1]
+        GH3 = GR2.view(2 * n, -1) @ R2w
+        gh3, dgh3 = GH3[:n], GH3[n:]
+        GR1 = _silu_pair(r1, gh3, dr1, dgh3)
+        gr1, dgr1 = GR1[0], GR1[1]
+        GS = GR1.view(2 * n, -1) @ R1w
+        gs, dgs = GS[:n], GS[n:]
+    else:
+        gr2 = _silu_d(r2, gh4)
+        gr1 = _silu_d(r1, gr2 @ R2w)
+        gs = (gr1 @ R1w).contiguous()
     gw_, gb_ = _wgrad(gr2, h3, w=w_atom, bias=True, group=grp)
     acc(grads, "energy_head.2.weight", gw_)
     acc(grads, "energy_head.2.bias", gb_)
-    gh3 = gr2 @ p["energy_head.2.weight"]
-    gr1 = _silu_d(r1, gh3)
     gw_, gb_ = _wgrad(gr1, sL, w=w_atom, bias=True, group=grp)
     acc(grads, "energy_head.0.weight", gw_)
     acc(grads, "energy_head.0.bias", gb_)
-    gs = (gr1 @ p["energy_head.0.weight"]).contiguous()
     gv = torch.zeros(n, 3, H, dtype=torch.float32, device=dev)
     if T:
-        dr1, dh3, dr2, dh4 = c["dr1"], c["dh3"], c["dr2"], c["dh4"]
         acc(tg, "energy_head.4.weight", (geps @ dh4).reshape(1, -1))
         acc(tg, "energy_head.4.bias", torch.zeros(1, dtype=torch.float32, device=dev))
-        dgr2 = _silu_dd(r2, gh4, dr2)
         gw_, gb_ = _wgrad(dgr2, h3, gr2, dh3, bias=True, group=grp)
         acc(tg, "energy_head.2.weight", gw_)
         acc(tg, "energy_head.2.bias", gb_)
-        dgh3 = dgr2 @ p["energy_head.2.weight"]
-        dgr1 = _silu_dd(r1, gh3, dr1, dgh3)
         gw_, gb_ = _wgrad(dgr1, sL, gr1, c["dsL"], bias=True, group=grp)
         acc(tg, "energy_head.0.weight", gw_)
         acc(tg, "energy_head.0.bias", gb_)
-        dgs = (dgr1 @ p["energy_head.0.weight"]).contiguous()
         dgv = torch.zeros_like(gv)
 
     u = c["u"]
@@ -733,31 +753,34 @@ def reverse(p: Dict[str, Tensor], c: dict, g: PairGraph, gE: Tensor,
         U2 = p[upre + ".update_mlp.2.weight"]
         M, v_m, Mv, y, xin = lay["M"], lay["v_m"], lay["Mv"], lay["y"], lay["xin"]
         # update reverse (student_model.py:434-470), gates and their tangent
-        gy = torch.empty(n, 3 * H, dtype=torch.float32, device=dev)
-        dgy = torch.empty_like(gy) if T else None
+        # the adjoint and its tangent stacked ([2, n, ...]): one product for both
+        YY = torch.empty(2 if T else 1, n, 3 * H, dtype=torch.float32, device=dev)
+        gy, dgy = YY[0], (YY[1] if T else None)
         dvm, dMv, dxin = (lay["dvm"], lay["dMv"], lay["dxin"]) if T else (None, None, None)
         _glue("mlff_update_gate_adjoint", n, H, gs, gv, v_m, Mv, dgs if T else None,
               dgv if T else None, dvm, dMv, gy, dgy)
+        if T:
+            GH2 = YY.view(2 * n, 3 * H) @ U2
+            GA2 = _silu_pair(lay["a2"], GH2[:n], lay["da2"], GH2[n:])
+            ga2, dga2 = GA2[0], GA2[1]
+            GX = GA2.view(2 * n, -1) @ U1
+            gx, dgx = GX[:n], GX[n:]
+        else:
+            ga2 = _silu_d(lay["a2"], gy @ U2)
+            gx, dgx = ga2 @ U1, None
         gw_, gb_ = _wgrad(gy, lay["h2"], w=w_atom, bias=True, group=grp)
         acc(grads, upre + ".update_mlp.2.weight", gw_)
         acc(grads, upre + ".update_mlp.2.bias", gb_)
-        gh2 = gy @ U2
-        ga2 = _silu_d(lay["a2"], gh2)
         gw_, gb_ = _wgrad(ga2, xin, w=w_atom, bias=True, group=grp)
         acc(grads, upre + ".update_mlp.0.weight", gw_)
         acc(grads, upre + ".update_mlp.0.bias", gb_)
-        gx = ga2 @ U1
-        dgx = None
         if T:
             gw_, gb_ = _wgrad(dgy, lay["h2"], gy, lay["dh2"], bias=True, group=grp)
             acc(tg, upre + ".update_mlp.2.weight", gw_)
             acc(tg, upre + ".update_mlp.2.bias", gb_)
-            dgh2 = dgy @ U2
-            dga2 = _silu_dd(lay["a2"], gh2, lay["da2"], dgh2)
             gw_, gb_ = _wgrad(dga2, xin, ga2, dxin, bias=True, group=grp)
             acc(tg, upre + ".update_mlp.0.weight", gw_)
             acc(tg, upre + ".update_mlp.0.bias", gb_)
-            dgx = dga2 @ U1
         gs_m = torch.empty(n, H, dtype=torch.float32, device=dev)
         gv_m = torch.empty(n, 3, H, dtype=torch.float32, device=dev)
         gvg2 = torch.empty_like(gv_m)
