@@ -138,14 +138,17 @@ __global__ __launch_bounds__(kTfBlock, 4) void train_filter_kernel(
 // and, when there are fewer than 4, the tile's row tiles (RG groups).
 constexpr int kTaBlock = 256;
 
-// mma_tile_k over two A operands (the adjoint and its tangent, same rows) with
-// one B operand: each B fetch, PF slabs ahead, feeds both products.
-template <int RT, int NT, int K, int PF = 2>
+// mma_tile_k over one or (PAIR) two A operands (the adjoint and its tangent,
+// same rows) with one B operand: each B fetch, PF slabs ahead, feeds both
+// products.  hook() runs right after the first PF slabs' B fetches are
+// issued: loads it issues then complete behind those slabs' MFMAs instead of
+// delaying the first B wait (memory loads complete in issue order).
+template <int RT, int NT, int K, bool PAIR, typename Hook, int PF = 2>
 __device__ __forceinline__ void mma_tile_k_pair(const float* __restrict__ A0,
                                                 const float* __restrict__ A1, int lda,
                                                 const float* __restrict__ B, int ldb,
                                                 const int (&cb)[NT], f4 (&acc0)[RT][NT],
-                                                f4 (&acc1)[RT][NT]) {
+                                                f4 (&acc1)[RT][NT], Hook&& hook) {
   static_assert(K % 16 == 0, "K must be a multiple of 16");
   constexpr int S = K / 16;
   const int lane = threadIdx.x & 63;
@@ -161,6 +164,7 @@ __device__ __forceinline__ void mma_tile_k_pair(const float* __restrict__ A0,
 #pragma unroll
     for (int ct = 0; ct < NT; ++ct)
       if (q < S) b[q][ct] = *reinterpret_cast<const f4*>(B + boff[ct] + 16 * q);
+  hook();
 #pragma unroll
   for (int sl = 0; sl < S; ++sl) {
     if (sl + PF < S) {
@@ -172,7 +176,7 @@ __device__ __forceinline__ void mma_tile_k_pair(const float* __restrict__ A0,
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
       a0[rt] = *reinterpret_cast<const f4*>(A0 + aoff + rt * 16 * lda + 16 * sl);
-      a1[rt] = *reinterpret_cast<const f4*>(A1 + aoff + rt * 16 * lda + 16 * sl);
+      if (PAIR) a1[rt] = *reinterpret_cast<const f4*>(A1 + aoff + rt * 16 * lda + 16 * sl);
     }
     const int cur = sl % (PF + 1);
 #pragma unroll
@@ -183,14 +187,15 @@ __device__ __forceinline__ void mma_tile_k_pair(const float* __restrict__ A0,
         for (int ct = 0; ct < NT; ++ct) {
           acc0[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt][s], b[cur][ct][s],
                                                               acc0[rt][ct], 0, 0, 0);
-          acc1[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt][s], b[cur][ct][s],
-                                                              acc1[rt][ct], 0, 0, 0);
+          if (PAIR)
+            acc1[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt][s], b[cur][ct][s],
+                                                                acc1[rt][ct], 0, 0, 0);
         }
   }
 }
 constexpr int kTaWaves = kTaBlock / 64;
 template <int H>
-constexpr int kTaPairs = H >= 128 ? 32 : (H >= 64 ? 64 : 128);
+constexpr int kTaPairs = H >= 128 ? 32 : 64;
 
 template <int H, int TM, bool TAN>
 __global__ __launch_bounds__(kTaBlock, 4) void train_filter_adjoint_kernel(
@@ -212,6 +217,29 @@ __global__ __launch_bounds__(kTaBlock, 4) void train_filter_adjoint_kernel(
   int cb[NTW];
 #pragma unroll
   for (int ct = 0; ct < NTW; ++ct) cb[ct] = (cw * NTW + ct) * 16;
+  // Each thread's share of a chunk (NV float4s per operand) is fetched into
+  // registers while the previous chunk's products run, then stored to LDS
+  // (PRE, the tangent kernels; 136 -> 132 µs at H = 128 on 50,132 pairs).  The
+  // tangent kernel at H = 64 would spill with those registers live across the
+  // products, and the forward-only kernels (one operand, half the products)
+  // gain nothing: those stage each chunk straight from memory.
+  constexpr bool PRE = TAN && H != 64;
+  constexpr int NV = TM * (KC / 4) / kTaBlock;
+  static_assert(NV * kTaBlock == TM * (KC / 4), "chunk split");
+  f4 st_g[NV], st_dg[TAN ? NV : 1];
+  auto fetch = [&](int t, int ch) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int idx = threadIdx.x + v * kTaBlock;
+      const int m = idx / (KC / 4), k4 = idx - m * (KC / 4), p = t * TM + m;
+      const f4 zero = {0.f, 0.f, 0.f, 0.f};
+      const bool on = t < ntiles && p < P;
+      const size_t o = on ? (size_t)p * 3 * H + ch * KC + 4 * k4 : 0;
+      st_g[v] = on ? *reinterpret_cast<const f4*>(gw + o) : zero;
+      if (TAN) st_dg[v] = on ? *reinterpret_cast<const f4*>(dgw + o) : zero;
+    }
+  };
+  if (PRE) fetch(blockIdx.x, 0);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int pbase = tile * TM;
     f4 acc[RTW][NTW], dacc[RTW][NTW];
@@ -220,24 +248,41 @@ __global__ __launch_bounds__(kTaBlock, 4) void train_filter_adjoint_kernel(
 #pragma unroll 1
     for (int ch = 0; ch < 3; ++ch) {
       __syncthreads();   // the previous chunk's products are done with the LDS
-      for (int idx = threadIdx.x; idx < TM * (KC / 4); idx += kTaBlock) {
-        const int m = idx / (KC / 4), k4 = idx - m * (KC / 4), p = pbase + m;
-        const size_t o = (size_t)p * 3 * H + ch * KC + 4 * k4;
-        const f4 zero = {0.f, 0.f, 0.f, 0.f};
-        *reinterpret_cast<f4*>(&a_lds[0][m * LDA + 4 * k4]) =
-            p < P ? *reinterpret_cast<const f4*>(gw + o) : zero;
-        if (TAN)
-          *reinterpret_cast<f4*>(&a_lds[NSTAGE - 1][m * LDA + 4 * k4]) =
-              p < P ? *reinterpret_cast<const f4*>(dgw + o) : zero;
+      if (PRE) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const int idx = threadIdx.x + v * kTaBlock;
+          const int m = idx / (KC / 4), k4 = idx - m * (KC / 4);
+          *reinterpret_cast<f4*>(&a_lds[0][m * LDA + 4 * k4]) = st_g[v];
+          if (TAN) *reinterpret_cast<f4*>(&a_lds[NSTAGE - 1][m * LDA + 4 * k4]) = st_dg[v];
+        }
+      } else {
+        for (int idx = threadIdx.x; idx < TM * (KC / 4); idx += kTaBlock) {
+          const int m = idx / (KC / 4), k4 = idx - m * (KC / 4), p = pbase + m;
+          const size_t o = (size_t)p * 3 * H + ch * KC + 4 * k4;
+          const f4 zero = {0.f, 0.f, 0.f, 0.f};
+          *reinterpret_cast<f4*>(&a_lds[0][m * LDA + 4 * k4]) =
+              p < P ? *reinterpret_cast<const f4*>(gw + o) : zero;
+          if (TAN)
+            *reinterpret_cast<f4*>(&a_lds[NSTAGE - 1][m * LDA + 4 * k4]) =
+                p < P ? *reinterpret_cast<const f4*>(dgw + o) : zero;
+        }
       }
       __syncthreads();
-      if (TAN)
-        mma_tile_k_pair<RTW, NTW, KC>(&a_lds[0][rg * RTW * 16 * LDA],
-                                      &a_lds[NSTAGE - 1][rg * RTW * 16 * LDA], LDA,
-                                      w2t + ch * KC, 3 * H, cb, acc, dacc);
-      else
+      if (TAN) {
+        // the next chunk (this tile's, or the next tile's first), in flight
+        // during this chunk's products
+        const int nt = ch < 2 ? tile : tile + gridDim.x, nch = ch < 2 ? ch + 1 : 0;
+        mma_tile_k_pair<RTW, NTW, KC, true>(&a_lds[0][rg * RTW * 16 * LDA],
+                                            &a_lds[NSTAGE - 1][rg * RTW * 16 * LDA], LDA,
+                                            w2t + ch * KC, 3 * H, cb, acc, dacc,
+                                            [&] {
+                                              if (PRE) fetch(nt, nch);
+                                            });
+      } else {
         mma_tile_k<RTW, NTW, KC>(&a_lds[0][rg * RTW * 16 * LDA], LDA, w2t + ch * KC, 3 * H,
                                  cb, acc);
+      }
     }
     // lane (r, g) holds rows (rg RTW + rt) 16 + 4g + i of columns cb[ct] + r
 #pragma unroll
