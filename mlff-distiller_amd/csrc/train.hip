@@ -30,6 +30,12 @@ constexpr int kTrainBlock = 256;
 // adjoint on 64 molecules); the same form made the forward kernel slower
 // (61 -> 98 µs), which keeps its per-edge loads.
 template <bool TAN> constexpr int kTrainUnroll = 2;
+// lane groups per receiver row in the message kernels: group 0 sums the first
+// half of the row's edges onto the row's own features, group 1 the second
+// half, and group 0 adds group 1's sums (a fixed partition and order).  Twice
+// the waves of one group per row: at training batch sizes (a few thousand
+// rows) one group per row leaves about one wave per SIMD.
+constexpr int kTrainSegs = 2;
 
 __device__ __forceinline__ f4 ld4(const float* p) { return *reinterpret_cast<const f4*>(p); }
 __device__ __forceinline__ void st4(float* p, f4 v) { *reinterpret_cast<f4*>(p) = v; }
@@ -49,21 +55,32 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_kernel(
     const float* __restrict__ ds, const float* __restrict__ dv, const float* __restrict__ dW,
     const float* __restrict__ du, float* __restrict__ s_out, float* __restrict__ v_out,
     float* __restrict__ ds_out, float* __restrict__ dv_out) {
-  constexpr int G = H / 4, RPB = kTrainBlock / G;
-  const int i = blockIdx.x * RPB + threadIdx.x / G;
+  static_assert(kTrainSegs == 2, "two lane groups per row");
+  constexpr int G = H / 4, RPB = kTrainBlock / (G * kTrainSegs);
+  __shared__ f4 part[RPB * G][TAN ? 8 : 4];   // group 1's sums, per (row, lane)
+  const int grp = threadIdx.x / G, seg = grp % kTrainSegs;
+  const int i = blockIdx.x * RPB + grp / kTrainSegs;
   const int c = 4 * (threadIdx.x % G);
-  if (i >= N) return;
-  f4 as = ld4(s + (size_t)i * H + c);
-  f4 av[3];
+  const int slot = (grp / kTrainSegs) * G + threadIdx.x % G;
+  const bool row_ok = i < N;
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  f4 as = zero, av[3] = {zero, zero, zero}, ads = zero, adv[3] = {zero, zero, zero};
+  int e0 = 0, e1 = 0;
+  if (row_ok) {
+    if (seg == 0) {
+      as = ld4(s + (size_t)i * H + c);
 #pragma unroll
-  for (int x = 0; x < 3; ++x) av[x] = ld4(v + ((size_t)i * 3 + x) * H + c);
-  f4 ads = {0.f, 0.f, 0.f, 0.f}, adv[3] = {ads, ads, ads};
-  if (TAN) {
-    ads = ld4(ds + (size_t)i * H + c);
+      for (int x = 0; x < 3; ++x) av[x] = ld4(v + ((size_t)i * 3 + x) * H + c);
+      if (TAN) {
+        ads = ld4(ds + (size_t)i * H + c);
 #pragma unroll
-    for (int x = 0; x < 3; ++x) adv[x] = ld4(dv + ((size_t)i * 3 + x) * H + c);
+        for (int x = 0; x < 3; ++x) adv[x] = ld4(dv + ((size_t)i * 3 + x) * H + c);
+      }
+    }
+    const int r0 = row_ptr[i], r1 = row_ptr[i + 1], mid = r0 + (r1 - r0 + 1) / 2;
+    e0 = seg == 0 ? r0 : mid;
+    e1 = seg == 0 ? mid : r1;
   }
-  const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
   for (int e = e0; e < e1; e += kTrainUnroll<TAN>) {
     int jj[kTrainUnroll<TAN>], pp[kTrainUnroll<TAN>];
 #pragma unroll
@@ -102,10 +119,28 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_kernel(
       }
     }
   }
+  if (seg == 1) {
+    part[slot][0] = as;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) part[slot][1 + x] = av[x];
+    if (TAN) {
+      part[slot][TAN ? 4 : 0] = ads;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) part[slot][TAN ? 5 + x : 0] = adv[x];
+    }
+  }
+  __syncthreads();
+  if (seg != 0 || !row_ok) return;
+  as += part[slot][0];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) av[x] += part[slot][1 + x];
   st4(s_out + (size_t)i * H + c, as);
 #pragma unroll
   for (int x = 0; x < 3; ++x) st4(v_out + ((size_t)i * 3 + x) * H + c, av[x]);
   if (TAN) {
+    ads += part[slot][TAN ? 4 : 0];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) adv[x] += part[slot][TAN ? 5 + x : 0];
     st4(ds_out + (size_t)i * H + c, ads);
 #pragma unroll
     for (int x = 0; x < 3; ++x) st4(dv_out + ((size_t)i * 3 + x) * H + c, adv[x]);
@@ -124,21 +159,32 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_adjoint_kernel(
     const float* __restrict__ gv, const float* __restrict__ W, const float* __restrict__ dgs,
     const float* __restrict__ dgv, const float* __restrict__ dW, float* __restrict__ gs_out,
     float* __restrict__ gv_out, float* __restrict__ dgs_out, float* __restrict__ dgv_out) {
-  constexpr int G = H / 4, RPB = kTrainBlock / G;
-  const int i = blockIdx.x * RPB + threadIdx.x / G;
+  static_assert(kTrainSegs == 2, "two lane groups per row");
+  constexpr int G = H / 4, RPB = kTrainBlock / (G * kTrainSegs);
+  __shared__ f4 part[RPB * G][TAN ? 8 : 4];   // group 1's sums, per (row, lane)
+  const int grp = threadIdx.x / G, seg = grp % kTrainSegs;
+  const int i = blockIdx.x * RPB + grp / kTrainSegs;
   const int c = 4 * (threadIdx.x % G);
-  if (i >= N) return;
-  f4 as = ld4(gs + (size_t)i * H + c);
-  f4 av[3];
+  const int slot = (grp / kTrainSegs) * G + threadIdx.x % G;
+  const bool row_ok = i < N;
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  f4 as = zero, av[3] = {zero, zero, zero}, ads = zero, adv[3] = {zero, zero, zero};
+  int e0 = 0, e1 = 0;
+  if (row_ok) {
+    if (seg == 0) {
+      as = ld4(gs + (size_t)i * H + c);
 #pragma unroll
-  for (int x = 0; x < 3; ++x) av[x] = ld4(gv + ((size_t)i * 3 + x) * H + c);
-  f4 ads = {0.f, 0.f, 0.f, 0.f}, adv[3] = {ads, ads, ads};
-  if (TAN) {
-    ads = ld4(dgs + (size_t)i * H + c);
+      for (int x = 0; x < 3; ++x) av[x] = ld4(gv + ((size_t)i * 3 + x) * H + c);
+      if (TAN) {
+        ads = ld4(dgs + (size_t)i * H + c);
 #pragma unroll
-    for (int x = 0; x < 3; ++x) adv[x] = ld4(dgv + ((size_t)i * 3 + x) * H + c);
+        for (int x = 0; x < 3; ++x) adv[x] = ld4(dgv + ((size_t)i * 3 + x) * H + c);
+      }
+    }
+    const int r0 = row_ptr[i], r1 = row_ptr[i + 1], mid = r0 + (r1 - r0 + 1) / 2;
+    e0 = seg == 0 ? r0 : mid;
+    e1 = seg == 0 ? mid : r1;
   }
-  const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
   constexpr int U = kTrainUnroll<TAN>;
   constexpr int UT = TAN ? U : 1;
   for (int e = e0; e < e1; e += U) {
@@ -182,10 +228,28 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_adjoint_kernel(
       }
     }
   }
+  if (seg == 1) {
+    part[slot][0] = as;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) part[slot][1 + x] = av[x];
+    if (TAN) {
+      part[slot][TAN ? 4 : 0] = ads;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) part[slot][TAN ? 5 + x : 0] = adv[x];
+    }
+  }
+  __syncthreads();
+  if (seg != 0 || !row_ok) return;
+  as += part[slot][0];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) av[x] += part[slot][1 + x];
   st4(gs_out + (size_t)i * H + c, as);
 #pragma unroll
   for (int x = 0; x < 3; ++x) st4(gv_out + ((size_t)i * 3 + x) * H + c, av[x]);
   if (TAN) {
+    ads += part[slot][TAN ? 4 : 0];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) adv[x] += part[slot][TAN ? 5 + x : 0];
     st4(dgs_out + (size_t)i * H + c, ads);
 #pragma unroll
     for (int x = 0; x < 3; ++x) st4(dgv_out + ((size_t)i * 3 + x) * H + c, adv[x]);
@@ -764,7 +828,7 @@ hipError_t message_impl(hipStream_t st, int N, const int32_t* row_ptr, const int
                         const float* u, const float* ds, const float* dv, const float* dW,
                         const float* du, float* s_out, float* v_out, float* ds_out,
                         float* dv_out) {
-  constexpr int RPB = kTrainBlock / (H / 4);
+  constexpr int RPB = kTrainBlock / (H / 4 * kTrainSegs);
   const int grid = (N + RPB - 1) / RPB;
   if (ds)
     train_message_kernel<H, true><<<grid, kTrainBlock, 0, st>>>(
@@ -780,7 +844,7 @@ hipError_t adjoint_impl(hipStream_t st, int N, const int32_t* row_ptr, const int
                         const int32_t* edge_pair, const float* gs, const float* gv, const float* W,
                         const float* dgs, const float* dgv, const float* dW, float* gs_out,
                         float* gv_out, float* dgs_out, float* dgv_out) {
-  constexpr int RPB = kTrainBlock / (H / 4);
+  constexpr int RPB = kTrainBlock / (H / 4 * kTrainSegs);
   const int grid = (N + RPB - 1) / RPB;
   if (dgs)
     train_message_adjoint_kernel<H, true><<<grid, kTrainBlock, 0, st>>>(
