@@ -30,12 +30,12 @@ constexpr int kTrainBlock = 256;
 // adjoint on 64 molecules); the same form made the forward kernel slower
 // (61 -> 98 µs), which keeps its per-edge loads.
 template <bool TAN> constexpr int kTrainUnroll = 2;
-// lane groups per receiver row in the message kernels: group 0 sums the first
-// half of the row's edges onto the row's own features, group 1 the second
-// half, and group 0 adds group 1's sums (a fixed partition and order).  Twice
-// the waves of one group per row: at training batch sizes (a few thousand
-// rows) one group per row leaves about one wave per SIMD.
-constexpr int kTrainSegs = 2;
+// lane groups per receiver row in the message kernels: group k sums the k-th
+// of kTrainSegs consecutive parts of the row's edges (group 0 onto the row's
+// own features), and group 0 adds the other groups' sums in group order (a
+// fixed partition and order).  At training batch sizes (a few thousand rows)
+// one group per row leaves about one wave per SIMD.
+constexpr int kTrainSegs = 4;
 
 __device__ __forceinline__ f4 ld4(const float* p) { return *reinterpret_cast<const f4*>(p); }
 __device__ __forceinline__ void st4(float* p, f4 v) { *reinterpret_cast<f4*>(p) = v; }
@@ -55,9 +55,8 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_kernel(
     const float* __restrict__ ds, const float* __restrict__ dv, const float* __restrict__ dW,
     const float* __restrict__ du, float* __restrict__ s_out, float* __restrict__ v_out,
     float* __restrict__ ds_out, float* __restrict__ dv_out) {
-  static_assert(kTrainSegs == 2, "two lane groups per row");
   constexpr int G = H / 4, RPB = kTrainBlock / (G * kTrainSegs);
-  __shared__ f4 part[RPB * G][TAN ? 8 : 4];   // group 1's sums, per (row, lane)
+  __shared__ f4 part[kTrainSegs - 1][RPB * G][TAN ? 8 : 4];   // groups 1.. sums, per (row, lane)
   const int grp = threadIdx.x / G, seg = grp % kTrainSegs;
   const int i = blockIdx.x * RPB + grp / kTrainSegs;
   const int c = 4 * (threadIdx.x % G);
@@ -77,9 +76,9 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_kernel(
         for (int x = 0; x < 3; ++x) adv[x] = ld4(dv + ((size_t)i * 3 + x) * H + c);
       }
     }
-    const int r0 = row_ptr[i], r1 = row_ptr[i + 1], mid = r0 + (r1 - r0 + 1) / 2;
-    e0 = seg == 0 ? r0 : mid;
-    e1 = seg == 0 ? mid : r1;
+    const int r0 = row_ptr[i], len = row_ptr[i + 1] - r0;
+    e0 = r0 + len * seg / kTrainSegs;
+    e1 = r0 + len * (seg + 1) / kTrainSegs;
   }
   for (int e = e0; e < e1; e += kTrainUnroll<TAN>) {
     int jj[kTrainUnroll<TAN>], pp[kTrainUnroll<TAN>];
@@ -119,28 +118,34 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_kernel(
       }
     }
   }
-  if (seg == 1) {
-    part[slot][0] = as;
+  if (seg > 0) {
+    f4* q = part[seg - 1][slot];
+    q[0] = as;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) part[slot][1 + x] = av[x];
+    for (int x = 0; x < 3; ++x) q[1 + x] = av[x];
     if (TAN) {
-      part[slot][TAN ? 4 : 0] = ads;
+      q[TAN ? 4 : 0] = ads;
 #pragma unroll
-      for (int x = 0; x < 3; ++x) part[slot][TAN ? 5 + x : 0] = adv[x];
+      for (int x = 0; x < 3; ++x) q[TAN ? 5 + x : 0] = adv[x];
     }
   }
   __syncthreads();
   if (seg != 0 || !row_ok) return;
-  as += part[slot][0];
 #pragma unroll
-  for (int x = 0; x < 3; ++x) av[x] += part[slot][1 + x];
+  for (int k = 0; k < kTrainSegs - 1; ++k) {
+    as += part[k][slot][0];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) av[x] += part[k][slot][1 + x];
+    if (TAN) {
+      ads += part[k][slot][TAN ? 4 : 0];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) adv[x] += part[k][slot][TAN ? 5 + x : 0];
+    }
+  }
   st4(s_out + (size_t)i * H + c, as);
 #pragma unroll
   for (int x = 0; x < 3; ++x) st4(v_out + ((size_t)i * 3 + x) * H + c, av[x]);
   if (TAN) {
-    ads += part[slot][TAN ? 4 : 0];
-#pragma unroll
-    for (int x = 0; x < 3; ++x) adv[x] += part[slot][TAN ? 5 + x : 0];
     st4(ds_out + (size_t)i * H + c, ads);
 #pragma unroll
     for (int x = 0; x < 3; ++x) st4(dv_out + ((size_t)i * 3 + x) * H + c, adv[x]);
@@ -159,9 +164,8 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_adjoint_kernel(
     const float* __restrict__ gv, const float* __restrict__ W, const float* __restrict__ dgs,
     const float* __restrict__ dgv, const float* __restrict__ dW, float* __restrict__ gs_out,
     float* __restrict__ gv_out, float* __restrict__ dgs_out, float* __restrict__ dgv_out) {
-  static_assert(kTrainSegs == 2, "two lane groups per row");
   constexpr int G = H / 4, RPB = kTrainBlock / (G * kTrainSegs);
-  __shared__ f4 part[RPB * G][TAN ? 8 : 4];   // group 1's sums, per (row, lane)
+  __shared__ f4 part[kTrainSegs - 1][RPB * G][TAN ? 8 : 4];   // groups 1.. sums, per (row, lane)
   const int grp = threadIdx.x / G, seg = grp % kTrainSegs;
   const int i = blockIdx.x * RPB + grp / kTrainSegs;
   const int c = 4 * (threadIdx.x % G);
@@ -181,9 +185,9 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_adjoint_kernel(
         for (int x = 0; x < 3; ++x) adv[x] = ld4(dgv + ((size_t)i * 3 + x) * H + c);
       }
     }
-    const int r0 = row_ptr[i], r1 = row_ptr[i + 1], mid = r0 + (r1 - r0 + 1) / 2;
-    e0 = seg == 0 ? r0 : mid;
-    e1 = seg == 0 ? mid : r1;
+    const int r0 = row_ptr[i], len = row_ptr[i + 1] - r0;
+    e0 = r0 + len * seg / kTrainSegs;
+    e1 = r0 + len * (seg + 1) / kTrainSegs;
   }
   constexpr int U = kTrainUnroll<TAN>;
   constexpr int UT = TAN ? U : 1;
@@ -228,28 +232,34 @@ __global__ __launch_bounds__(kTrainBlock) void train_message_adjoint_kernel(
       }
     }
   }
-  if (seg == 1) {
-    part[slot][0] = as;
+  if (seg > 0) {
+    f4* q = part[seg - 1][slot];
+    q[0] = as;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) part[slot][1 + x] = av[x];
+    for (int x = 0; x < 3; ++x) q[1 + x] = av[x];
     if (TAN) {
-      part[slot][TAN ? 4 : 0] = ads;
+      q[TAN ? 4 : 0] = ads;
 #pragma unroll
-      for (int x = 0; x < 3; ++x) part[slot][TAN ? 5 + x : 0] = adv[x];
+      for (int x = 0; x < 3; ++x) q[TAN ? 5 + x : 0] = adv[x];
     }
   }
   __syncthreads();
   if (seg != 0 || !row_ok) return;
-  as += part[slot][0];
 #pragma unroll
-  for (int x = 0; x < 3; ++x) av[x] += part[slot][1 + x];
+  for (int k = 0; k < kTrainSegs - 1; ++k) {
+    as += part[k][slot][0];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) av[x] += part[k][slot][1 + x];
+    if (TAN) {
+      ads += part[k][slot][TAN ? 4 : 0];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) adv[x] += part[k][slot][TAN ? 5 + x : 0];
+    }
+  }
   st4(gs_out + (size_t)i * H + c, as);
 #pragma unroll
   for (int x = 0; x < 3; ++x) st4(gv_out + ((size_t)i * 3 + x) * H + c, av[x]);
   if (TAN) {
-    ads += part[slot][TAN ? 4 : 0];
-#pragma unroll
-    for (int x = 0; x < 3; ++x) adv[x] += part[slot][TAN ? 5 + x : 0];
     st4(dgs_out + (size_t)i * H + c, ads);
 #pragma unroll
     for (int x = 0; x < 3; ++x) st4(dgv_out + ((size_t)i * 3 + x) * H + c, adv[x]);
