@@ -35,7 +35,7 @@ template <bool TAN> constexpr int kTrainUnroll = 2;
 // own features), and group 0 adds the other groups' sums in group order (a
 // fixed partition and order).  At training batch sizes (a few thousand rows)
 // one group per row leaves about one wave per SIMD.
-constexpr int kTrainSegs = 4;
+constexpr int kTrainSegs = 8;
 
 __device__ __forceinline__ f4 ld4(const float* p) { return *reinterpret_cast<const f4*>(p); }
 __device__ __forceinline__ void st4(float* p, f4 v) { *reinterpret_cast<f4*>(p) = v; }
